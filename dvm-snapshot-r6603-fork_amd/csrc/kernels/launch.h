@@ -1,0 +1,50 @@
+// Launchers of the runtime's own HIP kernels (csrc/kernels/*.hip).  Host code
+// of the runtime calls only these; every launcher enqueues on the given stream
+// and returns without synchronising.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+// Fill count records of valueBytes bytes each with the same record
+// (rts/dvmh/cuda_reduction.cu:21-82, performReplicate*).
+void dvmh_hip_fill(void *dst, const void *value, long valueBytes, long count, hipStream_t s);
+
+// Fold n partial values (and their loc records) into one result.
+// func: rf_* code; rtType: rt_* element type.  Deterministic: fixed per-thread
+// order and fixed tree; on equal MAX/MIN values the lower partial index wins.
+// (replaces rts/dvmh/cuda_reduction.cu:168-312: no per-warp D2H and host fold)
+void dvmh_hip_reduce_finish(int func, int rtType, long n, const void *vals, const void *locs, long locSize,
+        void *outVal, void *outLoc, hipStream_t s);
+
+// Same for many independent segments: segment g is [offsets[g], offsets[g+1])
+// (offsets in device memory), result g goes to outVals[g] / outLocs[g].
+void dvmh_hip_reduce_segments(int func, int rtType, long groups, const long *offsets, const void *vals,
+        const void *locs, long locSize, void *outVals, void *outLocs, hipStream_t s);
+
+// Axis permutation copy.  src is stored with storage-position sizes srcSizes[0..rank-1]
+// (position 0 outermost); dst storage position p holds src storage position
+// dstFromSrc[p].  (rts/dvmh/cuda_transform.cu:273-652 semantics)
+void dvmh_hip_permute(const void *src, void *dst, long typeSize, int rank, const long *srcSizes,
+        const int *dstFromSrc, hipStream_t s);
+
+// Natural <-> diagonal layout of Rz planes of Ry x Rx elements (x fastest).
+// back = 0: natural -> diagonal; back = 1: diagonal -> natural.
+// (rts/dvmh/cuda_transform.cu:26-73 semantics; index map of dvmhlib_device.h:56-98)
+void dvmh_hip_diagonal(const void *src, void *dst, long typeSize, long Rx, long Ry, long Rz, int back, int slash,
+        hipStream_t s);
+
+// Strided sub-block copy (copyCutting semantics, rts/dvmh/cuda_copying.cu:46-303):
+// copies the block [lo[a], hi[a]] of every axis a from an array with sizes
+// srcSizes / origin srcOrigin to one with sizes dstSizes / origin dstOrigin
+// (both row-major, natural layout).  A contiguous dst (dstSizes == block
+// extents, dstOrigin == lo) is a pack; a contiguous src is an unpack.
+void dvmh_hip_copy_block(const void *src, const long *srcSizes, const long *srcOrigin, void *dst, const long *dstSizes,
+        const long *dstOrigin, const long *lo, const long *hi, int rank, long typeSize, hipStream_t s);
+
+#ifdef __cplusplus
+}
+#endif

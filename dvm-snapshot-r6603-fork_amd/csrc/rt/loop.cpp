@@ -1,0 +1,275 @@
+// Loop execution: handler trampoline, dvmh_loop_perform_C's sequence, GPU
+// reductions and their (optionally deferred) posting to host variables.
+#include <cstring>
+#include <memory>
+#include <utility>
+
+#include "rt.h"
+#include "../kernels/launch.h"
+
+namespace dvmh {
+
+// ------------------------------------------------------------ trampoline
+// The runtime calls a registered handler as f(pLoopRef, params...)
+// (rts/dvmh/util.cpp:659-720).  Every parameter is pointer sized, so the call
+// goes through a non-variadic pointer type of the right arity.
+namespace {
+template <size_t... I>
+void callWith(void *f, DvmType *ref, void *const *a, std::index_sequence<I...>) {
+    using Fn = void (*)(DvmType *, decltype((void)I, (void *)nullptr)...);
+    ((Fn)f)(ref, a[I]...);
+}
+template <size_t N>
+void callN(void *f, DvmType *ref, void *const *a) { callWith(f, ref, a, std::make_index_sequence<N>()); }
+
+template <size_t... N>
+constexpr auto makeTable(std::index_sequence<N...>) {
+    return std::array<void (*)(void *, DvmType *, void *const *), sizeof...(N)>{&callN<N>...};
+}
+}  // namespace
+}  // namespace dvmh
+
+#include <array>
+
+namespace dvmh {
+
+void callHandler(const HandlerDesc &h, DvmType *pLoopRef, const std::vector<void *> *override) {
+    static const auto table = makeTable(std::make_index_sequence<MAX_PARAMS + 1>());
+    const std::vector<void *> &p = override ? *override : h.params;
+    DVMH_CHECK(p.size() <= (size_t)MAX_PARAMS, "Handler with %zu parameters is not supported (max %d)", p.size(), MAX_PARAMS);
+    table[p.size()]((void *)h.func, pLoopRef, p.data());
+}
+
+// ------------------------------------------------------------ loop helpers
+Loop::~Loop() { magic = 0; }
+
+long Loop::iterations() const {
+    long n = 1;
+    for (int i = 0; i < rank; i++) {
+        long c = step[i] > 0 ? (hi[i] - lo[i]) / step[i] + 1 : (lo[i] - hi[i]) / (-step[i]) + 1;
+        if (c <= 0) return 0;
+        n *= c;
+    }
+    return n;
+}
+
+// Alignment of an array to the object a loop is mapped on: follow Data::align
+// links (see abi.cpp) to compose array axis -> template axis.
+extern bool alignToRoot(const Data *d, const Object *root, long tAxisOfArrayAxis[], long sign[]);
+
+// Array axis (0-based index) -> signed 1-based loop axis, 0 if none.
+std::vector<long> Loop::correspondence(const Data *d) const {
+    std::vector<long> res(d->rank, 0);
+    for (auto &c : correspondences)
+        if (c.first == d) return c.second;
+    const Object *root = mapData ? (const Object *)mapData : (const Object *)mapTempl;
+    if (root) {
+        long tAxis[MAX_RANK], sgn[MAX_RANK];
+        if (alignToRoot(d, root, tAxis, sgn)) {
+            for (int a = 0; a < d->rank; a++) {
+                long t = tAxis[a];
+                if (t <= 0) continue;
+                long la = mapAxis[t - 1];
+                if (la > 0) res[a] = (mapMult[t - 1] * sgn[a] < 0) ? -la : la;
+            }
+            return res;
+        }
+    }
+    // unmapped loop: identity on the common leading axes
+    for (int a = 0; a < d->rank && a < rank; a++) res[a] = a + 1;
+    return res;
+}
+
+// DvmhLoop::renewDependencyMask (rts/dvmh/loop.cpp:1117-1150): bit (rank - loopAxis)
+// for every loop axis tied to an array axis with a non-zero ACROSS width.
+void Loop::computeDependencyMask() {
+    depMask = 0;
+    long firstDep = -1;
+    for (auto &ac : acrosses) {
+        std::vector<long> corr = correspondence(ac.data);
+        for (int j = 0; j < ac.data->rank; j++) {
+            if (ac.low[j] > 0 || ac.high[j] > 0) {
+                long la = corr[j] < 0 ? -corr[j] : corr[j];
+                DVMH_CHECK(la > 0, "ACROSS array has dependency on dimension %d, which can not be tied with the parallel loop", j + 1);
+                if (firstDep == -1 || la < firstDep) firstDep = la;
+                long cnt = step[la - 1] > 0 ? (hi[la - 1] - lo[la - 1]) / step[la - 1] + 1 : (lo[la - 1] - hi[la - 1]) / (-step[la - 1]) + 1;
+                if (cnt > 1) depMask |= 1L << (rank - la);
+            }
+        }
+    }
+    if (firstDep > 0 && depMask == 0) depMask |= 1L << (rank - firstDep);
+}
+
+// ------------------------------------------------------------ deferred posts
+// A GPU reduction result lands in pinned host memory; it is merged into the
+// program's variable at the next synchronisation point: right after the loop
+// by default, or when results are needed with DVMH_AMD_SYNC=1.
+struct PendingPost {
+    std::shared_ptr<Reduction> red;
+    char *val;
+    char *loc;
+    bool addInitial;
+};
+static std::vector<PendingPost> pending;
+
+struct PinnedPool {
+    std::vector<char *> blocks;
+    size_t used = 0, blockSize = 1 << 20;
+    char *take(size_t bytes) {
+        bytes = (bytes + 63) & ~(size_t)63;
+        size_t b = used / blockSize, off = used % blockSize;
+        if (off + bytes > blockSize) { b++; off = 0; used = b * blockSize; }
+        while (b >= blocks.size()) {
+            void *p = nullptr;
+            HIP_CHECK(hipHostMalloc(&p, blockSize, hipHostMallocDefault));
+            blocks.push_back((char *)p);
+        }
+        used += bytes;
+        return blocks[b] + off;
+    }
+    void reset() { used = 0; }
+};
+static PinnedPool pinnedPool;
+
+char *pinnedTake(size_t bytes) { return pinnedPool.take(bytes); }
+
+void flushDeferredPosts() {
+    if (pending.empty()) return;
+    gpu.sync();
+    for (auto &p : pending) {
+        p.red->postValues(p.val, p.loc);
+        if (p.addInitial) p.red->postValues(p.red->initArr.data(), p.red->isLoc() ? p.red->initLoc.data() : nullptr);
+    }
+    pending.clear();
+    pinnedPool.reset();
+}
+
+void addPendingPost(const std::shared_ptr<Reduction> &r, char *val, char *loc) {
+    pending.push_back(PendingPost{r, val, loc, false});
+}
+
+// ------------------------------------------------------------ GPU reductions
+// DvmhReductionCuda::prepare (rts/dvmh/loop.cpp:274-297): `count` partial slots,
+// optionally pre-filled with the handler-local initial value.
+void gpuRedPrepare(SpecLoop *sl, Reduction *r, long count, bool fill) {
+    DVMH_CHECK(count > 0, "Reduction partial count must be positive");
+    DVMH_CHECK(r->gpuArr == nullptr, "dvmh_loop_cuda_red_prepare_C called twice for one reduction");
+    r->items = count;
+    r->gpuArr = gpu.alloc((size_t)(r->elemCount * r->elemSize * count));
+    if (r->gpuArrPtr) *r->gpuArrPtr = r->gpuArr;
+    if (r->isLoc()) {
+        r->gpuLoc = gpu.alloc((size_t)(r->elemCount * r->locSize * count));
+        if (r->gpuLocPtr) *r->gpuLocPtr = r->gpuLoc;
+    }
+    if (fill) {
+        dvmh_hip_fill(r->gpuArr, r->localArr.data(), r->elemCount * r->elemSize, count, sl->stream);
+        if (r->isLoc()) dvmh_hip_fill(r->gpuLoc, r->localLoc.data(), r->elemCount * r->locSize, count, sl->stream);
+    }
+}
+
+// DvmhReductionCuda::finish (rts/dvmh/loop.cpp:301-408): fold the partials on
+// the device, bring back one record, merge it into the variable.
+void gpuRedFinish(SpecLoop *sl, const std::shared_ptr<Reduction> &r) {
+    DVMH_CHECK(r->gpuArr != nullptr, "dvmh_loop_cuda_red_finish_C without dvmh_loop_cuda_red_prepare_C");
+    DVMH_CHECK(r->elemCount == 1, "Reduction arrays are not supported on the GPU");
+    void *dRes = gpu.alloc((size_t)(r->elemSize + (r->isLoc() ? r->locSize : 0)));
+    void *dLoc = r->isLoc() ? (char *)dRes + r->elemSize : nullptr;
+    dvmh_hip_reduce_finish(r->func, r->rtType, r->items, r->gpuArr, r->gpuLoc, r->locSize, dRes, dLoc, sl->stream);
+    size_t recBytes = (size_t)(r->elemSize + (r->isLoc() ? r->locSize : 0));
+    char *host = pinnedPool.take(recBytes);
+    HIP_CHECK(hipMemcpyAsync(host, dRes, recBytes, hipMemcpyDeviceToHost, sl->stream));
+    gpu.release(dRes);
+    gpu.release(r->gpuArr);
+    r->gpuArr = nullptr;
+    if (r->gpuLoc) { gpu.release(r->gpuLoc); r->gpuLoc = nullptr; }
+    if (r->gpuArrPtr) *r->gpuArrPtr = nullptr;
+    if (r->gpuLocPtr) *r->gpuLocPtr = nullptr;
+    addPendingPost(r, host, r->isLoc() ? host + r->elemSize : nullptr);
+}
+
+// ------------------------------------------------------------ layout policy
+// Arrays keep a transformed device layout between loops (like the reference's
+// persistent transform state); loops whose handlers index only permuted
+// layouts get the diagonal undone, PARALLEL loops get the natural layout back.
+static void prepareLayouts(Loop *loop) {
+    if (!loop->region) return;
+    int depCount = 0;
+    for (long m = loop->depMask; m; m &= m - 1) depCount++;
+    for (auto &ra : loop->region->arrays) {
+        Data *d = ra.data;
+        if (!d->hasDev || !d->dev.isTransformed(d->rank)) continue;
+        if (depCount >= 2) continue;
+        if (depCount == 1) {
+            if (d->dev.diagState) d->setLayout(d->dev.axisPerm, 0);
+        } else {
+            d->setLayout(nullptr, 0);
+        }
+    }
+}
+
+// ------------------------------------------------------------ perform
+
+// After the handler: allreduce is trivial for one process; then the
+// variable's pre-loop value is folded in (rts/dvmh/dvmh2.cpp:3947-3953).  A GPU
+// result still in flight gets the initial value folded in after it is posted.
+static void finishLoopReductions(Loop *loop) {
+    for (auto &r : loop->reds) {
+        bool inFlight = false;
+        for (auto it = pending.rbegin(); it != pending.rend(); ++it)
+            if (it->red == r) { it->addInitial = true; inFlight = true; break; }
+        if (!inFlight) r->postValues(r->initArr.data(), r->isLoc() ? r->initLoc.data() : nullptr);
+    }
+}
+
+void performLoop(Loop *loop) {
+    // rts/dvmh/dvmh2.cpp:3769-3969
+    for (auto &r : loop->reds) r->initValues(r->arrayAddr, r->locAddr);
+    loop->computeDependencyMask();
+    bool onGpu = loop->region && loop->region->onGpu && loop->hasGpu;
+    if (!onGpu) {
+        DVMH_CHECK(!(settings.requireGpu && loop->region), "Loop at %s:%ld would run on the host but DVMH_AMD_REQUIRE_GPU is set",
+                loop->file.c_str(), loop->line);
+        DVMH_CHECK(loop->hasHost, "No host handler is registered for the loop at %s:%ld", loop->file.c_str(), loop->line);
+    }
+    if (loop->iterations() == 0) {
+        finishLoopReductions(loop);
+        return;
+    }
+    if (onGpu && batchQueue(loop)) return;
+    if (onGpu) prepareLayouts(loop);
+    SpecLoop sl(loop, onGpu ? 1 : 0);
+    sl.stream = gpu.stream;
+    loop->spec = &sl;
+    DvmType ref = (DvmType)&sl;
+    if (onGpu) {
+        hipEvent_t ev0 = nullptr, ev1 = nullptr;
+        bool timing = settings.tuneMode != 0;
+        if (timing) {
+            HIP_CHECK(hipEventCreate(&ev0));
+            HIP_CHECK(hipEventCreate(&ev1));
+            HIP_CHECK(hipEventRecord(ev0, sl.stream));
+        }
+        callHandler(loop->gpuH, &ref);
+        rt.loopsOnGpu++;
+        if (timing) {
+            HIP_CHECK(hipEventRecord(ev1, sl.stream));
+            HIP_CHECK(hipEventSynchronize(ev1));
+            float ms = 0;
+            HIP_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
+            HIP_CHECK(hipEventDestroy(ev0));
+            HIP_CHECK(hipEventDestroy(ev1));
+            rt.gpuLoopSeconds += ms * 1e-3;
+            if (sl.candidate >= 0) tunerRecord(&sl, ms * 1e-3);
+        }
+        for (void *p : sl.toFree) gpu.release(p);
+        finishLoopReductions(loop);
+        if (settings.syncMode == 0) flushDeferredPosts();
+    } else {
+        callHandler(loop->hostH, &ref);
+        rt.loopsOnHost++;
+        finishLoopReductions(loop);
+    }
+    loop->spec = nullptr;
+}
+
+}  // namespace dvmh

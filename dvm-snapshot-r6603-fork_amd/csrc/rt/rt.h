@@ -1,0 +1,292 @@
+// Internal declarations of libdvmh_amd: the one-process, one-GPU DVMH runtime.
+//
+// Object model (single process, so every array is wholly local):
+//   Data       a distributed array or a registered regular variable; owns a host
+//              buffer and at most one device representative (DevBuffer) whose
+//              layout may be axis-permuted or diagonalised.
+//   Region     a "#pragma dvm region": the arrays/scalars it uses and where it runs.
+//   Loop       a parallel loop as registered by converted code (bounds, mapping,
+//              reductions, ACROSS widths, handlers).
+//   SpecLoop   the loop as seen by a handler on one device ("loop_ref").
+// Reference counterparts: rts/dvmh/dvmh_data.h (DvmhData/DvmhRepresentative),
+// rts/dvmh/dvmh_buffer.h, rts/dvmh/region.h, rts/dvmh/loop.h (DvmhLoop,
+// DvmhSpecLoop/DvmhLoopCuda, DvmhReduction).
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <atomic>
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "dvmhlib2.h"
+#include "dvmh_amd.h"
+
+namespace dvmh {
+
+constexpr int MAX_RANK = 8;         // rts/dvmh/dvmh_types.h:223
+constexpr int MAX_LOOP_RANK = 12;   // rts/dvmh/dvmh_types.h:224
+constexpr int MAX_PARAMS = 64;      // handler parameters supported by the trampoline
+
+// ---------------------------------------------------------------- errors / log
+[[noreturn]] void fatal(const char *kind, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+void logMsg(int level, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+#define DVMH_CHECK(cond, ...) do { if (!(cond)) ::dvmh::fatal("error", __VA_ARGS__); } while (0)
+#define DVMH_INTERNAL(cond, ...) do { if (!(cond)) ::dvmh::fatal("internal error", __VA_ARGS__); } while (0)
+#define HIP_CHECK(expr) do { hipError_t e_ = (expr); if (e_ != hipSuccess) \
+    ::dvmh::fatal("HIP error", "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); } while (0)
+enum LogLevel { LOG_FATAL = 0, LOG_ERROR = 1, LOG_WARN = 2, LOG_INFO = 3, LOG_DEBUG = 4, LOG_TRACE = 5 };
+
+// ---------------------------------------------------------------- settings
+struct Settings {
+    int logLevel = LOG_WARN;
+    int numGpus = 1;            // DVMH_NUM_CUDAS: 0 runs every region on the host
+    int hostThreads = 0;        // DVMH_NUM_THREADS (0: OpenMP default)
+    bool requireGpu = false;    // DVMH_AMD_REQUIRE_GPU: abort instead of running a region on the host
+    int tuneMode = 0;           // DVMH_AMD_TUNE: 0 default config, 1 exhaustive sweep, 2 cost model, 3 fixed (from DB)
+    int syncMode = 0;           // DVMH_AMD_SYNC: 0 synchronise after every loop, 1 only when results are needed
+    bool graph = false;         // DVMH_AMD_GRAPH: capture repeating regions into HIP graphs
+    int tuneRepeats = 1;        // DVMH_AMD_TUNE_REPEATS: timed executions per candidate
+    std::string tuneDb;         // DVMH_AMD_TUNEDB: file of recorded timings
+    std::string modelFile;      // DVMH_AMD_MODEL: trained cost-model weights
+    void load();
+};
+extern Settings settings;
+
+// ---------------------------------------------------------------- device
+struct DeviceProps {
+    int cuCount = 0, waveSize = 64, maxThreadsPerBlock = 1024, maxBlock[3] = {1024, 1024, 1024};
+    long ldsPerBlock = 65536, ldsPerCu = 163840, regsPerCu = 0, l2Bytes = 0;
+    int clockKhz = 0, memClockKhz = 0, busWidth = 0;
+    char name[256] = {0}, arch[64] = {0};
+};
+
+class Gpu {
+public:
+    bool present = false;
+    int ordinal = 0;
+    DeviceProps props;
+    hipStream_t stream = nullptr;
+    void init();
+    void *alloc(size_t bytes);          // size-keyed cache in front of hipMalloc
+    void release(void *p);              // back to the cache
+    void trimCache();
+    void *pinnedStaging(size_t bytes);  // reusable pinned host buffer
+    void h2d(void *dst, const void *src, size_t bytes);
+    void d2h(void *dst, const void *src, size_t bytes);
+    void sync();
+private:
+    std::map<size_t, std::vector<void *>> freeBlocks;
+    std::map<void *, size_t> liveBlocks;
+    void *staging = nullptr;
+    size_t stagingBytes = 0;
+};
+extern Gpu gpu;
+
+// ---------------------------------------------------------------- objects
+enum class Kind : uint32_t { Data = 1, Template, Region, Loop, SpecLoop };
+constexpr uint32_t OBJ_MAGIC = 0x44564D48u;   // "DVMH"
+
+struct Object {
+    uint32_t magic = OBJ_MAGIC;
+    Kind kind;
+    explicit Object(Kind k): kind(k) {}
+    virtual ~Object() { magic = 0; }
+};
+
+struct Interval {
+    long lo = 0, hi = -1;
+    long size() const { return hi - lo + 1; }
+};
+
+// Device representative of an array.  Element (idx[0..rank-1]) of the natural
+// layout lives at position p = sum over storage positions q of
+// (idx[axisPerm[q]-1] - portion[axisPerm[q]-1].lo) * coll[q], except that with
+// diagState != 0 the last two storage positions are one diagonalised plane
+// (rts/dvmh/dvmh_buffer.cpp:149-230).
+struct DevBuffer {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    int axisPerm[MAX_RANK] = {0};   // storage position -> original axis (1-based)
+    int diagState = 0;              // 0 none, 1 diagonal, 2 slash diagonal
+    bool isTransformed(int rank) const;
+    bool isDiagonalized() const { return diagState != 0; }
+};
+
+struct Data : Object {
+    int rank = 0;
+    long typeSize = 0;
+    int rtType = rt_UNKNOWN;
+    Interval space[MAX_RANK];
+    long shdLow[MAX_RANK] = {0}, shdHigh[MAX_RANK] = {0};
+    char *host = nullptr;
+    bool ownsHost = false;
+    bool isRegular = false;         // regular C variable (dvmh_data_enter_C / scalar)
+    DvmType *userHeader = nullptr;
+    DevBuffer dev;
+    bool hasDev = false;
+    bool hostActual = true, devActual = false;
+    // pending reduction results, see Loop::deferredPosts
+    Data(): Object(Kind::Data) {}
+    ~Data() override;
+    long elemCount() const;
+    size_t bytes() const { return (size_t)elemCount() * (size_t)typeSize; }
+    long hostStride(int axis /*1-based*/) const;
+    void ensureDev();
+    void toDevice();                // host -> device (natural layout), marks device actual
+    void toHost();                  // device -> host (undoing any layout transform)
+    void setLayout(const int newPerm[], int newDiag);   // re-layout the device copy
+    void fillDevHeader(const void *base, DvmType header[], DvmType ext[]) const;
+    void *devNaturalBase() const;
+    void *hostNaturalBase() const;
+};
+
+struct Template : Object {
+    int rank = 0;
+    Interval space[MAX_RANK];
+    Template(): Object(Kind::Template) {}
+};
+
+struct RegionArray {
+    Data *data;
+    long intent;
+    std::string name;
+};
+
+struct Region : Object {
+    long flags = 0;
+    long targets = DEVICE_TYPE_HOST | DEVICE_TYPE_CUDA;
+    bool onGpu = false;
+    std::vector<RegionArray> arrays;
+    std::vector<RegionArray> scalars;
+    Region(): Object(Kind::Region) {}
+};
+
+struct Reduction {
+    int func = 0;
+    int rtType = rt_UNKNOWN;
+    long elemCount = 1, elemSize = 0;
+    void *arrayAddr = nullptr, *locAddr = nullptr;
+    long locSize = 0;
+    std::vector<char> initArr, initLoc;   // values at registration (rts/dvmh/loop.cpp:17-35)
+    std::mutex lock;
+    // per-execution device state (DvmhReductionCuda, rts/dvmh/loop.cpp:258-297)
+    void **gpuArrPtr = nullptr, **gpuLocPtr = nullptr;
+    void *gpuArr = nullptr, *gpuLoc = nullptr;
+    long items = 0;
+    std::vector<char> localArr, localLoc;
+    void initValues(void *arr, void *loc) const;                               // :37-91
+    void performOperation(void *dArr, void *dLoc, const void *sArr, const void *sLoc) const;  // :125-254
+    void postValues(const void *arr, const void *loc);                           // :97-102
+    bool isLoc() const { return locAddr != nullptr; }
+};
+
+struct AcrossSpec {
+    Data *data;
+    bool isOut;
+    long low[MAX_RANK], high[MAX_RANK];
+};
+
+struct HandlerDesc {
+    DvmHandlerFunc func = nullptr;
+    std::vector<void *> params;
+    long type = 0;
+};
+
+struct Loop;
+
+// The loop as seen from inside a handler (rts/dvmh/loop.h DvmhSpecLoop / DvmhLoopCuda).
+struct SpecLoop : Object {
+    Loop *loop;
+    int device;                     // 0 host, 1 GPU
+    hipStream_t stream = nullptr;
+    int block[3] = {0, 0, 0};
+    DvmType tune[DVMH_AMD_TUNE_N] = {0};
+    int candidate = -1;             // index in the tuner's candidate list
+    std::string siteKey;            // tuner site of this execution
+    std::vector<void *> toFree;
+    SpecLoop(Loop *l, int dev): Object(Kind::SpecLoop), loop(l), device(dev) {}
+};
+
+struct Loop : Object {
+    Region *region = nullptr;
+    int rank = 0;
+    long lo[MAX_LOOP_RANK], hi[MAX_LOOP_RANK], step[MAX_LOOP_RANK];
+    // mapping: loop axis (1-based, signed by direction, 0 = none) tied to each axis of mapData
+    Data *mapData = nullptr;
+    Template *mapTempl = nullptr;
+    int mapRank = 0;
+    long mapAxis[MAX_RANK] = {0}, mapMult[MAX_RANK] = {0}, mapSum[MAX_RANK] = {0};
+    std::vector<std::pair<Data *, std::vector<long>>> correspondences;
+    std::vector<std::shared_ptr<Reduction>> reds;
+    std::vector<AcrossSpec> acrosses;
+    int userBlock[3] = {0, 0, 0};
+    bool userBlockFlag = false;
+    long stage = 0;
+    HandlerDesc hostH, gpuH;
+    bool hasHost = false, hasGpu = false;
+    long depMask = 0;
+    std::string file;
+    long line = 0;
+    SpecLoop *spec = nullptr;
+    Loop(): Object(Kind::Loop) {}
+    ~Loop() override;
+    void computeDependencyMask();                       // rts/dvmh/loop.cpp:1117-1150
+    std::vector<long> correspondence(const Data *d) const;   // array axis -> signed loop axis
+    long iterations() const;
+};
+
+// ---------------------------------------------------------------- runtime state
+struct Runtime {
+    bool inited = false;
+    std::string curFile = "?";
+    long curLine = 0;
+    Region *curRegion = nullptr;
+    std::map<const void *, Data *> regularVars;     // address -> regular variable data
+    std::vector<Data *> allData;
+    // statistics of the current process (exported through dvmh_amd_stats_C)
+    long loopsOnHost = 0, loopsOnGpu = 0, kernelLaunchEstimate = 0;
+    double gpuLoopSeconds = 0.0;
+};
+extern Runtime rt;
+
+Object *asObject(DvmType handle);
+Data *asData(const DvmType desc[], const char *where);
+Loop *asLoop(DvmType h, const char *where);
+SpecLoop *asSpec(DvmType h, const char *where);
+Loop *loopOf(DvmType h, const char *where);         // Loop or SpecLoop handle
+long rtTypeSize(int rtType);
+
+// handler trampoline (rts/dvmh/util.cpp:659-720)
+void callHandler(const HandlerDesc &h, DvmType *pLoopRef, const std::vector<void *> *override = nullptr);
+
+// loop execution
+void performLoop(Loop *loop);
+
+// reductions on the GPU (kernels in csrc/kernels/reduce.hip)
+void gpuRedPrepare(SpecLoop *sl, Reduction *r, long count, bool fill);
+void gpuRedFinish(SpecLoop *sl, const std::shared_ptr<Reduction> &r);
+void flushDeferredPosts();
+
+// tuner
+struct TuneChoice {
+    int block[3];
+    DvmType tune[DVMH_AMD_TUNE_N];
+    int candidate;
+};
+void tunerPick(SpecLoop *sl, long kind, TuneChoice &out);
+void tunerRecord(SpecLoop *sl, double seconds);
+void tunerShutdown();
+
+// batching and graphs (batch.cpp)
+extern long graphReplays, batchGroups, batchLoops;
+bool batchQueue(Loop *loop);
+bool loopIsQueued(Loop *loop);
+int hostSlotCount();
+
+}  // namespace dvmh

@@ -1,0 +1,318 @@
+// Launch-configuration tuner.
+//
+// The reference tunes only the thread block, online, one trial per loop
+// execution, exploring by bisection of unseen intervals when DVMH_OPT_PARAMS=1
+// (rts/dvmh/loop.cpp:1487-1909).  Here a candidate is a whole launch
+// configuration (block, iterations per thread, XCD-aware order, kernel variant,
+// LDS window / wavefront tile, layout transform).  Modes:
+//   0  the default configuration (the reference's default blocks,
+//      rts/dvmh/loop.cpp:1464-1485);
+//   1  exhaustive: successive executions of a loop site try every candidate,
+//      then the fastest is used; every timing is recorded;
+//   2  cost model: the learned model ranks the candidates, the best few are
+//      measured and the fastest kept (falls back to mode 1 without a model);
+//   3  fixed: best recorded candidate from the tuning records.
+// Configurations never change arithmetic, only the mapping of iterations.
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <algorithm>
+
+#include "rt.h"
+#include "tuner.h"
+
+namespace dvmh {
+
+static std::map<std::string, SiteInfo> sites;
+
+std::map<std::string, SiteInfo> &tunerSites() { return sites; }
+
+static void addCand(std::vector<Candidate> &v, int bx, int by, int bz, std::initializer_list<std::pair<int, long>> t) {
+    Candidate c;
+    c.block[0] = bx; c.block[1] = by; c.block[2] = bz;
+    for (int i = 0; i < DVMH_AMD_TUNE_N; i++) c.tune[i] = 0;
+    c.tune[DVMH_AMD_T_DEPTH] = 1;
+    for (auto &kv : t) c.tune[kv.first] = kv.second;
+    for (auto &e : v)
+        if (e.same(c)) return;
+    v.push_back(c);
+}
+
+// Candidate lists.  Index 0 is always the default configuration.
+std::vector<Candidate> makeCandidates(int kind, int loopRank, int depCount, const long extents[]) {
+    std::vector<Candidate> v;
+    int mapped = loopRank - (kind == DVMH_AMD_KIND_PARALLEL ? 0 : depCount);   // axes spread over threads
+    if (kind == DVMH_AMD_KIND_PARALLEL) {
+        if (loopRank >= 3) addCand(v, 32, 4, 2, {});
+        else if (loopRank == 2) addCand(v, 32, 8, 1, {});
+        else addCand(v, 256, 1, 1, {});
+        const int xs[] = {64, 128, 256};
+        const int ys[] = {1, 2, 4, 8, 16};
+        const int zs[] = {1, 2, 4};
+        const int depths[] = {1, 2, 4, 8};
+        for (int bx : xs)
+            for (int by : ys)
+                for (int bz : zs) {
+                    if (loopRank < 3 && bz > 1) continue;
+                    if (loopRank < 2 && by > 1) continue;
+                    int n = bx * by * bz;
+                    if (n < 64 || n > 1024) continue;
+                    for (int dp : depths) {
+                        if (loopRank < 2 && dp > 1) continue;
+                        for (int xcd = 0; xcd <= 1; xcd++)
+                            addCand(v, bx, by, bz, {{DVMH_AMD_T_DEPTH, dp}, {DVMH_AMD_T_XCD, xcd}});
+                    }
+                }
+    } else if (kind == DVMH_AMD_KIND_LINE) {
+        // threads over the independent axes, each walks its line
+        if (mapped >= 2) addCand(v, 32, 8, 1, {});
+        else addCand(v, 256, 1, 1, {});
+        const int xs[] = {64, 128, 256};
+        const int ys[] = {1, 2, 4};
+        const int chunks[] = {0, 8, 16, 32};
+        for (int bx : xs)
+            for (int by : ys) {
+                if (mapped < 2 && by > 1) continue;
+                int n = bx * by;
+                if (n < 64 || n > 1024) continue;
+                for (int ch : chunks)
+                    for (int tf = 0; tf <= 1; tf++)
+                        addCand(v, bx, by, 1, {{DVMH_AMD_T_CHUNK, ch}, {DVMH_AMD_T_TRANSFORM, tf}});
+            }
+    } else {
+        // wavefront over tiles: tile extents on the two innermost dependent axes
+        // (A: second innermost, B: innermost) and along the outer dependent axis (CHUNK)
+        addCand(v, 64, 1, 1, {{DVMH_AMD_T_TILE_A, 4}, {DVMH_AMD_T_TILE_B, 64}, {DVMH_AMD_T_CHUNK, 4}});
+        const int tas[] = {2, 4, 8, 16};
+        const int tbs[] = {32, 64, 128};
+        const int tcs[] = {2, 4, 8};
+        for (int ta : tas)
+            for (int tb : tbs)
+                for (int tc : tcs)
+                    for (int var = 0; var <= 1; var++)
+                        addCand(v, tb < 64 ? 64 : tb, 1, 1,
+                                {{DVMH_AMD_T_TILE_A, ta}, {DVMH_AMD_T_TILE_B, tb}, {DVMH_AMD_T_CHUNK, tc}, {DVMH_AMD_T_VARIANT, var}});
+    }
+    (void)extents;
+    return v;
+}
+
+std::string siteKey(const Loop *l, int kind) {
+    std::ostringstream k;
+    k << l->file << ":" << l->line << "|k" << kind << "|r" << l->rank << "|m" << l->depMask << "|e";
+    for (int i = 0; i < l->rank; i++) {
+        long c = l->step[i] > 0 ? (l->hi[i] - l->lo[i]) / l->step[i] + 1 : (l->lo[i] - l->hi[i]) / (-l->step[i]) + 1;
+        k << (i ? "x" : "") << c;
+    }
+    return k.str();
+}
+
+static SiteInfo &siteFor(SpecLoop *sl, int kind) {
+    std::string key = siteKey(sl->loop, kind);
+    auto it = sites.find(key);
+    if (it != sites.end()) return it->second;
+    SiteInfo &s = sites[key];
+    s.key = key;
+    s.kind = kind;
+    s.rank = sl->loop->rank;
+    s.depCount = __builtin_popcountl((unsigned long)sl->loop->depMask);
+    for (int i = 0; i < sl->loop->rank && i < 8; i++) {
+        Loop *l = sl->loop;
+        s.extents[i] = l->step[i] > 0 ? (l->hi[i] - l->lo[i]) / l->step[i] + 1 : (l->lo[i] - l->hi[i]) / (-l->step[i]) + 1;
+    }
+    s.cands = makeCandidates(kind, s.rank, s.depCount, s.extents);
+    s.times.assign(s.cands.size(), std::vector<double>());
+    return s;
+}
+
+double SiteInfo::measured(int c) const {
+    const auto &t = times[c];
+    if (t.empty()) return -1;
+    // the minimum of the repeats: the configuration's cost without interference
+    return *std::min_element(t.begin(), t.end());
+}
+
+int SiteInfo::bestMeasured() const {
+    int best = -1;
+    double bt = 0;
+    for (size_t c = 0; c < cands.size(); c++) {
+        double t = measured((int)c);
+        if (t < 0) continue;
+        if (best < 0 || t < bt) { best = (int)c; bt = t; }
+    }
+    return best;
+}
+
+extern int modelRank(SiteInfo &s, std::vector<int> &order);   // cost model (model.cpp)
+
+void tunerPick(SpecLoop *sl, long kind, TuneChoice &out) {
+    Loop *l = sl->loop;
+    int chosen = 0;
+    SiteInfo *s = nullptr;
+    if (l->userBlockFlag) {
+        // a user cuda_block wins over everything (rts/dvmh/loop.cpp:2141-2147)
+        std::vector<Candidate> c = makeCandidates((int)kind, l->rank, __builtin_popcountl((unsigned long)l->depMask), nullptr);
+        Candidate d = c[0];
+        for (int i = 0; i < 3; i++) out.block[i] = l->userBlock[i];
+        for (int i = 0; i < DVMH_AMD_TUNE_N; i++) out.tune[i] = d.tune[i];
+        out.candidate = -1;
+        return;
+    }
+    s = &siteFor(sl, (int)kind);
+    int mode = settings.tuneMode;
+    if (mode == 2 && !s->modelRanked) {
+        s->modelOrder.clear();
+        if (modelRank(*s, s->modelOrder) > 0) s->modelRanked = true;
+        else mode = 1;
+    }
+    if (mode == 0) {
+        chosen = 0;
+    } else if (mode == 3) {
+        int b = s->bestMeasured();
+        chosen = b >= 0 ? b : 0;
+    } else if (mode == 1 || (mode == 2 && !s->modelRanked)) {
+        // next candidate lacking its repeats, else the best
+        chosen = -1;
+        for (size_t c = 0; c < s->cands.size(); c++)
+            if ((int)s->times[c].size() < settings.tuneRepeats) { chosen = (int)c; break; }
+        if (chosen < 0) chosen = s->bestMeasured();
+    } else {
+        // model: measure the top-K predictions (plus the default), then keep the best
+        const int K = 4;
+        chosen = -1;
+        std::vector<int> trial;
+        trial.push_back(0);
+        for (int i = 0; i < (int)s->modelOrder.size() && (int)trial.size() < K + 1; i++)
+            if (s->modelOrder[i] != 0) trial.push_back(s->modelOrder[i]);
+        for (int c : trial)
+            if ((int)s->times[c].size() < settings.tuneRepeats) { chosen = c; break; }
+        if (chosen < 0) {
+            double bt = 0;
+            for (int c : trial) {
+                double t = s->measured(c);
+                if (t >= 0 && (chosen < 0 || t < bt)) { chosen = c; bt = t; }
+            }
+        }
+    }
+    const Candidate &c = s->cands[chosen];
+    for (int i = 0; i < 3; i++) out.block[i] = c.block[i];
+    for (int i = 0; i < DVMH_AMD_TUNE_N; i++) out.tune[i] = c.tune[i];
+    out.candidate = chosen;
+    s->lastPicked = chosen;
+    sl->candidate = chosen;
+    sl->siteKey = s->key;
+}
+
+void tunerRecord(SpecLoop *sl, double seconds) {
+    auto it = sites.find(sl->siteKey);
+    if (it == sites.end() || sl->candidate < 0) return;
+    SiteInfo &s = it->second;
+    if (sl->candidate < (int)s.times.size()) s.times[sl->candidate].push_back(seconds);
+}
+
+// ---------------------------------------------------------------- records
+// One line per measurement: key \t candidate \t bx by bz \t tune[8] \t seconds
+long tunerSave(const std::string &path) {
+    std::ofstream f(path);
+    if (!f) return -1;
+    long n = 0;
+    for (auto &kv : sites) {
+        const SiteInfo &s = kv.second;
+        for (size_t c = 0; c < s.cands.size(); c++)
+            for (double t : s.times[c]) {
+                f << s.key << "\t" << c << "\t" << s.cands[c].block[0] << " " << s.cands[c].block[1] << " " << s.cands[c].block[2] << "\t";
+                for (int i = 0; i < DVMH_AMD_TUNE_N; i++) f << (i ? " " : "") << s.cands[c].tune[i];
+                f << "\t" << t << "\n";
+                n++;
+            }
+    }
+    return n;
+}
+
+// Parse a key back into the site description (kind, rank, depCount, extents).
+static bool parseKey(const std::string &key, SiteInfo &s) {
+    size_t k = key.find("|k");
+    if (k == std::string::npos) return false;
+    s.kind = atoi(key.c_str() + k + 2);
+    size_t r = key.find("|r", k);
+    size_t m = key.find("|m", k);
+    size_t e = key.find("|e", k);
+    if (r == std::string::npos || m == std::string::npos || e == std::string::npos) return false;
+    s.rank = atoi(key.c_str() + r + 2);
+    long mask = atol(key.c_str() + m + 2);
+    s.depCount = __builtin_popcountl((unsigned long)mask);
+    std::string ex = key.substr(e + 2);
+    std::stringstream ss(ex);
+    std::string part;
+    int i = 0;
+    while (std::getline(ss, part, 'x') && i < 8) s.extents[i++] = atol(part.c_str());
+    return true;
+}
+
+long tunerLoad(const std::string &path) {
+    std::ifstream f(path);
+    if (!f) return -1;
+    std::string line;
+    long n = 0;
+    while (std::getline(f, line)) {
+        std::stringstream ss(line);
+        std::string key, cs, bs, ts, secs;
+        if (!std::getline(ss, key, '\t') || !std::getline(ss, cs, '\t') || !std::getline(ss, bs, '\t') || !std::getline(ss, ts, '\t') ||
+                !std::getline(ss, secs, '\t'))
+            continue;
+        auto it = sites.find(key);
+        if (it == sites.end()) {
+            SiteInfo s;
+            s.key = key;
+            if (!parseKey(key, s)) continue;
+            s.cands = makeCandidates(s.kind, s.rank, s.depCount, s.extents);
+            s.times.assign(s.cands.size(), std::vector<double>());
+            it = sites.emplace(key, s).first;
+        }
+        int c = atoi(cs.c_str());
+        if (c >= 0 && c < (int)it->second.times.size()) {
+            it->second.times[c].push_back(atof(secs.c_str()));
+            n++;
+        }
+    }
+    return n;
+}
+
+void tunerInit() {
+    if (!settings.tuneDb.empty()) tunerLoad(settings.tuneDb);
+}
+
+void tunerShutdown() {
+    if (!settings.tuneDb.empty() && settings.tuneMode != 0 && settings.tuneMode != 3) tunerSave(settings.tuneDb);
+}
+
+}  // namespace dvmh
+
+using namespace dvmh;
+
+extern "C" void dvmh_amd_tuner_set_mode_C(DvmType mode) {
+    DVMH_CHECK(mode >= 0 && mode <= 3, "Tuner mode must be 0..3");
+    settings.tuneMode = (int)mode;
+}
+
+extern "C" DvmType dvmh_amd_tuner_save_C(const char *path) { return tunerSave(path); }
+extern "C" DvmType dvmh_amd_tuner_load_C(const char *path) { return tunerLoad(path); }
+extern "C" void dvmh_amd_tuner_clear_C(void) { sites.clear(); }
+
+extern "C" DvmType dvmh_amd_tuner_query_C(const char *file, DvmType line, DvmType out[10]) {
+    std::string prefix = std::string(file) + ":" + std::to_string(line) + "|";
+    for (auto &kv : sites) {
+        if (kv.first.compare(0, prefix.size(), prefix) != 0) continue;
+        int b = kv.second.bestMeasured();
+        if (b < 0) return 0;
+        const Candidate &c = kv.second.cands[b];
+        out[0] = b;
+        out[1] = (DvmType)(kv.second.measured(b) * 1e9);
+        out[2] = c.block[0]; out[3] = c.block[1]; out[4] = c.block[2];
+        for (int i = 0; i < 5; i++) out[5 + i] = c.tune[i];
+        return 1;
+    }
+    return 0;
+}

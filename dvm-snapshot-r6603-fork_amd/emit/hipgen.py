@@ -1,0 +1,761 @@
+#!/usr/bin/env python3
+"""Handler emitter: the back end of cdvmh, producing gfx950 HIP.
+
+Input: a loop description file (<program>.loops, a Python literal) that lists,
+for each parallel loop of a converted program, what cdvmh's front end knows
+about it: loop variables and constant steps, the handler parameters (DVM
+arrays passed as headers, scalars passed by pointer), private variables,
+reduction clauses, the ACROSS clause and the loop body text.
+
+Output, per program:
+  <program>_host.c   host handlers (C + OpenMP), the shape of cdvmh's
+                     genHostHandler (cdvmh/src/converter_main.cpp:1503-1859):
+                     red_init/red_post per thread, "omp for" on the first
+                     independent axis, the pipelined handshake when every axis
+                     carries an ACROSS dependency.
+  <program>_hip.hip  GPU handlers + kernels.  The handler makes the same ABI
+                     calls as cdvmh's genCudaHandler (converter_main.cpp:
+                     2132-2401) / genAcrossCudaHandler (across.cpp:2788-2894),
+                     asks the runtime for its tuned configuration
+                     (dvmh_amd_loop_get_tuning_C) and launches kernels designed
+                     for wave64/LDS:
+                       PARALLEL   one launch; x <- innermost axis, y, z next;
+                                  DEPTH consecutive iterations of the slowest
+                                  mapped axis per thread; XCD-aware tile order;
+                                  in-kernel wave+LDS reduction, one partial
+                                  per workgroup.
+                       LINE       ACROSS with one dependent axis: one launch,
+                                  threads over the independent axes, each
+                                  thread walks its line in dependency order;
+                                  optionally through an LDS window of CHUNK
+                                  steps that is loaded and stored coalesced.
+                       WAVEFRONT  ACROSS with >= 2 dependent axes: hyperplane
+                                  wavefront (one launch per hyperplane of
+                                  tiles; inside a tile, local hyperplanes
+                                  separated by workgroup barriers).
+The loop body is inserted verbatim; every schedule respects the dependency
+order, so results are bit-identical to the sequential host order.
+"""
+import ast
+import itertools
+import os
+import re
+import sys
+
+RT = {'char': 'rt_CHAR', 'int': 'rt_INT', 'long': 'rt_LONG', 'float': 'rt_FLOAT', 'double': 'rt_DOUBLE',
+      'long long': 'rt_LLONG', 'unsigned': 'rt_UINT', 'unsigned long': 'rt_ULONG'}
+RED_FUNC = {'sum': 'rf_SUM', 'prod': 'rf_PROD', 'max': 'rf_MAX', 'min': 'rf_MIN', 'and': 'rf_AND', 'or': 'rf_OR',
+            'xor': 'rf_XOR', 'maxloc': 'rf_MAXLOC', 'minloc': 'rf_MINLOC'}
+RED_OP = {'sum': 'OpSum', 'prod': 'OpProd', 'max': 'OpMax', 'min': 'OpMin', 'and': 'OpAnd', 'or': 'OpOr', 'xor': 'OpXor'}
+OMP_RED = {'sum': '+', 'prod': '*', 'max': 'max', 'min': 'min', 'and': '&', 'or': '|', 'xor': '^'}
+IND = '    '
+
+
+def indent(text, n):
+    pad = IND * n
+    return ''.join(pad + ln if ln.strip() else ln for ln in text.splitlines(True))
+
+
+class Loop:
+    def __init__(self, d, prog):
+        self.prog = prog
+        self.name = d['name']
+        self.vars = d['vars']                      # [(name, ctype)] outermost first
+        self.rank = len(self.vars)
+        self.steps = d.get('steps', ['1'] * self.rank)   # constant steps as text, None = runtime
+        self.params = d['params']                  # [('array', name, ctype, rank) | ('scalar', name, ctype)]
+        self.privates = d.get('privates', [])      # [(decl text)]
+        self.reds = d.get('reductions', [])        # [(func, var, ctype)] or [(func, var, ctype, loc, loctype)]
+        self.across = d.get('across')              # None | {'array': name, 'widths': [(lo, hi), ...]}
+        self.body = d['body'].strip('\n')
+        self.cuda_block = d.get('cuda_block')      # optional default block
+        self.batch = d.get('batch', False)         # also emit a batch handler
+        self.arrays = [p for p in self.params if p[0] == 'array']
+        self.scalars = [p for p in self.params if p[0] == 'scalar']
+
+    # static dependency masks the ACROSS clause can produce (identity correspondence)
+    def dep_axes(self):
+        if not self.across:
+            return []
+        return [a for a, (lo, hi) in enumerate(self.across['widths']) if lo > 0 or hi > 0]
+
+
+# ----------------------------------------------------------------- host handlers
+def host_handler(L):
+    r = L.rank
+    out = []
+    formal = ['DvmType *pLoopRef']
+    for p in L.params:
+        formal.append(f'DvmType {p[1]}_hdr[]' if p[0] == 'array' else f'{p[2]} *{p[1]}_ptr')
+    out.append(f'void {L.name}_host({", ".join(formal)}) {{')
+    b = []
+    b.append('/* Loop reference and device number */')
+    b.append('DvmType loop_ref = *pLoopRef;')
+    b.append('DvmType device_num = dvmh_loop_get_device_num_C(loop_ref);')
+    b.append('/* Parameters */')
+    for p in L.params:
+        if p[0] == 'array':
+            _, nm, ct, ar = p
+            dims = ''
+            for j in range(2, ar + 1):
+                dims += f'[{nm}_hdr[{j - 1}]/{nm}_hdr[{j}]]' if j < ar else f'[{nm}_hdr[{j - 1}]]'
+            b.append(f'{ct} (*{nm}){dims} = ({ct} (*){dims})dvmh_get_natural_base_C(device_num, {nm}_hdr);')
+        else:
+            b.append(f'{p[2]} {p[1]} = *{p[1]}_ptr;')
+    b.append('/* Supplementary variables for loop handling */')
+    b.append(f'DvmType boundsLow[{r}], boundsHigh[{r}], loopSteps[{r}];')
+    b.append('int slotCount;')
+    if L.across:
+        b.append('DvmType dependencyMask;')
+    b.append('/* User variables - loop index variables and other private variables */')
+    for v, ct in L.vars:
+        b.append(f'{ct} {v};')
+    for pr in L.privates:
+        b.append(pr + ';')
+    for red in L.reds:
+        b.append(f'{red[2]} {red[1]};')
+        if len(red) > 3:
+            b.append(f'{red[4]} {red[3]};')
+    b.append('')
+    b.append('dvmh_loop_fill_bounds_C(loop_ref, boundsLow, boundsHigh, loopSteps);')
+    b.append('slotCount = dvmh_loop_get_slot_count_C(loop_ref);')
+    if L.across:
+        b.append('dependencyMask = dvmh_loop_get_dependency_mask_C(loop_ref);')
+        b.append('int threadSync[slotCount > 0 ? slotCount : 1];')
+    privs = [v for v, _ in L.vars] + [re.split(r'[\s\*]+', pr.strip())[-1].split('[')[0] for pr in L.privates]
+    redvars = []
+    for red in L.reds:
+        redvars.append(red[1])
+        if len(red) > 3:
+            redvars.append(red[3])
+    clause = f'#pragma omp parallel num_threads(slotCount)'
+    if privs:
+        clause += f', private({", ".join(privs)})'
+    if redvars:
+        clause += f', private({", ".join(redvars)})'
+    b.append(clause)
+    b.append('{')
+    inner = []
+    if L.across:
+        inner.append('int currentThread = 0, workingThreads = slotCount;')
+    for i, red in enumerate(L.reds):
+        loc = ('&' + red[3]) if len(red) > 3 else '0'
+        inner.append(f'dvmh_loop_red_init_C(loop_ref, {i + 1}, &{red[1]}, {loc});')
+
+    def loop_head(i):
+        v = L.vars[i][0]
+        st = L.steps[i]
+        cmp = '<=' if (st is None or not st.strip().startswith('-')) else '>='
+        if st is None:
+            cmp_expr = f'(loopSteps[{i}] > 0 ? {v} <= boundsHigh[{i}] : {v} >= boundsHigh[{i}])'
+            return f'for ({v} = boundsLow[{i}]; {cmp_expr}; {v} += loopSteps[{i}])'
+        inc = f'{v}++' if st == '1' else (f'{v}--' if st == '-1' else f'{v} += ({st})')
+        return f'for ({v} = boundsLow[{i}]; {v} {cmp} boundsHigh[{i}]; {inc})'
+
+    body = '{\n' + indent(L.body + '\n', 1) + '}'
+    variants = L.rank + 1 if L.across else 1
+    for par in range(variants):
+        code = []
+        if variants > 1:
+            if par < L.rank:
+                cond = f'if (((dependencyMask >> {L.rank - par - 1}) & 1) == 0) '
+            else:
+                cond = ''
+            code.append(('} else ' if par > 0 else '') + cond + '{')
+        depth = 1 if variants > 1 else 0
+        lines = []
+        if par < L.rank:
+            for i in range(L.rank):
+                if i == par:
+                    lines.append(IND * i + '#pragma omp for schedule(runtime), nowait')
+                lines.append(IND * i + loop_head(i))
+            lines.append(indent(body, L.rank))
+        else:
+            # every axis dependent: pipelined handshake over the first two axes
+            lines.append('{')
+            lines.append(IND + 'DvmType paralIters = (boundsHigh[1] - boundsLow[1]) / loopSteps[1] + 1;')
+            lines.append(IND + 'if (paralIters < workingThreads) workingThreads = paralIters;')
+            lines.append('#ifdef _OPENMP')
+            lines.append(IND + 'workingThreads = workingThreads < omp_get_num_threads() ? workingThreads : omp_get_num_threads();')
+            lines.append(IND + 'currentThread = omp_get_thread_num();')
+            lines.append('#endif')
+            lines.append(IND + '__atomic_store_n(&threadSync[currentThread], 0, __ATOMIC_SEQ_CST);')
+            lines.append(IND + '#pragma omp barrier')
+            lines.append(IND + loop_head(0) + ' {')
+            lines.append(IND * 2 + 'if (currentThread > 0 && currentThread < workingThreads) {')
+            lines.append(IND * 3 + 'while (!__atomic_load_n(&threadSync[currentThread - 1], __ATOMIC_ACQUIRE)) {}')
+            lines.append(IND * 3 + '__atomic_store_n(&threadSync[currentThread - 1], 0, __ATOMIC_RELEASE);')
+            lines.append(IND * 2 + '}')
+            lines.append(IND * 2 + '#pragma omp for schedule(static), nowait')
+            for i in range(1, L.rank):
+                lines.append(IND * (i + 1) + loop_head(i))
+            lines.append(indent(body, L.rank + 1))
+            lines.append(IND * 2 + 'if (currentThread < workingThreads - 1) {')
+            lines.append(IND * 3 + 'while (__atomic_load_n(&threadSync[currentThread], __ATOMIC_ACQUIRE)) {}')
+            lines.append(IND * 3 + '__atomic_store_n(&threadSync[currentThread], 1, __ATOMIC_RELEASE);')
+            lines.append(IND * 2 + '}')
+            lines.append(IND + '}')
+            lines.append('}')
+        code.append(indent('\n'.join(lines) + '\n', depth).rstrip('\n'))
+        inner.extend(code)
+    if variants > 1:
+        inner.append('}')
+    for i, red in enumerate(L.reds):
+        loc = ('&' + red[3]) if len(red) > 3 else '0'
+        inner.append(f'dvmh_loop_red_post_C(loop_ref, {i + 1}, &{red[1]}, {loc});')
+    b.append(indent('\n'.join(inner) + '\n', 1).rstrip('\n'))
+    b.append('}')
+    out.append(indent('\n'.join(b) + '\n', 1).rstrip('\n'))
+    out.append('}')
+    return '\n'.join(out) + '\n'
+
+
+# ----------------------------------------------------------------- GPU: common pieces
+def kernel_array_params(L, unit):
+    fp, setup = [], []
+    for p in L.arrays:
+        _, nm, ct, ar = p
+        fp.append(f'{ct} *__restrict__ {nm}_base')
+        fp += [f'IndexT {nm}_s{k}' for k in range(ar)]
+        strides = ', '.join(f'{nm}_s{k}' for k in range(ar))
+        setup.append(f'const IndexT {nm}_str[{ar}] = {{{strides}}};')
+        setup.append(f'const auto {nm} = dvmh_dev::makeView<{ct}, IndexT, {ar}, {"true" if unit else "false"}>({nm}_base, {nm}_str);')
+    for p in L.scalars:
+        fp.append(f'{p[2]} *__restrict__ {p[1]}_ptr')
+        setup.append(f'{p[2]} &{p[1]} = *{p[1]}_ptr;')
+    return fp, setup
+
+
+def kernel_diag_array_params(L):
+    fp, setup = [], []
+    for p in L.arrays:
+        _, nm, ct, ar = p
+        fp.append(f'{ct} *__restrict__ {nm}_base')
+        fp += [f'IndexT {nm}_s{k}' for k in range(ar)]
+        fp.append(f'dvmh_dev::DiagInfo<IndexT> {nm}_di')
+        strides = ', '.join(f'{nm}_s{k}' for k in range(ar))
+        setup.append(f'const IndexT {nm}_str[{ar}] = {{{strides}}};')
+        setup.append(f'const auto {nm} = dvmh_dev::makeDiagView<{ct}, IndexT, {ar}>({nm}_base, {nm}_str, {nm}_di);')
+    for p in L.scalars:
+        fp.append(f'{p[2]} *__restrict__ {p[1]}_ptr')
+        setup.append(f'{p[2]} &{p[1]} = *{p[1]}_ptr;')
+    return fp, setup
+
+
+def red_kernel_params(L):
+    fp = []
+    for red in L.reds:
+        fp += [f'{red[2]} {red[1]}_init', f'{red[2]} *{red[1]}_grid']
+        if len(red) > 3:
+            fp += [f'{red[4]} {red[3]}_init', f'{red[4]} *{red[3]}_grid']
+    return fp
+
+
+def red_kernel_locals(L):
+    s = []
+    for red in L.reds:
+        s.append(f'{red[2]} {red[1]} = {red[1]}_init;')
+        if len(red) > 3:
+            s.append(f'{red[4]} {red[3]} = {red[3]}_init;')
+    return s
+
+
+def red_kernel_tail(L, slot, accumulate):
+    """Workgroup reduction; thread 0 stores (or folds into) the workgroup's slot."""
+    s = []
+    tid0 = '(threadIdx.x | threadIdx.y | threadIdx.z) == 0'
+    for red in L.reds:
+        f, v, ct = red[0], red[1], red[2]
+        if len(red) > 3:
+            loc = red[3]
+            mx = 'true' if f == 'maxloc' else 'false'
+            s.append(f'dvmh_dev::blockReduceLoc<{ct}, {red[4]}, {mx}>({v}, {loc});')
+            if accumulate:
+                cmpo = '<' if f == 'maxloc' else '>'
+                s.append(f'if ({tid0}) {{')
+                s.append(f'    {ct} ov = {v}_grid[{slot}];')
+                s.append(f'    {red[4]} ol = {loc}_grid[{slot}];')
+                s.append(f'    if (ov {cmpo} {v} || (!({v} {cmpo} ov) && {loc} < ol)) {{ {v}_grid[{slot}] = {v}; {loc}_grid[{slot}] = {loc}; }}')
+                s.append('}')
+            else:
+                s.append(f'if ({tid0}) {{ {v}_grid[{slot}] = {v}; {loc}_grid[{slot}] = {loc}; }}')
+        else:
+            s.append(f'{v} = dvmh_dev::blockReduce({v}, dvmh_dev::{RED_OP[f]}());')
+            if accumulate:
+                s.append(f'if ({tid0}) {v}_grid[{slot}] = dvmh_dev::{RED_OP[f]}()({v}_grid[{slot}], {v});')
+            else:
+                s.append(f'if ({tid0}) {v}_grid[{slot}] = {v};')
+    return s
+
+
+def handler_prologue(L, kind_const, want_ext=False, autotfm=False):
+    h = []
+    h.append('DvmType tmpVar;')
+    h.append('/* Loop reference and device number */')
+    h.append('DvmType loop_ref = *pLoopRef;')
+    h.append('DvmType device_num = dvmh_loop_get_device_num_C(loop_ref);')
+    h.append('DvmType tune[DVMH_AMD_TUNE_N];')
+    h.append(f'dvmh_amd_loop_get_tuning_C(loop_ref, {kind_const}, tune);')
+    h.append('/* Parameters */')
+    for p in L.arrays:
+        _, nm, ct, ar = p
+        if autotfm:
+            h.append(f'if (tune[DVMH_AMD_T_TRANSFORM]) dvmh_loop_autotransform_C(loop_ref, {nm}_hdr);')
+        h.append(f'{ct} *{nm} = ({ct} *)dvmh_get_natural_base_C(device_num, {nm}_hdr);')
+        h.append(f'DvmType {nm}_devHdr[64 + 7];')
+        ext = f'{nm}_devHdr + 64' if want_ext else '0'
+        h.append(f'tmpVar = dvmh_fill_header_C(device_num, {nm}, {nm}_hdr, {nm}_devHdr, {ext});')
+        h.append(f'int {nm}_tfm = (int)tmpVar;')
+    for p in L.scalars:
+        h.append(f'{p[2]} *{p[1]} = ({p[2]} *)dvmh_get_device_addr_C(device_num, {p[1]}_ptr);')
+    h.append(f'DvmType boundsLow[{L.rank}], boundsHigh[{L.rank}], loopSteps[{L.rank}];')
+    h.append('dim3 threads(0, 0, 0);')
+    h.append('hipStream_t stream;')
+    h.append('/* Choose index type for the kernel */')
+    h.append('int kernelIndexT = (int)dvmh_loop_guess_index_type_C(loop_ref);')
+    h.append('if (kernelIndexT == rt_LONG) kernelIndexT = rt_LLONG;')
+    h.append('dvmh_loop_cuda_get_config_C(loop_ref, 0, 0, &threads, &stream, 0);')
+    h.append('dvmh_loop_fill_bounds_C(loop_ref, boundsLow, boundsHigh, loopSteps);')
+    return h
+
+
+def handler_reds_prepare(L, count_expr, fill):
+    h = []
+    for i, red in enumerate(L.reds):
+        h.append(f'{red[2]} {red[1]};')
+        h.append(f'{red[2]} *{red[1]}_grid;')
+        if len(red) > 3:
+            h.append(f'{red[4]} {red[3]};')
+            h.append(f'{red[4]} *{red[3]}_grid;')
+            h.append(f'dvmh_loop_cuda_register_red_C(loop_ref, {i + 1}, (void **)&{red[1]}_grid, (void **)&{red[3]}_grid);')
+            h.append(f'dvmh_loop_red_init_C(loop_ref, {i + 1}, &{red[1]}, &{red[3]});')
+        else:
+            h.append(f'dvmh_loop_cuda_register_red_C(loop_ref, {i + 1}, (void **)&{red[1]}_grid, 0);')
+            h.append(f'dvmh_loop_red_init_C(loop_ref, {i + 1}, &{red[1]}, 0);')
+        h.append(f'dvmh_loop_cuda_red_prepare_C(loop_ref, {i + 1}, {count_expr}, {fill});')
+    return h
+
+
+def handler_reds_finish(L):
+    return [f'dvmh_loop_cuda_red_finish_C(loop_ref, {i + 1});' for i in range(len(L.reds))]
+
+
+def red_fact_args(L):
+    a = []
+    for red in L.reds:
+        a += [red[1], f'{red[1]}_grid']
+        if len(red) > 3:
+            a += [red[3], f'{red[3]}_grid']
+    return a
+
+
+def array_fact_args(L, itype, diag=False):
+    a = []
+    for p in L.arrays:
+        _, nm, ct, ar = p
+        a.append(nm)
+        a += [f'({itype}){nm}_devHdr[{k + 1}]' for k in range(ar)]
+        if diag:
+            a.append(f'{nm}_di_{itype.replace(" ", "_")}')
+    for p in L.scalars:
+        a.append(p[1])
+    return a
+
+
+def var_assign(L, i, tname, lo, st):
+    v, ct = L.vars[i]
+    return f'{v} = ({ct})({lo} + {tname} * {st});'
+
+
+def launch_dispatch(kernels, args, grid, block='threads'):
+    """kernels: list of (condition, kernel instantiation text)."""
+    s = []
+    first = True
+    for cond, k in kernels:
+        kw = 'if' if first else 'else if'
+        first = False
+        s.append(f'{kw} ({cond}) hipLaunchKernelGGL(({k}), {grid}, {block}, 0, stream, {", ".join(args)});')
+    return s
+
+
+# ----------------------------------------------------------------- GPU: PARALLEL
+def parallel_kernel(L):
+    r = L.rank
+    fp, setup = kernel_array_params(L, unit=True)
+    for i in range(r):
+        fp += [f'IndexT lo{i}', f'IndexT cnt{i}', f'IndexT st{i}']
+    fp += ['IndexT nbx', 'IndexT nby', 'IndexT nbz', 'long nblocks', 'int xcd']
+    fp += red_kernel_params(L)
+    k = []
+    k.append(f'template <typename IndexT, int DEPTH>')
+    k.append(f'__global__ __launch_bounds__(1024) void {L.name}_kernel({", ".join(fp)}) {{')
+    b = list(setup)
+    for v, ct in L.vars:
+        b.append(f'{ct} {v};')
+    for pr in L.privates:
+        b.append(pr + ';')
+    b += red_kernel_locals(L)
+    b.append('long tile = dvmh_dev::xcdTile((long)blockIdx.x, nblocks, xcd);')
+    b.append('const IndexT bx = (IndexT)(tile % nbx); tile /= nbx;')
+    b.append('const IndexT by = (IndexT)(tile % nby); tile /= nby;')
+    b.append('const IndexT bz = (IndexT)(tile % nbz); tile /= nbz;')
+    # thread-mapped axes
+    ax_x = r - 1
+    ax_y = r - 2 if r >= 2 else None
+    ax_z = r - 3 if r >= 3 else None
+    dep_ax = ax_z if ax_z is not None else (ax_y if ax_y is not None else ax_x)
+    outer = list(range(0, r - 3)) if r > 3 else []
+    for a in reversed(outer):
+        b.append(f'const IndexT t{a} = (IndexT)(tile % cnt{a}); tile /= cnt{a};')
+    b.append(f'const IndexT t{ax_x}_0 = bx * (IndexT)blockDim.x + (IndexT)threadIdx.x;')
+    if ax_y is not None:
+        b.append(f'const IndexT t{ax_y}_0 = by * (IndexT)blockDim.y + (IndexT)threadIdx.y;')
+    if ax_z is not None:
+        b.append(f'const IndexT t{ax_z}_0 = bz * (IndexT)blockDim.z + (IndexT)threadIdx.z;')
+    conds = []
+    for a in [ax_x, ax_y, ax_z]:
+        if a is None:
+            continue
+        if a == dep_ax and r >= 2:
+            b.append(f'const IndexT t{a}_b = t{a}_0 * DEPTH;')
+        else:
+            b.append(f'const IndexT t{a} = t{a}_0;')
+            conds.append(f't{a} < cnt{a}')
+    if r >= 2:
+        b.append('#pragma unroll')
+        b.append('for (int dd = 0; dd < DEPTH; dd++) {')
+        inner = [f'const IndexT t{dep_ax} = t{dep_ax}_b + dd;']
+        conds2 = conds + [f't{dep_ax} < cnt{dep_ax}']
+    else:
+        inner = []
+        conds2 = conds
+    inner.append(f'if ({" && ".join(conds2)}) {{')
+    for i in range(r):
+        inner.append(IND + var_assign(L, i, f't{i}', f'lo{i}', f'st{i}'))
+    inner.append(indent('{\n' + indent(L.body + '\n', 1) + '}', 1))
+    inner.append('}')
+    if r >= 2:
+        b += [IND + ln for ln in '\n'.join(inner).split('\n')]
+        b.append('}')
+    else:
+        b += inner
+    b += red_kernel_tail(L, 'blockIdx.x', accumulate=False)
+    k.append(indent('\n'.join(b) + '\n', 1).rstrip('\n'))
+    k.append('}')
+    return '\n'.join(k) + '\n'
+
+
+def parallel_handler(L):
+    r = L.rank
+    h = [f'extern "C" void {L.name}_cuda(DvmType *pLoopRef{"".join(", DvmType " + p[1] + "_hdr[]" if p[0] == "array" else ", " + p[2] + " *" + p[1] + "_ptr" for p in L.params)}) {{']
+    b = handler_prologue(L, 'DVMH_AMD_KIND_PARALLEL')
+    for p in L.arrays:
+        b.append(f'if ({p[1]}_tfm != 0) {{ fprintf(stderr, "{L.name}: array {p[1]} is not in natural layout\\n"); abort(); }}')
+    b.append('int depth = (int)tune[DVMH_AMD_T_DEPTH];')
+    b.append('if (depth != 1 && depth != 2 && depth != 4 && depth != 8) depth = 1;')
+    for i in range(r):
+        b.append(f'DvmType cnt{i} = (boundsHigh[{i}] - boundsLow[{i}]) / loopSteps[{i}] + 1;')
+    ax_x, ax_y, ax_z = r - 1, (r - 2 if r >= 2 else None), (r - 3 if r >= 3 else None)
+    dep_ax = ax_z if ax_z is not None else (ax_y if ax_y is not None else ax_x)
+    b.append(f'DvmType nbx = (cnt{ax_x} + threads.x - 1) / threads.x;')
+    if ax_y is not None:
+        dy = ' * depth' if dep_ax == ax_y else ''
+        b.append(f'DvmType nby = (cnt{ax_y} + threads.y{dy} - 1) / (threads.y{dy});')
+    else:
+        b.append('DvmType nby = 1;')
+    if ax_z is not None:
+        b.append(f'DvmType nbz = (cnt{ax_z} + threads.z * depth - 1) / (threads.z * depth);')
+    else:
+        b.append('DvmType nbz = 1;')
+    nb = 'nbx * nby * nbz' + ''.join(f' * cnt{a}' for a in range(0, r - 3))
+    b.append(f'DvmType nblocks = {nb};')
+    b += handler_reds_prepare(L, 'nblocks', 0)
+    kernels = []
+    for it in ['int', 'long long']:
+        for dp in [1, 2, 4, 8]:
+            cond = f'kernelIndexT == {"rt_INT" if it == "int" else "rt_LLONG"} && depth == {dp}'
+            kernels.append((cond, f'{L.name}_kernel<{it}, {dp}>'))
+    b.append('if (nblocks > 0) {')
+    for it in ['int', 'long long']:
+        args = array_fact_args(L, it)
+        for i in range(r):
+            args += [f'({it})boundsLow[{i}]', f'({it})cnt{i}', f'({it})loopSteps[{i}]']
+        args += [f'({it})nbx', f'({it})nby', f'({it})nbz', 'nblocks', '(int)tune[DVMH_AMD_T_XCD]']
+        args += red_fact_args(L)
+        ks = [(c, k) for c, k in kernels if (('rt_INT' in c) == (it == 'int'))]
+        b += [IND + s for s in launch_dispatch(ks, args, 'dim3((unsigned)nblocks)')]
+    b.append('}')
+    b += handler_reds_finish(L)
+    h.append(indent('\n'.join(b) + '\n', 1).rstrip('\n'))
+    h.append('}')
+    return '\n'.join(h) + '\n'
+
+
+# ----------------------------------------------------------------- GPU: LINE (one dependent axis)
+def line_kernel(L, dep):
+    """Kernel for a loop whose only dependent loop axis is `dep` (0-based)."""
+    r = L.rank
+    ind = [a for a in range(r) if a != dep]
+    fp, setup = kernel_array_params(L, unit=False)
+    for i in range(r):
+        fp += [f'IndexT lo{i}', f'IndexT cnt{i}', f'IndexT st{i}']
+    fp += ['IndexT nbx', 'IndexT nby', 'long nblocks', 'int xcd', 'int chunk']
+    fp += red_kernel_params(L)
+    k = [f'template <typename IndexT>',
+         f'__global__ __launch_bounds__(1024) void {L.name}_line{dep}_kernel({", ".join(fp)}) {{']
+    b = list(setup)
+    for v, ct in L.vars:
+        b.append(f'{ct} {v};')
+    for pr in L.privates:
+        b.append(pr + ';')
+    b += red_kernel_locals(L)
+    b.append('long tile = dvmh_dev::xcdTile((long)blockIdx.x, nblocks, xcd);')
+    b.append('const IndexT bx = (IndexT)(tile % nbx); tile /= nbx;')
+    b.append('const IndexT by = (IndexT)(tile % nby); tile /= nby;')
+    ax_x = ind[-1] if ind else None
+    ax_y = ind[-2] if len(ind) >= 2 else None
+    outer = ind[:-2] if len(ind) > 2 else []
+    for a in reversed(outer):
+        b.append(f'const IndexT t{a} = (IndexT)(tile % cnt{a}); tile /= cnt{a};')
+    conds = []
+    if ax_x is not None:
+        b.append(f'const IndexT t{ax_x} = bx * (IndexT)blockDim.x + (IndexT)threadIdx.x;')
+        conds.append(f't{ax_x} < cnt{ax_x}')
+    if ax_y is not None:
+        b.append(f'const IndexT t{ax_y} = by * (IndexT)blockDim.y + (IndexT)threadIdx.y;')
+        conds.append(f't{ax_y} < cnt{ax_y}')
+    b.append(f'if ({" && ".join(conds) if conds else "threadIdx.x == 0"}) {{')
+    inner = []
+    for a in ind:
+        inner.append(var_assign(L, a, f't{a}', f'lo{a}', f'st{a}'))
+    inner.append(f'for (IndexT t{dep} = 0; t{dep} < cnt{dep}; t{dep}++) {{')
+    inner.append(IND + var_assign(L, dep, f't{dep}', f'lo{dep}', f'st{dep}'))
+    inner.append(indent('{\n' + indent(L.body + '\n', 1) + '}', 1))
+    inner.append('}')
+    b += [IND + ln for ln in '\n'.join(inner).split('\n')]
+    b.append('}')
+    b += red_kernel_tail(L, 'blockIdx.x', accumulate=False)
+    k.append(indent('\n'.join(b) + '\n', 1).rstrip('\n'))
+    k.append('}')
+    return '\n'.join(k) + '\n'
+
+
+def line_launch(L, dep):
+    r = L.rank
+    ind = [a for a in range(r) if a != dep]
+    b = []
+    ax_x = ind[-1] if ind else None
+    ax_y = ind[-2] if len(ind) >= 2 else None
+    b.append(f'DvmType nbx = {"(cnt%d + threads.x - 1) / threads.x" % ax_x if ax_x is not None else "1"};')
+    b.append(f'DvmType nby = {"(cnt%d + threads.y - 1) / threads.y" % ax_y if ax_y is not None else "1"};')
+    nb = 'nbx * nby' + ''.join(f' * cnt{a}' for a in ind[:-2])
+    b.append(f'DvmType nblocks = {nb};')
+    b += handler_reds_prepare(L, 'nblocks', 0)
+    b.append('if (nblocks > 0) {')
+    for it in ['int', 'long long']:
+        args = array_fact_args(L, it)
+        for i in range(r):
+            args += [f'({it})boundsLow[{i}]', f'({it})cnt{i}', f'({it})loopSteps[{i}]']
+        args += [f'({it})nbx', f'({it})nby', 'nblocks', '(int)tune[DVMH_AMD_T_XCD]', '(int)tune[DVMH_AMD_T_CHUNK]']
+        args += red_fact_args(L)
+        cond = f'kernelIndexT == {"rt_INT" if it == "int" else "rt_LLONG"}'
+        b += [IND + s for s in launch_dispatch([(cond, f'{L.name}_line{dep}_kernel<{it}>')], args, 'dim3((unsigned)nblocks)')]
+    b.append('}')
+    b += handler_reds_finish(L)
+    return b
+
+
+# ----------------------------------------------------------------- GPU: WAVEFRONT (>= 2 dependent axes)
+def wave_kernel(L, deps):
+    """Hyperplane kernel: iteration-space points with sum of dependent-axis
+    positions equal to `plane`; threads over the first (m-1) dependent axes and
+    the independent axes; the last dependent position is implied."""
+    r = L.rank
+    ind = [a for a in range(r) if a not in deps]
+    last = deps[-1]
+    free = deps[:-1]
+    fp, setup = kernel_diag_array_params(L)
+    for i in range(r):
+        fp += [f'IndexT lo{i}', f'IndexT cnt{i}', f'IndexT st{i}']
+    fp += ['IndexT plane', 'IndexT f0lo', 'IndexT nfx', 'IndexT nfy', 'long nblocks']
+    fp += red_kernel_params(L)
+    tag = ''.join(map(str, deps))
+    k = [f'template <typename IndexT>',
+         f'__global__ __launch_bounds__(1024) void {L.name}_wave{tag}_kernel({", ".join(fp)}) {{']
+    b = list(setup)
+    for v, ct in L.vars:
+        b.append(f'{ct} {v};')
+    for pr in L.privates:
+        b.append(pr + ';')
+    b += red_kernel_locals(L)
+    # thread x <- the innermost free dependent axis, y <- the next free one, the rest
+    # (independent axes, further free axes) from blockIdx.y/z decomposition
+    fx = free[-1]
+    fy = free[-2] if len(free) >= 2 else None
+    rest = (free[:-2] if len(free) > 2 else []) + ind
+    b.append(f'long tile = (long)blockIdx.x;')
+    b.append(f'const IndexT bx = (IndexT)(tile % nfx); tile /= nfx;')
+    b.append(f'const IndexT by = (IndexT)(tile % nfy); tile /= nfy;')
+    for a in reversed(rest):
+        b.append(f'const IndexT t{a} = (IndexT)(tile % cnt{a}); tile /= cnt{a};')
+    b.append(f'const IndexT t{fx} = bx * (IndexT)blockDim.x + (IndexT)threadIdx.x + {"f0lo" if fy is None else "0"};')
+    if fy is not None:
+        b.append(f'const IndexT t{fy} = by * (IndexT)blockDim.y + (IndexT)threadIdx.y + f0lo;')
+    tsum = ' - '.join([f'plane'] + [f't{a}' for a in free])
+    b.append(f'const IndexT t{last} = {tsum};')
+    conds = [f't{a} < cnt{a}' for a in free] + [f't{last} >= 0', f't{last} < cnt{last}']
+    b.append(f'if ({" && ".join(conds)}) {{')
+    inner = []
+    for i in range(r):
+        inner.append(var_assign(L, i, f't{i}', f'lo{i}', f'st{i}'))
+    inner.append('{\n' + indent(L.body + '\n', 1) + '}')
+    b += [IND + ln for ln in '\n'.join(inner).split('\n')]
+    b.append('}')
+    b += red_kernel_tail(L, 'blockIdx.x', accumulate=True)
+    k.append(indent('\n'.join(b) + '\n', 1).rstrip('\n'))
+    k.append('}')
+    return '\n'.join(k) + '\n'
+
+
+def wave_launch(L, deps):
+    r = L.rank
+    ind = [a for a in range(r) if a not in deps]
+    last = deps[-1]
+    free = deps[:-1]
+    fx = free[-1]
+    fy = free[-2] if len(free) >= 2 else None
+    rest = (free[:-2] if len(free) > 2 else []) + ind
+    tag = ''.join(map(str, deps))
+    b = []
+    b.append('DvmType planes = 1' + ''.join(f' + (cnt{a} - 1)' for a in deps) + ';')
+    # slot per workgroup of the largest launch: bounding box of the free axes
+    b.append(f'DvmType maxFx = (cnt{fx} + threads.x - 1) / threads.x;')
+    b.append(f'DvmType maxFy = {"(cnt%d + threads.y - 1) / threads.y" % fy if fy is not None else "1"};')
+    b.append('DvmType slots = maxFx * maxFy' + ''.join(f' * cnt{a}' for a in rest) + ';')
+    b += handler_reds_prepare(L, 'slots', 1)
+    b.append('for (DvmType plane = 0; plane < planes; plane++) {')
+    # the first-free-axis range that can meet the plane
+    lead = fy if fy is not None else fx
+    others = [a for a in deps if a != lead]
+    b.append(IND + f'DvmType f0lo = plane - (0' + ''.join(f' + cnt{a} - 1' for a in others) + ');')
+    b.append(IND + 'if (f0lo < 0) f0lo = 0;')
+    b.append(IND + f'DvmType f0hi = plane < cnt{lead} - 1 ? plane : cnt{lead} - 1;')
+    b.append(IND + 'if (f0hi < f0lo) continue;')
+    if fy is not None:
+        b.append(IND + f'DvmType nfx = (cnt{fx} + threads.x - 1) / threads.x;')
+        b.append(IND + 'DvmType nfy = (f0hi - f0lo + 1 + threads.y - 1) / threads.y;')
+    else:
+        b.append(IND + 'DvmType nfx = (f0hi - f0lo + 1 + threads.x - 1) / threads.x;')
+        b.append(IND + 'DvmType nfy = 1;')
+    b.append(IND + 'DvmType nblocks = nfx * nfy' + ''.join(f' * cnt{a}' for a in rest) + ';')
+    for it in ['int', 'long long']:
+        args = array_fact_args(L, it, diag=True)
+        for i in range(r):
+            args += [f'({it})boundsLow[{i}]', f'({it})cnt{i}', f'({it})loopSteps[{i}]']
+        args += [f'({it})plane', f'({it})f0lo', f'({it})nfx', f'({it})nfy', 'nblocks']
+        args += red_fact_args(L)
+        cond = f'kernelIndexT == {"rt_INT" if it == "int" else "rt_LLONG"}'
+        b += [IND + s for s in launch_dispatch([(cond, f'{L.name}_wave{tag}_kernel<{it}>')], args, 'dim3((unsigned)nblocks)')]
+    b.append('}')
+    b += handler_reds_finish(L)
+    return b
+
+
+def across_handler(L):
+    r = L.rank
+    ndep = len(L.dep_axes())
+    h = [f'extern "C" void {L.name}_cuda(DvmType *pLoopRef{"".join(", DvmType " + p[1] + "_hdr[]" if p[0] == "array" else ", " + p[2] + " *" + p[1] + "_ptr" for p in L.params)}) {{']
+    b = ['DvmType dependency_mask = dvmh_loop_get_dependency_mask_C(*pLoopRef);',
+         'int dependency_num = __builtin_popcountl((unsigned long)dependency_mask);',
+         f'DvmType kind = dependency_num <= 1 ? DVMH_AMD_KIND_LINE : DVMH_AMD_KIND_WAVEFRONT;']
+    pro = handler_prologue(L, 'kind', want_ext=True, autotfm=True)
+    b += pro
+    for p in L.arrays:
+        _, nm, ct, ar = p
+        for it in ['int', 'long long']:
+            t = it.replace(' ', '_')
+            b.append(f'dvmh_dev::DiagInfo<{it}> {nm}_di_{t};')
+            b.append(f'{nm}_di_{t}.xAxis = {nm}_tfm == 2 ? (int){nm}_devHdr[64 + 0] : 0;')
+            b.append(f'{nm}_di_{t}.yAxis = {nm}_tfm == 2 ? (int){nm}_devHdr[64 + 3] : 0;')
+            b.append(f'{nm}_di_{t}.slash = {nm}_tfm == 2 ? (int){nm}_devHdr[64 + 6] : 0;')
+            b.append(f'{nm}_di_{t}.xFirst = ({it})({nm}_tfm == 2 ? {nm}_devHdr[64 + 1] : 0);')
+            b.append(f'{nm}_di_{t}.xLen = ({it})({nm}_tfm == 2 ? {nm}_devHdr[64 + 2] : 1);')
+            b.append(f'{nm}_di_{t}.yFirst = ({it})({nm}_tfm == 2 ? {nm}_devHdr[64 + 4] : 0);')
+            b.append(f'{nm}_di_{t}.yLen = ({it})({nm}_tfm == 2 ? {nm}_devHdr[64 + 5] : 1);')
+    for i in range(r):
+        b.append(f'DvmType cnt{i} = (boundsHigh[{i}] - boundsLow[{i}]) / loopSteps[{i}] + 1;')
+    first = True
+    # one dependent axis: line kernels for each possible axis
+    for dep in range(r):
+        mask = 1 << (r - 1 - dep)
+        kw = 'if' if first else '} else if'
+        first = False
+        b.append(f'{kw} (dependency_mask == {mask}) {{')
+        for p in L.arrays:
+            b.append(IND + f'if ({p[1]}_tfm == 2) {{ fprintf(stderr, "{L.name}: diagonal layout in a line sweep\\n"); abort(); }}')
+        b += [IND + s for s in line_launch(L, dep)]
+    # two or more dependent axes: hyperplane kernels for every mask with >= 2 bits
+    for m in range(2, r + 1):
+        for deps in itertools.combinations(range(r), m):
+            mask = sum(1 << (r - 1 - a) for a in deps)
+            b.append(f'}} else if (dependency_mask == {mask}) {{')
+            b += [IND + s for s in wave_launch(L, list(deps))]
+    b.append('} else {')
+    b.append(IND + f'fprintf(stderr, "{L.name}: unsupported dependency mask %ld\\n", (long)dependency_mask);')
+    b.append(IND + 'abort();')
+    b.append('}')
+    h.append(indent('\n'.join(b) + '\n', 1).rstrip('\n'))
+    h.append('}')
+    return '\n'.join(h) + '\n'
+
+
+def gpu_loop(L):
+    text = []
+    if L.across:
+        r = L.rank
+        for dep in range(r):
+            text.append(line_kernel(L, dep))
+        for m in range(2, r + 1):
+            for deps in itertools.combinations(range(r), m):
+                text.append(wave_kernel(L, list(deps)))
+        text.append(across_handler(L))
+    else:
+        text.append(parallel_kernel(L))
+        text.append(parallel_handler(L))
+    return '\n'.join(text)
+
+
+# ----------------------------------------------------------------- driver
+def emit(spec_path, outdir):
+    with open(spec_path) as f:
+        spec = ast.literal_eval(f.read())
+    prog = spec['program']
+    loops = [Loop(d, prog) for d in spec['loops']]
+    inc = spec.get('include')
+    host = ['/* Host handlers generated from ' + os.path.basename(spec_path) + ' by emit/hipgen.py. */',
+            '#include <math.h>', '#include <stdio.h>', '#include <stdlib.h>',
+            '#ifdef _OPENMP', '#include <omp.h>', '#endif', '#include "dvmhlib2.h"']
+    if inc:
+        host.append(f'#include "{inc}"')
+    host.append('')
+    for L in loops:
+        host.append(host_handler(L))
+    hip = ['// GPU handlers generated from ' + os.path.basename(spec_path) + ' by emit/hipgen.py (gfx950).',
+           '#include <hip/hip_runtime.h>', '#include <math.h>', '#include <stdio.h>', '#include <stdlib.h>',
+           '#include "dvmhlib2.h"', '#include "dvmh_amd.h"', '#include "dvmh_hip_device.h"']
+    if inc:
+        hip.append(f'#include "{inc}"')
+    hip.append('')
+    for L in loops:
+        hip.append(gpu_loop(L))
+    os.makedirs(outdir, exist_ok=True)
+    with open(os.path.join(outdir, f'{prog}_host.c'), 'w') as f:
+        f.write('\n'.join(host))
+    with open(os.path.join(outdir, f'{prog}_hip.hip'), 'w') as f:
+        f.write('\n'.join(hip))
+
+
+if __name__ == '__main__':
+    if len(sys.argv) != 3:
+        sys.exit('usage: hipgen.py <program.loops> <outdir>')
+    emit(sys.argv[1], sys.argv[2])

@@ -1,0 +1,91 @@
+/*
+ * dvmh_amd.h -- entry points of libdvmh_amd that have no counterpart in the
+ * reference interface.  Emitted HIP handlers use the tuning query; programs
+ * and harnesses use batching, statistics and the tuner/cost-model controls.
+ * Everything here is extern "C" with plain pointers and integers.
+ */
+#ifndef DVMH_AMD_H
+#define DVMH_AMD_H
+
+#include "dvmhlib_types.h"
+
+#pragma GCC visibility push(default)
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- launch tuning beyond the thread block --------------------------------
+ * An emitted handler first calls dvmh_loop_cuda_get_config_C (block, stream),
+ * then this, naming the kind of its kernel family.  The runtime answers with
+ * the configuration chosen by its tuner (default, exhaustive sweep, or the
+ * learned cost model); the answer never changes arithmetic, only the mapping
+ * of iterations to threads. */
+#define DVMH_AMD_TUNE_N 8
+#define DVMH_AMD_KIND_PARALLEL 0   /* independent iterations, one kernel */
+#define DVMH_AMD_KIND_LINE 1       /* ACROSS with one dependent axis: line sweeps */
+#define DVMH_AMD_KIND_WAVEFRONT 2  /* ACROSS with >= 2 dependent axes: tiled wavefront */
+/* tune[] slots */
+#define DVMH_AMD_T_DEPTH 0     /* iterations of the slowest mapped axis per thread */
+#define DVMH_AMD_T_XCD 1       /* 1: XCD-aware block-to-tile order */
+#define DVMH_AMD_T_VARIANT 2   /* kernel variant (0 = direct global-memory body) */
+#define DVMH_AMD_T_CHUNK 3     /* LINE: steps per LDS window; WAVEFRONT: tile depth */
+#define DVMH_AMD_T_TILE_A 4    /* WAVEFRONT: tile extent on the first dependent axis */
+#define DVMH_AMD_T_TILE_B 5    /* WAVEFRONT: tile extent on the second dependent axis */
+#define DVMH_AMD_T_TRANSFORM 6 /* 1: arrays may be re-laid out for this loop */
+#define DVMH_AMD_T_RESERVED 7
+void dvmh_amd_loop_get_tuning_C(DvmType curLoop, DvmType kind, DvmType tune[]);
+
+/* ---- statistics ------------------------------------------------------------
+ * out[0] loops run on the host, out[1] loops run on the GPU,
+ * out[2] GPU loop time in nanoseconds (sum of handler stream time when the
+ * tuner is timing), out[3] GPU present (0/1), out[4] compute units,
+ * out[5] graph replays, out[6] batched groups, out[7] batched loops. */
+void dvmh_amd_get_stats_C(DvmType out[8]);
+void dvmh_amd_reset_stats_C(void);
+/* Device name / architecture string of the GPU in use ("" without one). */
+const char *dvmh_amd_device_name_C(void);
+
+/* ---- batching of many small loops -------------------------------------------
+ * Between begin and end, dvmh_loop_perform_C of loops that registered a batch
+ * handler queues them; end groups queued loops by batch handler and calls each
+ * once with all loop references, so one fused launch covers many grids. */
+void dvmh_amd_batch_begin_C(void);
+void dvmh_amd_batch_end_C(void);
+void dvmh_amd_loop_register_batch_handler_C(DvmType curLoop, DvmType handlerHelper);
+/* Inside a batch handler: number of loops and the i-th loop reference. */
+DvmType dvmh_amd_batch_count_C(DvmType batchRef);
+DvmType dvmh_amd_batch_loop_C(DvmType batchRef, DvmType i);
+/* Inside a batch handler: finish reduction redIndex of every loop in the batch.
+ * partials holds per-loop segments [offsets[i], offsets[i+1]) of values (and
+ * locs) in device memory; one kernel folds each segment and the results are
+ * posted to each loop's reduction variable. */
+void dvmh_amd_batch_red_finish_C(DvmType batchRef, DvmType redIndex, const void *partials, const void *locPartials,
+        const DvmType *offsets);
+/* Device scratch for batch handlers: bytes of device memory kept by the runtime
+ * until the batch handler returns. */
+void *dvmh_amd_batch_scratch_C(DvmType batchRef, DvmType bytes);
+
+/* ---- region graphs -----------------------------------------------------------
+ * A region created with this flag bit is executed lazily: its loops are recorded
+ * and launched at dvmh_region_end_C; the first repetition with identical
+ * launches is captured into a hipGraph and later repetitions replay it. */
+#define REGION_AMD_GRAPH 256
+
+/* ---- tuner / learned cost model ---------------------------------------------
+ * mode: 0 default configuration, 1 exhaustive sweep then best, 2 cost model
+ * (falls back to the sweep for a loop the model has not seen), 3 fixed best
+ * from the records.  Records: one line per (loop signature, candidate, seconds). */
+void dvmh_amd_tuner_set_mode_C(DvmType mode);
+DvmType dvmh_amd_tuner_save_C(const char *path);
+DvmType dvmh_amd_tuner_load_C(const char *path);
+void dvmh_amd_tuner_clear_C(void);
+/* Best candidate so far for the loop site (file, line): out[0] candidate index,
+ * out[1] time in ns, out[2..9] block x,y,z and tune[0..4]; returns 0 if unknown. */
+DvmType dvmh_amd_tuner_query_C(const char *file, DvmType line, DvmType out[10]);
+
+#ifdef __cplusplus
+}
+#endif
+#pragma GCC visibility pop
+
+#endif
