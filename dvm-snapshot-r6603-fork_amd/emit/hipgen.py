@@ -167,7 +167,11 @@ def host_handler(L):
         if par < L.rank:
             for i in range(L.rank):
                 if i == par:
-                    lines.append(IND * i + '#pragma omp for schedule(runtime), nowait')
+                    # Under a sequential dependent axis every thread must get the
+                    # same chunk each time (the "nowait" relies on it), so ACROSS
+                    # loops use a static schedule; libgomp's runtime default is dynamic.
+                    sched = 'static' if L.across else 'runtime'
+                    lines.append(IND * i + f'#pragma omp for schedule({sched}), nowait')
                 lines.append(IND * i + loop_head(i))
             lines.append(indent(body, L.rank))
         else:

@@ -92,3 +92,102 @@ def run_jac2d(n, iters, seed=None, targets=D.DEVICE_TYPE_HOST | D.DEVICE_TYPE_CU
     P.fn('fetch', [_c_dp, _c_dp])(dp(A), dp(B))
     P.fn('teardown', [])()
     return {'A': A, 'B': B, 'eps': eps, 'seconds': t}
+
+
+def oracle_jac3d(n, iters, seed=None, threads=0):
+    seed = D.SEED_BASE + 2 if seed is None else seed
+    o = oracle()
+    A = np.zeros(n ** 3)
+    o.orc_init_grid3d(vp(A), n, seed, 0.0, 1)
+    B = A.copy()
+    eps = np.zeros(iters)
+    s = np.zeros(iters)
+    o.orc_jac3d(vp(A), vp(B), n, iters, vp(eps), vp(s), threads)
+    return {'A': A, 'B': B, 'eps': eps, 'sum': s}
+
+
+def oracle_sor3d(n, iters, w=1.5, seed=None, threads=0):
+    seed = D.SEED_BASE + 3 if seed is None else seed
+    o = oracle()
+    A = np.zeros(n ** 3)
+    o.orc_init_grid3d(vp(A), n, seed, 0.0, 1)
+    eps = np.zeros(iters)
+    o.orc_sor3d(vp(A), n, iters, w, vp(eps), threads)
+    return {'A': A, 'eps': eps}
+
+
+def oracle_adi3d(n, steps, c=0.3, seed=None, threads=0):
+    seed = D.SEED_BASE + 4 if seed is None else seed
+    o = oracle()
+    A = np.zeros(n ** 3)
+    o.orc_init_grid3d(vp(A), n, seed, 0.0, 0)
+    mx = np.zeros(steps)
+    loc = np.zeros(steps, dtype=np.int64)
+    o.orc_adi3d(vp(A), n, steps, c, vp(mx), vp(loc), threads)
+    return {'A': A, 'max': mx, 'loc': loc}
+
+
+def run_jac3d(n, iters, seed=None, targets=D.DEVICE_TYPE_HOST | D.DEVICE_TYPE_CUDA, flags=0, keep=False):
+    seed = D.SEED_BASE + 2 if seed is None else seed
+    P = _program('jac3d')
+    P.set_targets(targets, flags)
+    P.fn('setup', [ctypes.c_long, ctypes.c_ulonglong])(n, seed)
+    eps = np.zeros(iters)
+    s = np.zeros(iters)
+    t = P.fn('iterate', [ctypes.c_int, _c_dp, _c_dp], ctypes.c_double)(iters, dp(eps), dp(s))
+    A = np.zeros(n ** 3)
+    B = np.zeros(n ** 3)
+    P.fn('fetch', [_c_dp, _c_dp])(dp(A), dp(B))
+    P.fn('teardown', [])()
+    return {'A': A, 'B': B, 'eps': eps, 'sum': s, 'seconds': t}
+
+
+def run_sor3d(n, iters, w=1.5, seed=None, targets=D.DEVICE_TYPE_HOST | D.DEVICE_TYPE_CUDA, flags=0):
+    seed = D.SEED_BASE + 3 if seed is None else seed
+    P = _program('sor3d')
+    P.set_targets(targets, flags)
+    P.fn('setup', [ctypes.c_long, ctypes.c_double, ctypes.c_ulonglong])(n, w, seed)
+    eps = np.zeros(iters)
+    t = P.fn('iterate', [ctypes.c_int, _c_dp], ctypes.c_double)(iters, dp(eps))
+    A = np.zeros(n ** 3)
+    P.fn('fetch', [_c_dp])(dp(A))
+    P.fn('teardown', [])()
+    return {'A': A, 'eps': eps, 'seconds': t}
+
+
+def run_adi3d(n, steps, c=0.3, seed=None, targets=D.DEVICE_TYPE_HOST | D.DEVICE_TYPE_CUDA, flags=0):
+    seed = D.SEED_BASE + 4 if seed is None else seed
+    P = _program('adi3d')
+    P.set_targets(targets, flags)
+    P.fn('setup', [ctypes.c_long, ctypes.c_double, ctypes.c_ulonglong])(n, c, seed)
+    mx = np.zeros(steps)
+    loc = np.zeros(steps, dtype=np.int64)
+    t = P.fn('iterate', [ctypes.c_int, _c_dp, _c_lp], ctypes.c_double)(steps, dp(mx), loc.ctypes.data_as(_c_lp))
+    A = np.zeros(n ** 3)
+    P.fn('fetch', [_c_dp])(dp(A))
+    P.fn('teardown', [])()
+    return {'A': A, 'max': mx, 'loc': loc, 'seconds': t}
+
+
+def sum_close(got, ref, rtol=1e-12):
+    """SUM reductions: within rtol relative of the compensated oracle sum."""
+    got = np.asarray(got)
+    ref = np.asarray(ref)
+    return bool(np.all(np.abs(got - ref) <= rtol * np.abs(ref)))
+
+
+def run_adikat(n, itmax, maxeps=0.01, targets=D.DEVICE_TYPE_HOST | D.DEVICE_TYPE_CUDA):
+    P = _program('adikat')
+    P.set_targets(targets, 0)
+    eps = np.zeros(itmax)
+    done = P.fn('run', [ctypes.c_long, ctypes.c_int, ctypes.c_double, _c_dp], ctypes.c_int)(n, itmax, maxeps, dp(eps))
+    return eps[:done]
+
+
+def run_jackat(rank, n, itmax, maxeps=0.5, targets=D.DEVICE_TYPE_HOST | D.DEVICE_TYPE_CUDA):
+    P = _program('jackat')
+    P.set_targets(targets, 0)
+    eps = np.zeros(itmax, dtype=np.float32)
+    last = P.fn('run', [ctypes.c_int, ctypes.c_long, ctypes.c_int, ctypes.c_float, ctypes.POINTER(ctypes.c_float)],
+                ctypes.c_float)(rank, n, itmax, maxeps, eps.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+    return float(last), eps
