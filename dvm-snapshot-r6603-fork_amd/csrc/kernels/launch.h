@@ -14,11 +14,15 @@ extern "C" {
 void dvmh_hip_fill(void *dst, const void *value, long valueBytes, long count, hipStream_t s);
 
 // Fold n partial values (and their loc records) into one result.
-// func: rf_* code; rtType: rt_* element type.  Deterministic: fixed per-thread
-// order and fixed tree; on equal MAX/MIN values the lower partial index wins.
+// func: rf_* code; rtType: rt_* element type.  Deterministic: fixed partition,
+// fixed per-thread order and fixed tree; on equal MAX/MIN values the lower
+// partial index wins.
 // (replaces rts/dvmh/cuda_reduction.cu:168-312: no per-warp D2H and host fold)
+// Large n is folded in two passes (blocks of 16384 partials, then one
+// workgroup) through `scratch` of dvmh_hip_reduce_scratch_bytes(n) bytes.
+long dvmh_hip_reduce_scratch_bytes(long n);
 void dvmh_hip_reduce_finish(int func, int rtType, long n, const void *vals, const void *locs, long locSize,
-        void *outVal, void *outLoc, hipStream_t s);
+        void *outVal, void *outLoc, void *scratch, hipStream_t s);
 
 // Same for many independent segments: segment g is [offsets[g], offsets[g+1])
 // (offsets in device memory), result g goes to outVals[g] / outLocs[g].

@@ -63,10 +63,12 @@ __device__ inline void comb(int f, T &acc, long &accIdx, T v, long vIdx) {
     else combine<T>(f, acc, accIdx, v, vIdx);
 }
 
-// One workgroup folds vals[begin, end) -> outVal (+ the winner's loc record).
+// One workgroup folds vals[begin, end).  Element e of the range carries the
+// tie-break index idx ? idx[e] : e.  The result and the winner's index go to
+// outVal / outIdx (index -1 for an empty range).
 template <typename T, bool BITS>
-__device__ void foldRange(int f, long begin, long end, const T *__restrict__ vals, const char *__restrict__ locs,
-        long locSize, T *outVal, char *outLoc) {
+__device__ void foldRange(int f, long begin, long end, const T *__restrict__ vals, const long *__restrict__ idx, T *outVal,
+        long *outIdx) {
     __shared__ T sVal[FIN_THREADS];
     __shared__ long sIdx[FIN_THREADS];
     __shared__ int sHas[FIN_THREADS];
@@ -74,13 +76,25 @@ __device__ void foldRange(int f, long begin, long end, const T *__restrict__ val
     T acc = identityOf<T>(f);
     long accIdx = -1;
     int has = 0;
-    for (long i = begin + t; i < end; i += FIN_THREADS) {
+    // 4 independent loads in flight per thread, folded in index order
+    long i = begin + t;
+    for (; i + 3 * FIN_THREADS < end; i += 4 * FIN_THREADS) {
+        T v0 = vals[i], v1 = vals[i + FIN_THREADS], v2 = vals[i + 2 * FIN_THREADS], v3 = vals[i + 3 * FIN_THREADS];
+        long i0 = idx ? idx[i] : i, i1 = idx ? idx[i + FIN_THREADS] : i + FIN_THREADS;
+        long i2 = idx ? idx[i + 2 * FIN_THREADS] : i + 2 * FIN_THREADS, i3 = idx ? idx[i + 3 * FIN_THREADS] : i + 3 * FIN_THREADS;
+        if (!has) { acc = v0; accIdx = i0; has = 1; } else comb<T, BITS>(f, acc, accIdx, v0, i0);
+        comb<T, BITS>(f, acc, accIdx, v1, i1);
+        comb<T, BITS>(f, acc, accIdx, v2, i2);
+        comb<T, BITS>(f, acc, accIdx, v3, i3);
+    }
+    for (; i < end; i += FIN_THREADS) {
         T v = vals[i];
-        if (!has) { acc = v; accIdx = i; has = 1; }
-        else comb<T, BITS>(f, acc, accIdx, v, i);
+        long vi = idx ? idx[i] : i;
+        if (!has) { acc = v; accIdx = vi; has = 1; }
+        else comb<T, BITS>(f, acc, accIdx, v, vi);
     }
     sVal[t] = acc;
-    sIdx[t] = accIdx;
+    sIdx[t] = has ? accIdx : -1;
     sHas[t] = has;
     __syncthreads();
     for (int w = FIN_THREADS / 2; w > 0; w >>= 1) {
@@ -97,27 +111,44 @@ __device__ void foldRange(int f, long begin, long end, const T *__restrict__ val
         __syncthreads();
     }
     if (t == 0) {
-        T r = sHas[0] ? sVal[0] : identityOf<T>(f);
-        *outVal = r;
-    }
-    if (outLoc && locSize > 0 && sHas[0]) {
-        long w = sIdx[0];
-        for (long b = t; b < locSize; b += FIN_THREADS) outLoc[b] = locs[w * locSize + b];
+        *outVal = sHas[0] ? sVal[0] : identityOf<T>(f);
+        if (outIdx) *outIdx = sHas[0] ? sIdx[0] : -1;
     }
 }
 
+// Copies the winner's loc record; *winShared was written by thread 0.
+template <typename T>
+__device__ void copyLoc(const char *locs, long locSize, const long *winShared, char *outLoc) {
+    __syncthreads();
+    const long w = *winShared;
+    if (outLoc && locSize > 0 && w >= 0)
+        for (long b = threadIdx.x; b < locSize; b += blockDim.x) outLoc[b] = locs[w * locSize + b];
+}
+
+// pass 1: block b folds [b * chunk, (b + 1) * chunk)
 template <typename T, bool BITS>
-__global__ __launch_bounds__(FIN_THREADS) void finishKernel(int f, long n, const T *vals, const char *locs, long locSize,
-        T *outVal, char *outLoc) {
-    foldRange<T, BITS>(f, 0, n, vals, locs, locSize, outVal, outLoc);
+__global__ __launch_bounds__(FIN_THREADS) void chunkKernel(int f, long n, long chunk, const T *vals, T *outVals, long *outIdx) {
+    long b = blockIdx.x;
+    long e = (b + 1) * chunk < n ? (b + 1) * chunk : n;
+    foldRange<T, BITS>(f, b * chunk, e, vals, nullptr, outVals + b, outIdx + b);
+}
+
+// final pass over n values (with their original indices if idx != 0)
+template <typename T, bool BITS>
+__global__ __launch_bounds__(FIN_THREADS) void finishKernel(int f, long n, const T *vals, const long *idx, const char *locs,
+        long locSize, T *outVal, char *outLoc) {
+    __shared__ long win;
+    foldRange<T, BITS>(f, 0, n, vals, idx, outVal, &win);
+    copyLoc<T>(locs, locSize, &win, outLoc);
 }
 
 template <typename T, bool BITS>
 __global__ __launch_bounds__(FIN_THREADS) void segmentsKernel(int f, const long *offsets, const T *vals, const char *locs,
         long locSize, T *outVals, char *outLocs) {
+    __shared__ long win;
     const long g = blockIdx.x;
-    foldRange<T, BITS>(f, offsets[g], offsets[g + 1], vals, locs, locSize, outVals + g,
-            outLocs ? outLocs + g * locSize : nullptr);
+    foldRange<T, BITS>(f, offsets[g], offsets[g + 1], vals, nullptr, outVals + g, &win);
+    copyLoc<T>(locs, locSize, &win, outLocs ? outLocs + g * locSize : nullptr);
 }
 
 __global__ void fillBytesKernel(char *dst, const char *rec, long bytes, long count) {
@@ -136,10 +167,21 @@ __global__ void fillSmallKernel(char *dst, Rec64 rec, long bytes, long count) {
         dst[i] = r[i % bytes];
 }
 
+constexpr long FIN_CHUNK = 16 * FIN_THREADS;
+
 template <typename T, bool BITS>
-void launchFinish(int f, long n, const void *vals, const void *locs, long locSize, void *outVal, void *outLoc,
+void launchFinish(int f, long n, const void *vals, const void *locs, long locSize, void *outVal, void *outLoc, void *scratch,
         hipStream_t s) {
-    hipLaunchKernelGGL((finishKernel<T, BITS>), dim3(1), dim3(FIN_THREADS), 0, s, f, n, (const T *)vals,
+    if (n > FIN_CHUNK && scratch) {
+        long g = (n + FIN_CHUNK - 1) / FIN_CHUNK;
+        T *pv = (T *)scratch;
+        long *pi = (long *)((char *)scratch + ((g * sizeof(T) + 15) & ~15L));
+        hipLaunchKernelGGL((chunkKernel<T, BITS>), dim3((unsigned)g), dim3(FIN_THREADS), 0, s, f, n, FIN_CHUNK, (const T *)vals, pv, pi);
+        hipLaunchKernelGGL((finishKernel<T, BITS>), dim3(1), dim3(FIN_THREADS), 0, s, f, g, (const T *)pv, (const long *)pi,
+                (const char *)locs, locSize, (T *)outVal, (char *)outLoc);
+        return;
+    }
+    hipLaunchKernelGGL((finishKernel<T, BITS>), dim3(1), dim3(FIN_THREADS), 0, s, f, n, (const T *)vals, (const long *)nullptr,
             (const char *)locs, locSize, (T *)outVal, (char *)outLoc);
 }
 
@@ -163,9 +205,14 @@ void launchSegments(int f, long groups, const long *offsets, const void *vals, c
         default: break;                                                                    \
     }
 
+extern "C" long dvmh_hip_reduce_scratch_bytes(long n) {
+    long g = (n + FIN_CHUNK - 1) / FIN_CHUNK;
+    return n > FIN_CHUNK ? g * 24 + 64 : 0;
+}
+
 extern "C" void dvmh_hip_reduce_finish(int func, int rtType, long n, const void *vals, const void *locs, long locSize,
-        void *outVal, void *outLoc, hipStream_t s) {
-    DISPATCH_TYPES(launchFinish, func, n, vals, locs, locSize, outVal, outLoc, s)
+        void *outVal, void *outLoc, void *scratch, hipStream_t s) {
+    DISPATCH_TYPES(launchFinish, func, n, vals, locs, locSize, outVal, outLoc, scratch, s)
 }
 
 extern "C" void dvmh_hip_reduce_segments(int func, int rtType, long groups, const long *offsets, const void *vals,

@@ -769,10 +769,11 @@ extern "C" void dvmh_region_end_C(DvmType curRegion) {   // :3122
             for (auto &rs : r->scalars)
                 if (rs.intent & INTENT_OUT) gpu.d2h(rs.data->host, rs.data->dev.ptr, (size_t)rs.data->typeSize);
         }
+        if (!r->deferred.empty()) graphRunRegion(r);
         if (settings.syncMode == 1 && !(r->flags & REGION_ASYNC)) flushDeferredPosts();
     }
     if (rt.curRegion == r) rt.curRegion = nullptr;
-    delete r;
+    if (!batchKeepsRegion(r)) delete r;   // else freed when its queued loops have run
 }
 
 // ============================================================ loops
@@ -911,15 +912,20 @@ extern "C" void dvmh_loop_register_handler_C(DvmType curLoop, DvmType deviceType
     else fatal("error", "Unknown device type %ld in dvmh_loop_register_handler_C", deviceType);
 }
 
+extern "C" void dvmh_amd_loop_register_batch_handler_C(DvmType curLoop, DvmType handlerHelper) {
+    Guard g;
+    Loop *l = asLoop(curLoop, "dvmh_amd_loop_register_batch_handler_C");
+    l->batchH = loadHandler(handlerHelper);
+    l->hasBatch = true;
+}
+
 extern "C" void dvmh_loop_perform_C(DvmType curLoop) {   // :3769
     Guard g;
     Loop *l = asLoop(curLoop, "dvmh_loop_perform_C");
     saved.clear();
     performLoop(l);
-    // a batched loop is owned by the batch until it ends
-    if (l->magic == OBJ_MAGIC && !l->spec) {
-        if (!loopIsQueued(l)) delete l;
-    }
+    // a queued loop is owned by its lazy region or batch until it runs
+    if (!l->queued) delete l;
 }
 
 // ============================================================ handler queries

@@ -1,8 +1,20 @@
 // Batching of many small loops into one fused launch per loop site, and the
 // host slot count used by host handlers.
+//
+// Between dvmh_amd_batch_begin_C and dvmh_amd_batch_end_C, GPU loops that
+// registered a batch handler are queued.  At the end the queue is split into
+// groups, in program order: a loop joins the most recent group of its batch
+// handler when it shares no array with any loop of that group or of a later
+// group (so moving it earlier cannot change results); otherwise it starts a new
+// group.  Each group runs once: a lone loop through its ordinary handler, a
+// larger group through the batch handler, which sees every loop's bounds and
+// headers and launches one persistent kernel over all of their tiles.
 #include <algorithm>
+#include <cstring>
+#include <set>
 
 #include "rt.h"
+#include "../kernels/launch.h"
 
 #ifdef _OPENMP
 #include <omp.h>
@@ -13,13 +25,39 @@ namespace dvmh {
 long graphReplays = 0, batchGroups = 0, batchLoops = 0;
 static bool batching = false;
 static std::vector<Loop *> queued;
+static std::vector<Region *> endedRegions;   // regions that ended while loops were queued
 
-bool batchQueue(Loop *loop) {
-    (void)loop;
-    return false;
+constexpr uint32_t BATCH_MAGIC = 0x42415443u;   // "BATC"
+
+struct BatchCtx {
+    uint32_t magic = BATCH_MAGIC;
+    std::vector<Loop *> loops;
+    std::vector<SpecLoop *> specs;
+    std::vector<void *> scratch;
+};
+
+static BatchCtx *asBatch(DvmType ref, const char *where) {
+    BatchCtx *b = (BatchCtx *)ref;
+    DVMH_CHECK(b && b->magic == BATCH_MAGIC, "Invalid batch reference passed to %s", where);
+    return b;
 }
 
-bool loopIsQueued(Loop *loop) { return std::find(queued.begin(), queued.end(), loop) != queued.end(); }
+bool batchQueue(Loop *loop) {
+    if (!batching || !loop->hasBatch) return false;
+    loop->queued = true;
+    if (loop->region) loop->region->pendingBatch++;
+    queued.push_back(loop);
+    return true;
+}
+
+bool loopIsQueued(Loop *loop) { return loop->queued; }
+
+// Called by dvmh_region_end_C: a region with queued loops lives until they ran.
+bool batchKeepsRegion(Region *r) {
+    if (r->pendingBatch == 0) return false;
+    endedRegions.push_back(r);
+    return true;
+}
 
 int hostSlotCount() {
     if (settings.hostThreads > 0) return settings.hostThreads;
@@ -30,9 +68,171 @@ int hostSlotCount() {
 #endif
 }
 
+static std::set<const Data *> arraysOf(const Loop *l) {
+    std::set<const Data *> s;
+    if (l->region)
+        for (auto &ra : l->region->arrays) s.insert(ra.data);
+    return s;
+}
+
+static bool disjoint(const std::set<const Data *> &a, const std::set<const Data *> &b) {
+    for (const Data *d : a)
+        if (b.count(d)) return false;
+    return true;
+}
+
+void finishLoopReductionsExt(Loop *loop);
+
+static void runBatchGroup(std::vector<Loop *> &group) {
+    BatchCtx ctx;
+    ctx.loops = group;
+    for (Loop *l : group) {
+        for (auto &r : l->reds) r->initValues(r->arrayAddr, r->locAddr);
+        SpecLoop *sl = new SpecLoop(l, 1);
+        sl->stream = gpu.stream;
+        l->spec = sl;
+        ctx.specs.push_back(sl);
+    }
+    DvmType ref = (DvmType)&ctx;
+    callHandler(group[0]->batchH, &ref);
+    rt.loopsOnGpu += (long)group.size();
+    batchGroups++;
+    batchLoops += (long)group.size();
+    for (size_t i = 0; i < group.size(); i++) {
+        Loop *l = group[i];
+        for (void *p : ctx.specs[i]->toFree) gpu.release(p);
+        finishLoopReductionsExt(l);
+        l->spec = nullptr;
+        delete ctx.specs[i];
+    }
+    for (void *p : ctx.scratch) gpu.release(p);
+    ctx.magic = 0;
+}
+
+static void releaseLoop(Loop *l) {
+    Region *r = l->region;
+    delete l;
+    if (r && --r->pendingBatch == 0) {
+        auto it = std::find(endedRegions.begin(), endedRegions.end(), r);
+        if (it != endedRegions.end()) {
+            endedRegions.erase(it);
+            delete r;
+        }
+    }
+}
+
+static void runQueue() {
+    struct Group {
+        DvmHandlerFunc func;
+        std::vector<Loop *> loops;
+        std::set<const Data *> arrays;
+    };
+    std::vector<Group> groups;
+    for (Loop *l : queued) {
+        std::set<const Data *> mine = arraysOf(l);
+        int target = -1;
+        for (int g = (int)groups.size() - 1; g >= 0; g--)
+            if (groups[g].func == l->batchH.func) { target = g; break; }
+        if (target >= 0) {
+            bool ok = disjoint(mine, groups[target].arrays);
+            for (size_t h = target + 1; h < groups.size() && ok; h++) ok = disjoint(mine, groups[h].arrays);
+            if (!ok) target = -1;
+        }
+        if (target < 0) {
+            groups.push_back(Group{l->batchH.func, {}, {}});
+            target = (int)groups.size() - 1;
+        }
+        groups[target].loops.push_back(l);
+        groups[target].arrays.insert(mine.begin(), mine.end());
+    }
+    queued.clear();
+    for (auto &g : groups) {
+        if (g.loops.size() == 1) {
+            Loop *l = g.loops[0];
+            l->queued = false;
+            performLoopNow(l);
+        } else {
+            runBatchGroup(g.loops);
+        }
+        for (Loop *l : g.loops) releaseLoop(l);
+        if (settings.syncMode == 0) flushDeferredPosts();
+    }
+}
+
 }  // namespace dvmh
 
 using namespace dvmh;
 
-extern "C" void dvmh_amd_batch_begin_C(void) { batching = true; }
-extern "C" void dvmh_amd_batch_end_C(void) { batching = false; }
+extern "C" void dvmh_amd_batch_begin_C(void) {
+    DVMH_CHECK(!batching, "dvmh_amd_batch_begin_C: a batch is already open");
+    batching = true;
+}
+
+extern "C" void dvmh_amd_batch_end_C(void) {
+    DVMH_CHECK(batching, "dvmh_amd_batch_end_C without dvmh_amd_batch_begin_C");
+    batching = false;
+    runQueue();
+}
+
+extern "C" DvmType dvmh_amd_batch_count_C(DvmType batchRef) {
+    return (DvmType)asBatch(batchRef, "dvmh_amd_batch_count_C")->loops.size();
+}
+
+extern "C" DvmType dvmh_amd_batch_loop_C(DvmType batchRef, DvmType i) {
+    BatchCtx *b = asBatch(batchRef, "dvmh_amd_batch_loop_C");
+    DVMH_CHECK(i >= 0 && i < (DvmType)b->loops.size(), "Batch loop index %ld out of range", i);
+    return (DvmType)b->specs[i];
+}
+
+extern "C" void *dvmh_amd_batch_param_C(DvmType batchRef, DvmType i, DvmType p) {
+    BatchCtx *b = asBatch(batchRef, "dvmh_amd_batch_param_C");
+    DVMH_CHECK(i >= 0 && i < (DvmType)b->loops.size(), "Batch loop index %ld out of range", i);
+    const auto &params = b->loops[i]->gpuH.params;
+    DVMH_CHECK(p >= 0 && p < (DvmType)params.size(), "Batch parameter index %ld out of range", p);
+    return params[p];
+}
+
+extern "C" void *dvmh_amd_batch_scratch_C(DvmType batchRef, DvmType bytes) {
+    BatchCtx *b = asBatch(batchRef, "dvmh_amd_batch_scratch_C");
+    void *p = gpu.alloc((size_t)(bytes > 0 ? bytes : 8));
+    b->scratch.push_back(p);
+    return p;
+}
+
+extern "C" void dvmh_amd_batch_upload_C(DvmType batchRef, void *dst, const void *src, DvmType bytes) {
+    asBatch(batchRef, "dvmh_amd_batch_upload_C");
+    void *stage = gpu.pinnedStaging((size_t)bytes);
+    memcpy(stage, src, (size_t)bytes);
+    HIP_CHECK(hipMemcpyAsync(dst, stage, (size_t)bytes, hipMemcpyHostToDevice, gpu.stream));
+    HIP_CHECK(hipStreamSynchronize(gpu.stream));   // the staging buffer is reused
+}
+
+// Folds each loop's partial segment and posts the results (one kernel, one D2H).
+extern "C" void dvmh_amd_batch_red_finish_C(DvmType batchRef, DvmType redIndex, const void *partials, const void *locPartials,
+        const DvmType *offsets) {
+    BatchCtx *b = asBatch(batchRef, "dvmh_amd_batch_red_finish_C");
+    long nl = (long)b->loops.size();
+    DVMH_CHECK(nl > 0, "empty batch");
+    auto &r0 = b->loops[0]->reds.at(redIndex - 1);
+    long es = r0->elemSize, ls = r0->isLoc() ? r0->locSize : 0;
+    for (Loop *l : b->loops) {
+        DVMH_CHECK(redIndex >= 1 && redIndex <= (DvmType)l->reds.size(), "Batch reduction index %ld out of range", redIndex);
+        auto &r = l->reds[redIndex - 1];
+        DVMH_CHECK(r->func == r0->func && r->rtType == r0->rtType && r->elemCount == 1 && (r->isLoc() ? r->locSize : 0) == ls,
+                "Batched loops disagree on reduction %ld", redIndex);
+    }
+    void *dOff = dvmh_amd_batch_scratch_C(batchRef, (nl + 1) * (long)sizeof(long));
+    dvmh_amd_batch_upload_C(batchRef, dOff, offsets, (nl + 1) * (long)sizeof(long));
+    void *dRes = dvmh_amd_batch_scratch_C(batchRef, nl * (es + ls) + 64);
+    void *dLoc = ls ? (char *)dRes + ((nl * es + 15) & ~15L) : nullptr;
+    dvmh_hip_reduce_segments(r0->func, r0->rtType, nl, (const long *)dOff, partials, locPartials, ls, dRes, dLoc, gpu.stream);
+    size_t total = (size_t)(((nl * es + 15) & ~15L) + nl * ls + 64);
+    char *host = pinnedTake(total);
+    HIP_CHECK(hipMemcpyAsync(host, dRes, total - 64, hipMemcpyDeviceToHost, gpu.stream));
+    for (long i = 0; i < nl; i++) {
+        auto &r = b->loops[i]->reds[redIndex - 1];
+        char *val = host + i * es;
+        char *loc = ls ? host + ((nl * es + 15) & ~15L) + i * ls : nullptr;
+        addPendingPost(r, val, loc, false);
+    }
+}

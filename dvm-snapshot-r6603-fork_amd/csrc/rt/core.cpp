@@ -120,6 +120,11 @@ void Gpu::release(void *p) {
     if (!p) return;
     auto it = liveBlocks.find(p);
     DVMH_INTERNAL(it != liveBlocks.end(), "releasing unknown device block %p", p);
+    if (graphCapturing()) {
+        // a graph under construction references it: it stays live with the graph
+        graphNoteAlloc(p);
+        return;
+    }
     freeBlocks[it->second].push_back(p);
     liveBlocks.erase(it);
 }
@@ -257,6 +262,9 @@ static void storageSizes(const Data *d, long out[]) {
     for (int q = 0; q < d->rank; q++) out[q] = d->space[d->dev.axisPerm[q] - 1].size();
 }
 
+static long layoutChangeCount = 0;
+long layoutChanges() { return layoutChangeCount; }
+
 // Re-lay the device copy.  newPerm[q] = original axis stored at position q.
 void Data::setLayout(const int newPerm[], int newDiag) {
     DVMH_INTERNAL(hasDev, "setLayout without a device copy");
@@ -271,6 +279,7 @@ void Data::setLayout(const int newPerm[], int newDiag) {
     bool same = newDiag == dev.diagState;
     for (int i = 0; i < rank && same; i++) same = perm[i] == dev.axisPerm[i];
     if (same) return;
+    layoutChangeCount++;
     const size_t nb = bytes();
     long sizes[MAX_RANK];
     if (dev.diagState) {

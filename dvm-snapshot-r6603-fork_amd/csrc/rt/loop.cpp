@@ -144,8 +144,8 @@ void flushDeferredPosts() {
     pinnedPool.reset();
 }
 
-void addPendingPost(const std::shared_ptr<Reduction> &r, char *val, char *loc) {
-    pending.push_back(PendingPost{r, val, loc, false});
+void addPendingPost(const std::shared_ptr<Reduction> &r, char *val, char *loc, bool addInitial) {
+    pending.push_back(PendingPost{r, val, loc, addInitial});
 }
 
 // ------------------------------------------------------------ GPU reductions
@@ -174,9 +174,14 @@ void gpuRedFinish(SpecLoop *sl, const std::shared_ptr<Reduction> &r) {
     DVMH_CHECK(r->elemCount == 1, "Reduction arrays are not supported on the GPU");
     void *dRes = gpu.alloc((size_t)(r->elemSize + (r->isLoc() ? r->locSize : 0)));
     void *dLoc = r->isLoc() ? (char *)dRes + r->elemSize : nullptr;
-    dvmh_hip_reduce_finish(r->func, r->rtType, r->items, r->gpuArr, r->gpuLoc, r->locSize, dRes, dLoc, sl->stream);
+    long scratchBytes = dvmh_hip_reduce_scratch_bytes(r->items);
+    void *scratch = scratchBytes ? gpu.alloc((size_t)scratchBytes) : nullptr;
+    dvmh_hip_reduce_finish(r->func, r->rtType, r->items, r->gpuArr, r->gpuLoc, r->locSize, dRes, dLoc, scratch, sl->stream);
+    if (scratch) gpu.release(scratch);
     size_t recBytes = (size_t)(r->elemSize + (r->isLoc() ? r->locSize : 0));
-    char *host = pinnedPool.take(recBytes);
+    // a captured finish writes into a slot owned by the graph
+    char *host = graphCapturing() ? graphPinnedSlot(recBytes) : pinnedPool.take(recBytes);
+    if (graphCapturing()) graphNotePost(r, host, r->isLoc() ? host + r->elemSize : nullptr);
     HIP_CHECK(hipMemcpyAsync(host, dRes, recBytes, hipMemcpyDeviceToHost, sl->stream));
     gpu.release(dRes);
     gpu.release(r->gpuArr);
@@ -221,9 +226,11 @@ static void finishLoopReductions(Loop *loop) {
     }
 }
 
+void finishLoopReductionsExt(Loop *loop) { finishLoopReductions(loop); }
+
+// dvmh_loop_perform_C (rts/dvmh/dvmh2.cpp:3769-3969): decide where the loop
+// runs; GPU loops of a lazy region or of an open batch are queued.
 void performLoop(Loop *loop) {
-    // rts/dvmh/dvmh2.cpp:3769-3969
-    for (auto &r : loop->reds) r->initValues(r->arrayAddr, r->locAddr);
     loop->computeDependencyMask();
     bool onGpu = loop->region && loop->region->onGpu && loop->hasGpu;
     if (!onGpu) {
@@ -231,11 +238,24 @@ void performLoop(Loop *loop) {
                 loop->file.c_str(), loop->line);
         DVMH_CHECK(loop->hasHost, "No host handler is registered for the loop at %s:%ld", loop->file.c_str(), loop->line);
     }
+    if (onGpu && loop->iterations() > 0) {
+        if (regionIsLazy(loop->region)) {
+            loop->queued = true;
+            loop->region->deferred.push_back(loop);
+            return;
+        }
+        if (batchQueue(loop)) return;
+    }
+    performLoopNow(loop);
+}
+
+void performLoopNow(Loop *loop) {
+    for (auto &r : loop->reds) r->initValues(r->arrayAddr, r->locAddr);
+    bool onGpu = loop->region && loop->region->onGpu && loop->hasGpu;
     if (loop->iterations() == 0) {
         finishLoopReductions(loop);
         return;
     }
-    if (onGpu && batchQueue(loop)) return;
     if (onGpu) prepareLayouts(loop);
     SpecLoop sl(loop, onGpu ? 1 : 0);
     sl.stream = gpu.stream;
@@ -243,7 +263,7 @@ void performLoop(Loop *loop) {
     DvmType ref = (DvmType)&sl;
     if (onGpu) {
         hipEvent_t ev0 = nullptr, ev1 = nullptr;
-        bool timing = settings.tuneMode != 0;
+        bool timing = settings.tuneMode != 0 && !graphCapturing();
         if (timing) {
             HIP_CHECK(hipEventCreate(&ev0));
             HIP_CHECK(hipEventCreate(&ev1));
@@ -263,7 +283,7 @@ void performLoop(Loop *loop) {
         }
         for (void *p : sl.toFree) gpu.release(p);
         finishLoopReductions(loop);
-        if (settings.syncMode == 0) flushDeferredPosts();
+        if (settings.syncMode == 0 && !graphCapturing()) flushDeferredPosts();
     } else {
         callHandler(loop->hostH, &ref);
         rt.loopsOnHost++;

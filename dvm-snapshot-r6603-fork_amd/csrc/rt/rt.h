@@ -164,6 +164,8 @@ struct Region : Object {
     bool onGpu = false;
     std::vector<RegionArray> arrays;
     std::vector<RegionArray> scalars;
+    std::vector<struct Loop *> deferred;   // GPU loops queued by a lazy (graph) region
+    int pendingBatch = 0;                  // loops of this region queued in an open batch
     Region(): Object(Kind::Region) {}
 };
 
@@ -234,6 +236,9 @@ struct Loop : Object {
     std::string file;
     long line = 0;
     SpecLoop *spec = nullptr;
+    bool queued = false;            // owned by a lazy region or a batch until it runs
+    HandlerDesc batchH;             // handler that runs many loops of this site at once
+    bool hasBatch = false;
     Loop(): Object(Kind::Loop) {}
     ~Loop() override;
     void computeDependencyMask();                       // rts/dvmh/loop.cpp:1117-1150
@@ -288,5 +293,19 @@ extern long graphReplays, batchGroups, batchLoops;
 bool batchQueue(Loop *loop);
 bool loopIsQueued(Loop *loop);
 int hostSlotCount();
+bool batchKeepsRegion(Region *r);
+char *pinnedTake(size_t bytes);
+void performLoopNow(Loop *loop);
+long layoutChanges();
+
+// lazy regions and graphs (graph.cpp)
+bool regionIsLazy(const Region *r);
+bool graphCapturing();
+void graphNoteAlloc(void *p);
+char *graphPinnedSlot(size_t bytes);
+void graphNotePost(const std::shared_ptr<Reduction> &r, char *val, char *loc);
+void graphRunRegion(Region *region);
+void graphReleaseAll();
+void addPendingPost(const std::shared_ptr<Reduction> &r, char *val, char *loc, bool addInitial = false);
 
 }  // namespace dvmh

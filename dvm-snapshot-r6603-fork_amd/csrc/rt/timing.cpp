@@ -1,0 +1,103 @@
+// Kernel timing marks used by emitted handlers, and execution options.
+#include <map>
+#include <string>
+#include <vector>
+
+#include "rt.h"
+
+namespace dvmh {
+
+struct SiteTiming {
+    long launches = 0;
+    double seconds = 0;
+};
+static bool timingOn = false;
+static std::map<std::string, SiteTiming> siteTimes;
+// events recorded but not yet read back: (site, start, stop)
+struct PendingEvents {
+    std::string site;
+    hipEvent_t a, b;
+};
+static std::vector<PendingEvents> inflight;
+static std::vector<hipEvent_t> eventPool;
+static hipEvent_t openStart = nullptr;
+
+static hipEvent_t takeEvent() {
+    if (!eventPool.empty()) {
+        hipEvent_t e = eventPool.back();
+        eventPool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    HIP_CHECK(hipEventCreate(&e));
+    return e;
+}
+
+static std::string siteOf(const Loop *l) { return l->file + ":" + std::to_string(l->line); }
+
+// Reads back every recorded pair (synchronising on the last one).
+void drainKernelTimes() {
+    if (inflight.empty()) return;
+    HIP_CHECK(hipEventSynchronize(inflight.back().b));
+    for (auto &p : inflight) {
+        float ms = 0;
+        HIP_CHECK(hipEventElapsedTime(&ms, p.a, p.b));
+        SiteTiming &t = siteTimes[p.site];
+        t.launches++;
+        t.seconds += ms * 1e-3;
+        eventPool.push_back(p.a);
+        eventPool.push_back(p.b);
+    }
+    inflight.clear();
+}
+
+bool kernelTimingOn() { return timingOn; }
+
+}  // namespace dvmh
+
+using namespace dvmh;
+
+extern "C" void dvmh_amd_mark_kernel_C(DvmType curLoop, DvmType phase) {
+    if (!timingOn) return;
+    SpecLoop *sl = asSpec(curLoop, "dvmh_amd_mark_kernel_C");
+    if (sl->device != 1) return;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    HIP_CHECK(hipStreamIsCapturing(sl->stream, &cs));
+    if (cs != hipStreamCaptureStatusNone) return;   // events are not recorded into graphs
+    if (phase == 0) {
+        openStart = takeEvent();
+        HIP_CHECK(hipEventRecord(openStart, sl->stream));
+    } else if (openStart) {
+        hipEvent_t b = takeEvent();
+        HIP_CHECK(hipEventRecord(b, sl->stream));
+        inflight.push_back(PendingEvents{siteOf(sl->loop), openStart, b});
+        openStart = nullptr;
+        if (inflight.size() > 4096) drainKernelTimes();
+    }
+}
+
+extern "C" void dvmh_amd_set_kernel_timing_C(DvmType enable) {
+    if (!enable) drainKernelTimes();
+    timingOn = enable != 0;
+}
+
+extern "C" DvmType dvmh_amd_kernel_stats_C(const char *file, DvmType line, DvmType out[2]) {
+    drainKernelTimes();
+    auto it = siteTimes.find(std::string(file) + ":" + std::to_string(line));
+    if (it == siteTimes.end()) return 0;
+    out[0] = it->second.launches;
+    out[1] = (DvmType)(it->second.seconds * 1e9);
+    return 1;
+}
+
+extern "C" void dvmh_amd_reset_kernel_stats_C(void) {
+    drainKernelTimes();
+    siteTimes.clear();
+}
+
+extern "C" void dvmh_amd_set_sync_mode_C(DvmType mode) {
+    flushDeferredPosts();
+    settings.syncMode = (int)mode;
+}
+
+extern "C" void dvmh_amd_set_graph_C(DvmType enable) { settings.graph = enable != 0; }

@@ -301,6 +301,30 @@ extern "C" DvmType dvmh_amd_tuner_save_C(const char *path) { return tunerSave(pa
 extern "C" DvmType dvmh_amd_tuner_load_C(const char *path) { return tunerLoad(path); }
 extern "C" void dvmh_amd_tuner_clear_C(void) { sites.clear(); }
 
+extern "C" DvmType dvmh_amd_tuner_pending_C(void) {
+    long pending = 0;
+    for (auto &kv : sites)
+        for (size_t c = 0; c < kv.second.cands.size(); c++)
+            if ((int)kv.second.times[c].size() < settings.tuneRepeats) pending++;
+    return pending;
+}
+
+namespace dvmh {
+// Candidates still unmeasured at the sites of one source loop (file, line).
+long tunerSitePending(const std::string &file, long line) {
+    std::string prefix = file + ":" + std::to_string(line) + "|";
+    long pending = 0;
+    for (auto &kv : sites) {
+        if (kv.first.compare(0, prefix.size(), prefix) != 0) continue;
+        for (size_t c = 0; c < kv.second.cands.size(); c++)
+            if ((int)kv.second.times[c].size() < settings.tuneRepeats) pending++;
+    }
+    return pending;
+}
+}  // namespace dvmh
+
+extern "C" DvmType dvmh_amd_tuner_site_pending_C(const char *file, DvmType line) { return tunerSitePending(file, line); }
+
 extern "C" DvmType dvmh_amd_tuner_query_C(const char *file, DvmType line, DvmType out[10]) {
     std::string prefix = std::string(file) + ":" + std::to_string(line) + "|";
     for (auto &kv : sites) {

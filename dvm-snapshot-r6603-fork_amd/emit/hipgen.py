@@ -480,6 +480,7 @@ def parallel_handler(L):
             cond = f'kernelIndexT == {"rt_INT" if it == "int" else "rt_LLONG"} && depth == {dp}'
             kernels.append((cond, f'{L.name}_kernel<{it}, {dp}>'))
     b.append('if (nblocks > 0) {')
+    b.append(IND + 'dvmh_amd_mark_kernel_C(loop_ref, 0);')
     for it in ['int', 'long long']:
         args = array_fact_args(L, it)
         for i in range(r):
@@ -488,8 +489,217 @@ def parallel_handler(L):
         args += red_fact_args(L)
         ks = [(c, k) for c, k in kernels if (('rt_INT' in c) == (it == 'int'))]
         b += [IND + s for s in launch_dispatch(ks, args, 'dim3((unsigned)nblocks)')]
+    b.append(IND + 'dvmh_amd_mark_kernel_C(loop_ref, 1);')
     b.append('}')
     b += handler_reds_finish(L)
+    h.append(indent('\n'.join(b) + '\n', 1).rstrip('\n'))
+    h.append('}')
+    return '\n'.join(h) + '\n'
+
+
+# ----------------------------------------------------------------- GPU: batched PARALLEL loops
+def batch_desc_layout(L):
+    """Per-loop descriptor (array of longs): array bases and strides, scalar
+    device addresses, (lo, cnt, step) per axis, tile counts, reduction inits."""
+    lay, q = {}, 0
+    for p in L.arrays:
+        lay[('base', p[1])] = q
+        q += 1
+        lay[('str', p[1])] = q
+        q += p[3]
+    for p in L.scalars:
+        lay[('sc', p[1])] = q
+        q += 1
+    lay['bounds'] = q
+    q += 3 * L.rank
+    lay['nb'] = q
+    q += 3
+    for red in L.reds:
+        lay[('ri', red[1])] = q
+        q += 1
+        if len(red) > 3:
+            lay[('ri', red[3])] = q
+            q += 1
+    return lay, q
+
+
+def batch_kernel(L):
+    r = L.rank
+    lay, dlen = batch_desc_layout(L)
+    fp = ['const long *__restrict__ desc', 'const long *__restrict__ tileStart', 'long nloops', 'long totalTiles']
+    for red in L.reds:
+        fp.append(f'{red[2]} *{red[1]}_grid')
+        if len(red) > 3:
+            fp.append(f'{red[4]} *{red[3]}_grid')
+    k = ['template <typename IndexT, int DEPTH>',
+         f'__global__ __launch_bounds__(1024) void {L.name}_batch_kernel({", ".join(fp)}) {{']
+    b = ['for (long tile = blockIdx.x; tile < totalTiles; tile += gridDim.x) {']
+    t = []
+    t.append('long lo_ = 0, hi_ = nloops - 1;')
+    t.append('while (lo_ < hi_) { long mid = (lo_ + hi_ + 1) >> 1; if (tileStart[mid] <= tile) lo_ = mid; else hi_ = mid - 1; }')
+    t.append(f'const long *d = desc + lo_ * {dlen};')
+    for p in L.arrays:
+        _, nm, ct, ar = p
+        so = lay[('str', nm)]
+        t.append(f'const IndexT {nm}_str[{ar}] = {{{", ".join(f"(IndexT)d[{so + q}]" for q in range(ar))}}};')
+        t.append(f'const auto {nm} = dvmh_dev::makeView<{ct}, IndexT, {ar}, true>(({ct} *)d[{lay[("base", nm)]}], {nm}_str);')
+    for p in L.scalars:
+        t.append(f'{p[2]} &{p[1]} = *({p[2]} *)d[{lay[("sc", p[1])]}];')
+    bo = lay['bounds']
+    for i in range(r):
+        t.append(f'const IndexT lo{i} = (IndexT)d[{bo + 3 * i}], cnt{i} = (IndexT)d[{bo + 3 * i + 1}], st{i} = (IndexT)d[{bo + 3 * i + 2}];')
+    nbo = lay['nb']
+    t.append(f'const long nbx = d[{nbo}], nby = d[{nbo + 1}], nbz = d[{nbo + 2}];')
+    for v, ct in L.vars:
+        t.append(f'{ct} {v};')
+    for pr in L.privates:
+        t.append(pr + ';')
+    for red in L.reds:
+        t.append(f'{red[2]} {red[1]} = *(const {red[2]} *)&d[{lay[("ri", red[1])]}];')
+        if len(red) > 3:
+            t.append(f'{red[4]} {red[3]} = *(const {red[4]} *)&d[{lay[("ri", red[3])]}];')
+    t.append('long tl = tile - tileStart[lo_];')
+    t.append('const IndexT bx = (IndexT)(tl % nbx); tl /= nbx;')
+    t.append('const IndexT by = (IndexT)(tl % nby); tl /= nby;')
+    t.append('const IndexT bz = (IndexT)(tl % nbz); tl /= nbz;')
+    ax_x, ax_y, ax_z = r - 1, (r - 2 if r >= 2 else None), (r - 3 if r >= 3 else None)
+    dep_ax = ax_z if ax_z is not None else (ax_y if ax_y is not None else ax_x)
+    for a in reversed(range(0, r - 3)):
+        t.append(f'const IndexT t{a} = (IndexT)(tl % cnt{a}); tl /= cnt{a};')
+    t.append(f'const IndexT t{ax_x}_0 = bx * (IndexT)blockDim.x + (IndexT)threadIdx.x;')
+    if ax_y is not None:
+        t.append(f'const IndexT t{ax_y}_0 = by * (IndexT)blockDim.y + (IndexT)threadIdx.y;')
+    if ax_z is not None:
+        t.append(f'const IndexT t{ax_z}_0 = bz * (IndexT)blockDim.z + (IndexT)threadIdx.z;')
+    conds = []
+    for a in [ax_x, ax_y, ax_z]:
+        if a is None:
+            continue
+        if a == dep_ax and r >= 2:
+            t.append(f'const IndexT t{a}_b = t{a}_0 * DEPTH;')
+        else:
+            t.append(f'const IndexT t{a} = t{a}_0;')
+            conds.append(f't{a} < cnt{a}')
+    inner = []
+    if r >= 2:
+        inner.append(f'const IndexT t{dep_ax} = t{dep_ax}_b + dd;')
+        conds = conds + [f't{dep_ax} < cnt{dep_ax}']
+    inner.append(f'if ({" && ".join(conds)}) {{')
+    for i in range(r):
+        inner.append(IND + var_assign(L, i, f't{i}', f'lo{i}', f'st{i}'))
+    inner.append(indent('{\n' + indent(L.body + '\n', 1) + '}', 1))
+    inner.append('}')
+    if r >= 2:
+        t.append('#pragma unroll')
+        t.append('for (int dd = 0; dd < DEPTH; dd++) {')
+        t += [IND + ln for ln in '\n'.join(inner).split('\n')]
+        t.append('}')
+    else:
+        t += inner
+    t += red_kernel_tail(L, 'tile', accumulate=False)
+    b += [IND + ln for ln in '\n'.join(t).split('\n')]
+    b.append('}')
+    k.append(indent('\n'.join(b) + '\n', 1).rstrip('\n'))
+    k.append('}')
+    return '\n'.join(k) + '\n'
+
+
+def batch_handler(L):
+    r = L.rank
+    lay, dlen = batch_desc_layout(L)
+    h = [f'extern "C" void {L.name}_cuda_batch(DvmType *pBatchRef) {{']
+    b = ['DvmType batchRef = *pBatchRef;',
+         'long nl = (long)dvmh_amd_batch_count_C(batchRef);',
+         'DvmType lr0 = dvmh_amd_batch_loop_C(batchRef, 0);',
+         'DvmType tune[DVMH_AMD_TUNE_N];',
+         'dvmh_amd_loop_get_tuning_C(lr0, DVMH_AMD_KIND_PARALLEL, tune);',
+         'dim3 threads(0, 0, 0);',
+         'hipStream_t stream;',
+         'dvmh_loop_cuda_get_config_C(lr0, 0, 0, &threads, &stream, 0);',
+         'int depth = (int)tune[DVMH_AMD_T_DEPTH];',
+         'if (depth != 1 && depth != 2 && depth != 4 && depth != 8) depth = 1;',
+         f'long *desc = (long *)calloc((size_t)nl * {dlen}, sizeof(long));',
+         'DvmType *tstart = (DvmType *)calloc((size_t)nl + 1, sizeof(DvmType));',
+         'long tiles = 0;',
+         'int wide = 0;',
+         'for (long li = 0; li < nl; li++) {']
+    lb = ['DvmType lr = dvmh_amd_batch_loop_C(batchRef, li);',
+          f'long *d = desc + li * {dlen};',
+          'long maxStride = 1;']
+    for pi, p in enumerate(L.params):
+        if p[0] == 'array':
+            _, nm, ct, ar = p
+            lb.append(f'{{ DvmType *hdr = (DvmType *)dvmh_amd_batch_param_C(batchRef, li, {pi});')
+            lb.append(f'  {ct} *base = ({ct} *)dvmh_get_natural_base_C(1, hdr);')
+            lb.append('  DvmType devHdr[64 + 7];')
+            lb.append('  if (dvmh_fill_header_C(1, base, hdr, devHdr, 0) != 0) { fprintf(stderr, "batched loop: array not in natural layout\\n"); abort(); }')
+            lb.append(f'  d[{lay[("base", nm)]}] = (long)base;')
+            for q in range(ar):
+                lb.append(f'  d[{lay[("str", nm)] + q}] = devHdr[{q + 1}];')
+            lb.append(f'  for (int q = 0; q < {ar}; q++) if (labs(devHdr[q + 1]) > maxStride) maxStride = labs(devHdr[q + 1]); }}')
+        else:
+            lb.append(f'd[{lay[("sc", p[1])]}] = (long)dvmh_get_device_addr_C(1, dvmh_amd_batch_param_C(batchRef, li, {pi}));')
+    lb.append(f'DvmType blo[{r}], bhi[{r}], bst[{r}];')
+    lb.append('dvmh_loop_fill_bounds_C(lr, blo, bhi, bst);')
+    for i in range(r):
+        lb.append(f'long cnt{i} = (bhi[{i}] - blo[{i}]) / bst[{i}] + 1;')
+        lb.append(f'd[{lay["bounds"] + 3 * i}] = blo[{i}]; d[{lay["bounds"] + 3 * i + 1}] = cnt{i}; d[{lay["bounds"] + 3 * i + 2}] = bst[{i}];')
+    ax_x, ax_y, ax_z = r - 1, (r - 2 if r >= 2 else None), (r - 3 if r >= 3 else None)
+    dep_ax = ax_z if ax_z is not None else (ax_y if ax_y is not None else ax_x)
+    lb.append(f'long nbx = (cnt{ax_x} + threads.x - 1) / threads.x;')
+    if ax_y is not None:
+        dy = ' * depth' if dep_ax == ax_y else ''
+        lb.append(f'long nby = (cnt{ax_y} + threads.y{dy} - 1) / (threads.y{dy});')
+    else:
+        lb.append('long nby = 1;')
+    if ax_z is not None:
+        lb.append(f'long nbz = (cnt{ax_z} + threads.z * depth - 1) / (threads.z * depth);')
+    else:
+        lb.append('long nbz = 1;')
+    lb.append(f'd[{lay["nb"]}] = nbx; d[{lay["nb"] + 1}] = nby; d[{lay["nb"] + 2}] = nbz;')
+    for i in range(r):
+        lb.append(f'if ((labs(blo[{i}]) + cnt{i} * labs(bst[{i}]) + 64) * maxStride > 2147483647L) wide = 1;')
+    for ri, red in enumerate(L.reds):
+        lb.append(f'{{ {red[2]} v; ')
+        if len(red) > 3:
+            lb.append(f'  {red[4]} lv; dvmh_loop_red_init_C(lr, {ri + 1}, &v, &lv);')
+            lb.append(f'  memcpy(&d[{lay[("ri", red[3])]}], &lv, sizeof(lv));')
+        else:
+            lb.append(f'  dvmh_loop_red_init_C(lr, {ri + 1}, &v, 0);')
+        lb.append(f'  memcpy(&d[{lay[("ri", red[1])]}], &v, sizeof(v)); }}')
+    lb.append('tstart[li] = tiles;')
+    lb.append('tiles += nbx * nby * nbz' + ''.join(f' * cnt{a}' for a in range(0, r - 3)) + ';')
+    b += [IND + s for s in lb]
+    b.append('}')
+    b.append('tstart[nl] = tiles;')
+    b.append(f'long *dDesc = (long *)dvmh_amd_batch_scratch_C(batchRef, nl * {dlen} * (long)sizeof(long));')
+    b.append(f'dvmh_amd_batch_upload_C(batchRef, dDesc, desc, nl * {dlen} * (long)sizeof(long));')
+    b.append('long *dStart = (long *)dvmh_amd_batch_scratch_C(batchRef, (nl + 1) * (long)sizeof(long));')
+    b.append('dvmh_amd_batch_upload_C(batchRef, dStart, tstart, (nl + 1) * (long)sizeof(long));')
+    for red in L.reds:
+        b.append(f'{red[2]} *{red[1]}_grid = ({red[2]} *)dvmh_amd_batch_scratch_C(batchRef, (tiles > 0 ? tiles : 1) * (long)sizeof({red[2]}));')
+        if len(red) > 3:
+            b.append(f'{red[4]} *{red[3]}_grid = ({red[4]} *)dvmh_amd_batch_scratch_C(batchRef, (tiles > 0 ? tiles : 1) * (long)sizeof({red[4]}));')
+    b.append('long wgs = tiles < 16384 ? tiles : 16384;   // persistent: each workgroup walks tiles')
+    b.append('if (tiles > 0) {')
+    b.append(IND + 'dvmh_amd_mark_kernel_C(lr0, 0);')
+    ra = []
+    for red in L.reds:
+        ra.append(f'{red[1]}_grid')
+        if len(red) > 3:
+            ra.append(f'{red[3]}_grid')
+    for it in ['int', 'long long']:
+        for dp in [1, 2, 4, 8]:
+            cond = f'{"!wide" if it == "int" else "wide"} && depth == {dp}'
+            args = ['dDesc', 'dStart', 'nl', 'tiles'] + ra
+            b.append(IND + f'if ({cond}) hipLaunchKernelGGL(({L.name}_batch_kernel<{it}, {dp}>), dim3((unsigned)wgs), threads, 0, stream, {", ".join(args)});')
+    b.append(IND + 'dvmh_amd_mark_kernel_C(lr0, 1);')
+    b.append('}')
+    for ri, red in enumerate(L.reds):
+        loc = f'{red[3]}_grid' if len(red) > 3 else '0'
+        b.append(f'dvmh_amd_batch_red_finish_C(batchRef, {ri + 1}, {red[1]}_grid, {loc}, tstart);')
+    b.append('free(desc);')
+    b.append('free(tstart);')
     h.append(indent('\n'.join(b) + '\n', 1).rstrip('\n'))
     h.append('}')
     return '\n'.join(h) + '\n'
@@ -556,6 +766,7 @@ def line_launch(L, dep):
     b.append(f'DvmType nblocks = {nb};')
     b += handler_reds_prepare(L, 'nblocks', 0)
     b.append('if (nblocks > 0) {')
+    b.append(IND + 'dvmh_amd_mark_kernel_C(loop_ref, 0);')
     for it in ['int', 'long long']:
         args = array_fact_args(L, it)
         for i in range(r):
@@ -564,6 +775,7 @@ def line_launch(L, dep):
         args += red_fact_args(L)
         cond = f'kernelIndexT == {"rt_INT" if it == "int" else "rt_LLONG"}'
         b += [IND + s for s in launch_dispatch([(cond, f'{L.name}_line{dep}_kernel<{it}>')], args, 'dim3((unsigned)nblocks)')]
+    b.append(IND + 'dvmh_amd_mark_kernel_C(loop_ref, 1);')
     b.append('}')
     b += handler_reds_finish(L)
     return b
@@ -637,6 +849,7 @@ def wave_launch(L, deps):
     b.append(f'DvmType maxFy = {"(cnt%d + threads.y - 1) / threads.y" % fy if fy is not None else "1"};')
     b.append('DvmType slots = maxFx * maxFy' + ''.join(f' * cnt{a}' for a in rest) + ';')
     b += handler_reds_prepare(L, 'slots', 1)
+    b.append('dvmh_amd_mark_kernel_C(loop_ref, 0);')
     b.append('for (DvmType plane = 0; plane < planes; plane++) {')
     # the first-free-axis range that can meet the plane
     lead = fy if fy is not None else fx
@@ -661,6 +874,7 @@ def wave_launch(L, deps):
         cond = f'kernelIndexT == {"rt_INT" if it == "int" else "rt_LLONG"}'
         b += [IND + s for s in launch_dispatch([(cond, f'{L.name}_wave{tag}_kernel<{it}>')], args, 'dim3((unsigned)nblocks)')]
     b.append('}')
+    b.append('dvmh_amd_mark_kernel_C(loop_ref, 1);')
     b += handler_reds_finish(L)
     return b
 
@@ -726,6 +940,9 @@ def gpu_loop(L):
     else:
         text.append(parallel_kernel(L))
         text.append(parallel_handler(L))
+        if L.batch:
+            text.append(batch_kernel(L))
+            text.append(batch_handler(L))
     return '\n'.join(text)
 
 
@@ -745,7 +962,7 @@ def emit(spec_path, outdir):
     for L in loops:
         host.append(host_handler(L))
     hip = ['// GPU handlers generated from ' + os.path.basename(spec_path) + ' by emit/hipgen.py (gfx950).',
-           '#include <hip/hip_runtime.h>', '#include <math.h>', '#include <stdio.h>', '#include <stdlib.h>',
+           '#include <hip/hip_runtime.h>', '#include <math.h>', '#include <stdio.h>', '#include <stdlib.h>', '#include <string.h>',
            '#include "dvmhlib2.h"', '#include "dvmh_amd.h"', '#include "dvmh_hip_device.h"']
     if inc:
         hip.append(f'#include "{inc}"')

@@ -35,6 +35,25 @@ extern "C" {
 #define DVMH_AMD_T_RESERVED 7
 void dvmh_amd_loop_get_tuning_C(DvmType curLoop, DvmType kind, DvmType tune[]);
 
+/* ---- kernel timing -----------------------------------------------------------
+ * Emitted handlers bracket their main kernel launches with phase 0 / phase 1
+ * marks.  With timing enabled the runtime records HIP events on the loop's
+ * stream at the marks and accumulates, per loop site (file, line), the number
+ * of bracketed launches and their total duration. */
+void dvmh_amd_mark_kernel_C(DvmType curLoop, DvmType phase);
+void dvmh_amd_set_kernel_timing_C(DvmType enable);
+/* out[0] launches, out[1] total nanoseconds; returns 0 if the site is unknown. */
+DvmType dvmh_amd_kernel_stats_C(const char *file, DvmType line, DvmType out[2]);
+void dvmh_amd_reset_kernel_stats_C(void);
+
+/* ---- execution options --------------------------------------------------------
+ * sync mode 0: synchronise after every loop (the reference's behaviour);
+ * 1: only when a result is needed (get_actual, region end, barrier). */
+void dvmh_amd_set_sync_mode_C(DvmType mode);
+/* 1: regions created with REGION_AMD_GRAPH are run lazily and replayed as graphs
+ * (same as DVMH_AMD_GRAPH=1); 0: every region runs eagerly. */
+void dvmh_amd_set_graph_C(DvmType enable);
+
 /* ---- statistics ------------------------------------------------------------
  * out[0] loops run on the host, out[1] loops run on the GPU,
  * out[2] GPU loop time in nanoseconds (sum of handler stream time when the
@@ -52,9 +71,14 @@ const char *dvmh_amd_device_name_C(void);
 void dvmh_amd_batch_begin_C(void);
 void dvmh_amd_batch_end_C(void);
 void dvmh_amd_loop_register_batch_handler_C(DvmType curLoop, DvmType handlerHelper);
-/* Inside a batch handler: number of loops and the i-th loop reference. */
+/* Inside a batch handler (called as f(&batchRef)): number of loops, the i-th
+ * loop reference (usable with every handler query), and the p-th parameter the
+ * i-th loop registered for its ordinary GPU handler. */
 DvmType dvmh_amd_batch_count_C(DvmType batchRef);
 DvmType dvmh_amd_batch_loop_C(DvmType batchRef, DvmType i);
+void *dvmh_amd_batch_param_C(DvmType batchRef, DvmType i, DvmType p);
+/* Copies bytes from host memory to device memory dst on the runtime's stream. */
+void dvmh_amd_batch_upload_C(DvmType batchRef, void *dst, const void *src, DvmType bytes);
 /* Inside a batch handler: finish reduction redIndex of every loop in the batch.
  * partials holds per-loop segments [offsets[i], offsets[i+1]) of values (and
  * locs) in device memory; one kernel folds each segment and the results are
@@ -79,6 +103,10 @@ void dvmh_amd_tuner_set_mode_C(DvmType mode);
 DvmType dvmh_amd_tuner_save_C(const char *path);
 DvmType dvmh_amd_tuner_load_C(const char *path);
 void dvmh_amd_tuner_clear_C(void);
+/* Candidates still to be measured at the sites seen so far (exhaustive sweep). */
+DvmType dvmh_amd_tuner_pending_C(void);
+/* The same for the sites of one source loop (file, line). */
+DvmType dvmh_amd_tuner_site_pending_C(const char *file, DvmType line);
 /* Best candidate so far for the loop site (file, line): out[0] candidate index,
  * out[1] time in ns, out[2..9] block x,y,z and tune[0..4]; returns 0 if unknown. */
 DvmType dvmh_amd_tuner_query_C(const char *file, DvmType line, DvmType out[10]);

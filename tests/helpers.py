@@ -191,3 +191,54 @@ def run_jackat(rank, n, itmax, maxeps=0.5, targets=D.DEVICE_TYPE_HOST | D.DEVICE
     last = P.fn('run', [ctypes.c_int, ctypes.c_long, ctypes.c_int, ctypes.c_float, ctypes.POINTER(ctypes.c_float)],
                 ctypes.c_float)(rank, n, itmax, maxeps, eps.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
     return float(last), eps
+
+
+# ------------------------------------------------------------------ batched uneven grids
+def batched_sides(count, seed=None):
+    seed = D.SEED_BASE + 5 if seed is None else seed
+    s = np.zeros(count, dtype=np.int64)
+    oracle().orc_batched_sides(vp(s), count, seed)
+    return s
+
+
+def oracle_batched(sides, iters, seed=None, grids=None, threads=0):
+    """Each grid g: orc_jac2d on its own grid seeded by orc_batched_grid_seed."""
+    seed = D.SEED_BASE + 5 if seed is None else seed
+    o = oracle()
+    out = []
+    for g in (range(len(sides)) if grids is None else grids):
+        n = int(sides[g])
+        A = np.zeros(n * n)
+        o.orc_init_grid2d(vp(A), n, o.orc_batched_grid_seed(seed, g), 1.0)
+        B = A.copy()
+        eps = np.zeros(iters)
+        o.orc_jac2d(vp(A), vp(B), n, iters, vp(eps), threads)
+        out.append({'g': g, 'A': A, 'B': B, 'eps': eps})
+    return out
+
+
+def run_batched(count, iters, batched, sides=None, seed=None, targets=D.DEVICE_TYPE_HOST | D.DEVICE_TYPE_CUDA, flags=0,
+                fetch=True):
+    """The converted batched program: returns eps history [iters, count], the
+    grids (if fetch) and the iteration wall time."""
+    seed = D.SEED_BASE + 5 if seed is None else seed
+    P = _program('batched')
+    P.set_targets(targets, flags)
+    sides_arr = None if sides is None else np.ascontiguousarray(sides, dtype=np.int64)
+    P.fn('setup', [ctypes.c_int, _c_lp, ctypes.c_ulonglong])(
+        count, None if sides_arr is None else sides_arr.ctypes.data_as(_c_lp), seed)
+    side = P.fn('side', [ctypes.c_int], ctypes.c_long)
+    got_sides = np.array([side(g) for g in range(count)], dtype=np.int64)
+    eps = np.zeros(iters * count)
+    t = P.fn('iterate', [ctypes.c_int, ctypes.c_int, _c_dp], ctypes.c_double)(iters, int(batched), dp(eps))
+    grids = []
+    if fetch:
+        f = P.fn('fetch', [ctypes.c_int, _c_dp, _c_dp])
+        for g in range(count):
+            n = int(got_sides[g])
+            A = np.zeros(n * n)
+            B = np.zeros(n * n)
+            f(g, dp(A), dp(B))
+            grids.append({'A': A, 'B': B})
+    P.fn('teardown', [])()
+    return {'sides': got_sides, 'eps': eps.reshape(iters, count), 'grids': grids, 'seconds': t}

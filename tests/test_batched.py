@@ -1,0 +1,94 @@
+"""Batched uneven grids: many independent 2-D Jacobi grids, each its own DVM
+arrays, regions and loops.  The per-grid form runs every grid's loops through
+their ordinary handlers; the batched form queues one iteration's loops between
+dvmh_amd_batch_begin_C / dvmh_amd_batch_end_C and runs each loop site as one
+persistent kernel over all grids' tiles.  Both are compared bit for bit with
+the oracle (grid values) and exactly (per-grid MAX eps)."""
+import numpy as np
+import pytest
+
+import dvmh_amd as D
+import helpers as H
+
+
+def _check(res, ref_list, iters):
+    for ref in ref_list:
+        g = ref['g']
+        got = res['grids'][g]
+        assert H.bits_equal(got['A'], ref['A']), f'grid {g} (side {res["sides"][g]}): A differs'
+        assert H.bits_equal(got['B'], ref['B']), f'grid {g}: B differs'
+        assert np.array_equal(res['eps'][:, g], ref['eps']), f'grid {g}: eps differs'
+
+
+def test_sides_follow_seed():
+    """The program draws the same sides as the oracle, all within [64, 1024]."""
+    s = H.batched_sides(512)
+    assert s.min() >= 64 and s.max() <= 1024
+    res = H.run_batched(6, 0, 0, targets=D.DEVICE_TYPE_HOST, fetch=False)
+    assert np.array_equal(res['sides'], s[:6])
+
+
+@pytest.mark.parametrize('batched', [0, 1])
+def test_host_path_bit_exact(batched):
+    """Host handlers (batching only applies to GPU loops, so batched=1 must be a no-op here)."""
+    sides = [5, 2, 3, 17, 64]
+    res = H.run_batched(len(sides), 4, batched, sides=sides, targets=D.DEVICE_TYPE_HOST)
+    _check(res, H.oracle_batched(sides, 4), 4)
+
+
+def _gpu_run(count, iters, batched, **kw):
+    before = D.stats()
+    res = H.run_batched(count, iters, batched, **kw)
+    after = D.stats()
+    assert after['gpu_present'] == 1
+    assert after['loops_host'] == before['loops_host']
+    return res, {k: after[k] - before[k] for k in after}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('batched', [0, 1])
+def test_gpu_ragged_bit_exact(batched):
+    """Ragged sides including a grid with an empty interior (side 2), a single
+    interior point (side 3) and sides that are not multiples of any tile."""
+    sides = [2, 3, 67, 5, 130, 64, 1, 257, 33, 1024]
+    sides = [max(s, 2) for s in sides]
+    iters = 5
+    res, d = _gpu_run(len(sides), iters, batched, sides=sides)
+    _check(res, H.oracle_batched(sides, iters), iters)
+    if batched:
+        # two fused launches per iteration (stencil site, copy+MAX site)
+        assert d['batch_groups'] == 2 * iters
+        nonempty = sum(1 for s in sides if s > 2)
+        assert d['batch_loops'] == 2 * iters * nonempty
+
+
+@pytest.mark.gpu
+def test_gpu_batched_matches_per_grid_seeded():
+    """64 seeded grids, 6 iterations: batched and per-grid forms agree bit for
+    bit with each other and with the oracle on a sample of grids."""
+    count, iters = 64, 6
+    a, _ = _gpu_run(count, iters, 0)
+    b, d = _gpu_run(count, iters, 1)
+    assert d['batch_groups'] == 2 * iters and d['batch_loops'] == 2 * iters * count
+    assert np.array_equal(a['eps'], b['eps'])
+    for g in range(count):
+        assert H.bits_equal(a['grids'][g]['A'], b['grids'][g]['A'])
+    sample = [0, 1, 17, 63, int(np.argmax(a['sides'])), int(np.argmin(a['sides']))]
+    _check(b, H.oracle_batched(a['sides'], iters, grids=sample), iters)
+
+
+@pytest.mark.gpu
+def test_gpu_full_workload_properties():
+    """512 seeded grids x 200 iterations (the workload's size), batched: every
+    grid's eps is non-increasing, the boundaries stay 1.0, and the smallest and
+    largest grids match the oracle bit for bit."""
+    count, iters = 512, 200
+    res, d = _gpu_run(count, iters, 1, fetch=True)
+    assert d['batch_loops'] == 2 * iters * count
+    assert np.all(np.diff(res['eps'], axis=0) <= 0)
+    for g in range(count):
+        n = int(res['sides'][g])
+        A = res['grids'][g]['A'].reshape(n, n)
+        assert np.all(A[0] == 1.0) and np.all(A[-1] == 1.0) and np.all(A[:, 0] == 1.0) and np.all(A[:, -1] == 1.0)
+    sample = [int(np.argmin(res['sides'])), int(np.argmax(res['sides']))]
+    _check(res, H.oracle_batched(res['sides'], iters, grids=sample), iters)
