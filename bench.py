@@ -1,25 +1,29 @@
 #!/usr/bin/env python3
-"""Benchmark: 3-D 7-point Jacobi, FP64 384^3, 50 iterations per step, through
-the DVMH C interface on one MI355X.
+"""Benchmark of libdvmh_amd on one MI355X: converted C-DVMH programs running
+their PARALLEL / ACROSS loops through the DVMH C interface.
 
-A step is one run of the program's iteration phase (50 iterations, each one
-PARALLEL loop with REDUCTION(MAX) and REDUCTION(SUM)) on inputs already
-resident in HBM.  Two forms run side by side:
-  plain   the default launch configuration, one synchronised launch sequence
-          per loop (what the reference runtime does);
-  chosen  the tuner's configuration (learned cost model when a trained model
-          is present, else the exhaustive sweep done during warm-up), results
-          synchronised only when needed, repeating regions replayed as graphs.
+Headline workload: 3-D 7-point Jacobi, FP64 384^3, 50 iterations per step
+(each iteration one PARALLEL loop with REDUCTION(MAX) and REDUCTION(SUM)),
+inputs resident in HBM.  Three forms run side by side:
+  plain       default launch configuration, synchronised after every loop
+              (what the reference runtime does);
+  exhaustive  the tuner's exhaustive sweep (every candidate configuration
+              timed during warm-up, then the fastest), results synchronised
+              only when needed;
+  chosen      the learned cost model ranks the candidates and only its best
+              four plus the default are timed during warm-up.
 `value` is the chosen form's point-updates/s.  The roofline uses the stencil
 kernel's average duration measured with HIP events on the runtime's stream
-during the timed region.  The CPU baseline is the oracle (the C/OpenMP
-restatement of the host handlers) timed on a bounded sample of the same
-workload on this machine's host cores.
+during the timed region.  The batched-grid workload (512 uneven 2-D grids) and
+the other configurations are reported in the same JSON line.  The CPU baseline
+is the oracle (the C/OpenMP restatement of the host handlers) timed on a
+bounded sample of the headline workload on this machine's host cores.
 """
 import argparse
 import ctypes
 import json
 import os
+import re
 import sys
 import time
 
@@ -31,77 +35,75 @@ sys.path.insert(0, os.path.join(REPO, 'tests'))
 import numpy as np  # noqa: E402
 
 import dvmh_amd as D  # noqa: E402
+from dvmh_amd import costmodel as CM  # noqa: E402
 
 METRIC = 'FP64 point-updates/s + % HBM roofline; cost-model/batched vs default, bit-exact'
 N, ITERS = 384, 50
 SEED = D.SEED_BASE + 2
 HBM_PEAK = 8.0e12
-B_ALG = 16          # bytes per point-update: read A, write B (8 B each)
+B_ALG = 16          # bytes per point-update of the stencil loop: read A, write B (8 B each)
 PTS = (N - 2) ** 3  # point-updates per iteration
 STENCIL_LINES = (41, 54)
-MODEL_FILE = os.path.join(PKG, 'models', 'costmodel.bin')
+PMC_SUMMARY = os.path.join(REPO, 'profiles', 'jac3d_384_stencil_pmc.txt')
+
+_dp = ctypes.POINTER(ctypes.c_double)
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def bind():
+    rt = CM.rt()
+    for name, at in [('dvmh_amd_set_sync_mode_C', [ctypes.c_long]), ('dvmh_amd_set_graph_C', [ctypes.c_long]),
+                     ('dvmh_amd_set_kernel_timing_C', [ctypes.c_long]), ('dvmh_amd_reset_kernel_stats_C', []),
+                     ('dvmh_amd_kernel_stats_C', [ctypes.c_char_p, ctypes.c_long, ctypes.POINTER(ctypes.c_long)]),
+                     ('dvmh_barrier', [])]:
+        getattr(rt, name).argtypes = at
+    rt.dvmh_amd_kernel_stats_C.restype = ctypes.c_long
+    return rt
+
+
+def kernel_stats(rt, cdv, lines):
+    launches, ns = 0, 0
+    out = (ctypes.c_long * 2)()
+    for line in lines:
+        if rt.dvmh_amd_kernel_stats_C(cdv, line, out):
+            launches += out[0]
+            ns += out[1]
+    return launches, ns
+
+
+def set_form(rt, mode, sync, graph=0):
+    rt.dvmh_amd_tuner_set_mode_C(mode)
+    rt.dvmh_amd_tuner_set_repeats_C(2)
+    rt.dvmh_amd_set_sync_mode_C(sync)
+    rt.dvmh_amd_set_graph_C(graph)
+
+
 class Jac3d:
-    def __init__(self):
-        self.rt = D.runtime()
+    def __init__(self, rt):
+        self.rt = rt
         self.P = D.Program('jac3d')
         self.setup = self.P.fn('setup', [ctypes.c_long, ctypes.c_ulonglong])
-        self.iterate = self.P.fn('iterate', [ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)],
-                                 ctypes.c_double)
-        self.fetch = self.P.fn('fetch', [ctypes.POINTER(ctypes.c_double)] * 2)
+        self.iterate = self.P.fn('iterate', [ctypes.c_int, _dp, _dp], ctypes.c_double)
+        self.fetch = self.P.fn('fetch', [_dp, _dp])
         self.teardown = self.P.fn('teardown', [])
-        for name, at in [('dvmh_amd_set_sync_mode_C', [ctypes.c_long]), ('dvmh_amd_set_graph_C', [ctypes.c_long]),
-                         ('dvmh_amd_set_kernel_timing_C', [ctypes.c_long]), ('dvmh_amd_reset_kernel_stats_C', []),
-                         ('dvmh_amd_kernel_stats_C', [ctypes.c_char_p, ctypes.c_long, ctypes.POINTER(ctypes.c_long)]),
-                         ('dvmh_amd_tuner_site_pending_C', [ctypes.c_char_p, ctypes.c_long]), ('dvmh_barrier', [])]:
-            getattr(self.rt, name).argtypes = at
-        self.rt.dvmh_amd_tuner_site_pending_C.restype = ctypes.c_long
-        self.rt.dvmh_amd_kernel_stats_C.restype = ctypes.c_long
-
-    def configure(self, form, tuner_mode):
-        self.rt.dvmh_amd_tuner_set_mode_C(tuner_mode)
-        if form == 'plain':
-            self.rt.dvmh_amd_set_sync_mode_C(0)
-            self.rt.dvmh_amd_set_graph_C(0)
-            self.P.set_targets(D.DEVICE_TYPE_HOST | D.DEVICE_TYPE_CUDA, 0)
-        else:
-            # One ~300 us kernel per time step: graph replay would save little and
-            # would hide the per-kernel events the roofline is measured with, so
-            # this workload's chosen form runs its regions eagerly.
-            self.rt.dvmh_amd_set_sync_mode_C(1)
-            self.rt.dvmh_amd_set_graph_C(0)
-            self.P.set_targets(D.DEVICE_TYPE_HOST | D.DEVICE_TYPE_CUDA, 0)
+        self.P.set_targets(D.DEVICE_TYPE_HOST | D.DEVICE_TYPE_CUDA, 0)
 
     def pending(self):
         return sum(self.rt.dvmh_amd_tuner_site_pending_C(b'jac3d.cdv', line) for line in STENCIL_LINES)
 
-    def kernel_stats(self):
-        launches, ns = 0, 0
-        out = (ctypes.c_long * 2)()
-        for line in STENCIL_LINES:
-            if self.rt.dvmh_amd_kernel_stats_C(b'jac3d.cdv', line, out):
-                launches += out[0]
-                ns += out[1]
-        return launches, ns
-
-    def run_form(self, form, tuner_mode, steps, warmup):
-        self.configure(form, tuner_mode)
+    def run_form(self, mode, sync, steps, warmup):
+        set_form(self.rt, mode, sync)
         self.setup(N, SEED)
-        # warm-up: at least `warmup` steps, and until the tuner has measured every candidate
+        # warm-up: at least `warmup` steps, and until the tuner has timed its candidates
         done = 0
-        while done < warmup or (tuner_mode in (1, 2) and self.pending() > 0):
+        while done < warmup or (mode in (1, 2) and self.pending() > 0):
             self.iterate(ITERS, None, None)
             done += 1
             if done > warmup + 40:
                 raise RuntimeError('tuner did not settle')
-        # one more step so that repeating regions are captured before timing
-        if form == 'chosen':
-            self.iterate(ITERS, None, None)
         self.rt.dvmh_amd_reset_kernel_stats_C()
         self.rt.dvmh_amd_set_kernel_timing_C(1)
         eps = np.zeros(ITERS)
@@ -109,39 +111,121 @@ class Jac3d:
         self.rt.dvmh_barrier()
         t0 = time.perf_counter()
         for _ in range(steps):
-            self.iterate(ITERS, eps.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), s.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+            self.iterate(ITERS, eps.ctypes.data_as(_dp), s.ctypes.data_as(_dp))
         self.rt.dvmh_barrier()
         t1 = time.perf_counter()
         self.rt.dvmh_amd_set_kernel_timing_C(0)
-        launches, ns = self.kernel_stats()
+        launches, ns = kernel_stats(self.rt, b'jac3d.cdv', STENCIL_LINES)
         self.teardown()
         sec = t1 - t0
-        value = PTS * ITERS * steps / sec
         kernel_s = ns * 1e-9 / launches if launches else None
-        return {'value': value, 'ms_per_step': sec / steps * 1e3, 'kernel_launches': launches,
+        return {'value': PTS * ITERS * steps / sec, 'ms_per_step': sec / steps * 1e3, 'kernel_launches': launches,
                 'kernel_us': kernel_s * 1e6 if kernel_s else None,
                 'achieved_GBps': PTS * B_ALG / kernel_s / 1e9 if kernel_s else None,
-                'warmup_steps': done, 'last_eps': float(eps[-1]), 'last_sum': float(s[-1])}
+                'frac': PTS * B_ALG / kernel_s / HBM_PEAK if kernel_s else None, 'warmup_steps': done}
+
+    def parity(self, mode, iters=6):
+        """The given form against the oracle at full size for a few iterations:
+        grid bit-exact, MAX exact, SUM within 1e-12 relative."""
+        import helpers as H
+        set_form(self.rt, mode, 1)
+        self.setup(N, SEED)
+        eps = np.zeros(iters)
+        s = np.zeros(iters)
+        self.iterate(iters, eps.ctypes.data_as(_dp), s.ctypes.data_as(_dp))
+        A = np.zeros(N ** 3)
+        B = np.zeros(N ** 3)
+        self.fetch(A.ctypes.data_as(_dp), B.ctypes.data_as(_dp))
+        self.teardown()
+        ref = H.oracle_jac3d(N, iters)
+        return {'iterations': iters,
+                'grid_bit_exact': bool(H.bits_equal(A, ref['A']) and H.bits_equal(B, ref['B'])),
+                'max_exact': bool(np.array_equal(eps, ref['eps'])),
+                'sum_max_rel_err': float(np.max(np.abs(s - ref['sum']) / np.abs(ref['sum'])))}
 
 
-def parity_check(bench, iters=6):
-    """Chosen form (graph capture + replay included) against the oracle at
-    full size for a few iterations: grid bit-exact, MAX exact, SUM within 1e-12."""
+def batched_leg(rt, iters):
+    """512 uneven 2-D Jacobi grids: per-grid launch sequences against one fused
+    launch per loop site per iteration; both bit-exact (checked on the
+    smallest and largest grid against the oracle)."""
     import helpers as H
-    bench.configure('chosen', bench.mode_chosen)
-    bench.setup(N, SEED)
-    eps = np.zeros(iters)
-    s = np.zeros(iters)
-    bench.iterate(iters, eps.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), s.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
-    A = np.zeros(N ** 3)
-    B = np.zeros(N ** 3)
-    bench.fetch(A.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), B.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
-    bench.teardown()
-    ref = H.oracle_jac3d(N, iters)
-    return {'iterations': iters,
-            'grid_bit_exact': bool(H.bits_equal(A, ref['A']) and H.bits_equal(B, ref['B'])),
-            'max_exact': bool(np.array_equal(eps, ref['eps'])),
-            'sum_max_rel_err': float(np.max(np.abs(s - ref['sum']) / np.abs(ref['sum'])))}
+    P = D.Program('batched')
+    P.set_targets(D.DEVICE_TYPE_HOST | D.DEVICE_TYPE_CUDA, 0)
+    count = 512
+    setup = P.fn('setup', [ctypes.c_int, ctypes.c_void_p, ctypes.c_ulonglong])
+    it = P.fn('iterate', [ctypes.c_int, ctypes.c_int, _dp], ctypes.c_double)
+    side = P.fn('side', [ctypes.c_int], ctypes.c_long)
+    fetch = P.fn('fetch', [ctypes.c_int, _dp, _dp])
+    res = {}
+    set_form(rt, 0, 0)
+    setup(count, None, D.SEED_BASE + 5)
+    sides = np.array([side(g) for g in range(count)])
+    pts = int(np.sum((sides - 2) ** 2))
+    for name, batched, sync in (('per_grid', 0, 0), ('batched', 1, 1)):
+        rt.dvmh_amd_set_sync_mode_C(sync)
+        it(2, batched, None)
+        rt.dvmh_amd_reset_kernel_stats_C()
+        rt.dvmh_amd_set_kernel_timing_C(1)
+        sec = it(iters, batched, None)
+        rt.dvmh_amd_set_kernel_timing_C(0)
+        launches, ns = kernel_stats(rt, b'batched.cdv', (31, 36))
+        res[name] = {'value': pts * iters / sec, 'ms_per_iteration': sec / iters * 1e3, 'kernel_launches': launches,
+                     'kernel_ms_per_iteration': ns * 1e-6 / iters,
+                     'achieved_GBps': pts * 40 * iters / (ns * 1e-9) / 1e9 if ns else None}
+    P.fn('teardown', [])()
+    # parity: a fresh batched run of 5 iterations, smallest and largest grid
+    set_form(rt, 0, 1)
+    setup(count, None, D.SEED_BASE + 5)
+    eps = np.zeros(5 * count)
+    it(5, 1, eps.ctypes.data_as(_dp))
+    eps = eps.reshape(5, count)
+    picks = [int(np.argmin(sides)), int(np.argmax(sides))]
+    ok = True
+    for r in H.oracle_batched(sides, 5, grids=picks):
+        n = int(sides[r['g']])
+        A = np.zeros(n * n)
+        Bb = np.zeros(n * n)
+        fetch(r['g'], A.ctypes.data_as(_dp), Bb.ctypes.data_as(_dp))
+        ok = ok and H.bits_equal(A, r['A']) and H.bits_equal(Bb, r['B']) and np.array_equal(eps[:, r['g']], r['eps'])
+    P.fn('teardown', [])()
+    rt.dvmh_amd_set_sync_mode_C(0)
+    return {'workload': '512 uneven 2-D Jacobi grids (sides 64..1024 by seed), FP64, 5-point + REDUCTION(MAX) per grid',
+            'points_per_iteration': pts, 'iterations': iters, 'bytes_per_point_update': 40,
+            'per_grid': res['per_grid'], 'batched': res['batched'],
+            'speedup': res['batched']['value'] / res['per_grid']['value'], 'parity_bit_exact': bool(ok)}
+
+
+def other_configs(rt):
+    """The remaining configurations, plain against the cost-model form, one
+    timed run each (their parity is covered by the GPU test suite)."""
+    from dvmh_amd import workloads as W
+    specs = [('jac2d', 2048, 100, 16 + 24, 'C1 2-D Jacobi 2048^2 x 100 (two loops per iteration)'),
+             ('sor3d', 320, 30, 16, 'C3 SOR 320^3 x 30 (ACROSS on three axes, REDUCTION(MAX))'),
+             ('adi3d', 256, 20, 6 * 16 + 8, 'C4 ADI 256^3 x 20 (six line sweeps + MAXLOC)')]
+    out = {}
+    for name, n, iters, bpp, desc in specs:
+        r = {'workload': desc}
+        for form, mode, sync in (('plain', 0, 0), ('chosen', 2, 1)):
+            set_form(rt, mode, sync)
+            w = W.ALL[name](n)
+            w.setup()
+            for _ in range(40):
+                w.run(max(2, iters // 10))
+                if mode == 0 or sum(rt.dvmh_amd_tuner_site_pending_C(w.cdv, ln) for ln in w.lines) == 0:
+                    break
+            rt.dvmh_amd_reset_kernel_stats_C()
+            rt.dvmh_amd_set_kernel_timing_C(1)
+            sec = w.run(iters)
+            rt.dvmh_amd_set_kernel_timing_C(0)
+            launches, ns = kernel_stats(rt, w.cdv, w.lines)
+            w.teardown()
+            pts = w.points_per_iter()
+            r[form] = {'value': pts * iters / sec, 'ms': sec * 1e3, 'kernel_ms': ns * 1e-6,
+                       'frac': pts * iters * bpp / (ns * 1e-9) / HBM_PEAK if ns else None}
+        r['speedup'] = r['chosen']['value'] / r['plain']['value']
+        out[name] = r
+    set_form(rt, 0, 0)
+    return out
 
 
 def cpu_baseline(target_seconds=12.0):
@@ -152,18 +236,50 @@ def cpu_baseline(target_seconds=12.0):
     A = np.zeros(N ** 3)
     o.orc_init_grid3d(H.vp(A), N, SEED, 0.0, 1)
     B = A.copy()
-    eps = np.zeros(64)
-    s = np.zeros(64)
+    eps = np.zeros(1)
+    s = np.zeros(1)
     t0 = time.perf_counter()
     o.orc_jac3d(H.vp(A), H.vp(B), N, 1, H.vp(eps), H.vp(s), cores)
     one = time.perf_counter() - t0
     its = int(max(1, min(4000, target_seconds / max(one, 1e-3))))
+    eps = np.zeros(its)   # the oracle writes one history entry per iteration
+    s = np.zeros(its)
     t0 = time.perf_counter()
     o.orc_jac3d(H.vp(A), H.vp(B), N, its, H.vp(eps), H.vp(s), cores)
     sec = time.perf_counter() - t0
     return {'value': PTS * its / sec, 'unit': 'point-updates/s', 'cores': cores, 'kind': 'port',
             'sample': f'{its} iterations of the 384^3 sweep (oracle C/OpenMP host-handler restatement, '
                       f'{sec:.1f} s on {cores} threads)'}
+
+
+def measured_traffic():
+    """HBM bytes per stencil launch from the committed rocprofv3 PMC summary
+    (FETCH_SIZE and WRITE_SIZE, scaled by the copy-kernel calibration it states)."""
+    if not os.path.exists(PMC_SUMMARY):
+        return None
+    m = re.search(r'traffic_bytes_per_launch:\s*([0-9.eE+]+)', open(PMC_SUMMARY).read())
+    return float(m.group(1)) if m else None
+
+
+def calibrate_copy(nbytes=1 << 30):
+    """One device-to-device copy of nbytes (a blit dispatch of known traffic:
+    nbytes read, nbytes written), so FETCH_SIZE / WRITE_SIZE counters of a
+    profiling run can be scaled to bytes."""
+    hip = ctypes.CDLL('libamdhip64.so')
+    hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    hip.hipMemset.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t]
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    hip.hipFree.argtypes = [ctypes.c_void_p]
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(a), nbytes) or hip.hipMalloc(ctypes.byref(b), nbytes):
+        raise RuntimeError('hipMalloc failed')
+    hip.hipMemset(a, 1, nbytes)
+    hip.hipDeviceSynchronize()
+    hip.hipMemcpy(b, a, nbytes, 3)   # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log('calibration copy of', nbytes, 'bytes done')
 
 
 def main():
@@ -173,29 +289,46 @@ def main():
     ap.add_argument('--warmup', type=int, default=2)
     ap.add_argument('--skip-cpu', action='store_true', help='omit the CPU baseline leg')
     ap.add_argument('--skip-parity', action='store_true')
+    ap.add_argument('--skip-others', action='store_true', help='omit the batched-grid and other-config legs')
+    ap.add_argument('--chosen-only', action='store_true',
+                    help='run only the chosen form (for profiling runs); prints the JSON line without the other legs')
+    ap.add_argument('--calibrate-copy', action='store_true',
+                    help='also run one 1 GiB device-to-device copy (scale reference for HBM counters)')
     args = ap.parse_args()
     if args.gpus != 1:
         print(f'bench.py: this build runs on one GPU only (the path is not sharded); --gpus {args.gpus} is not supported',
               file=sys.stderr)
         sys.exit(2)
-    bench = Jac3d()
+    rt = bind()
+    if not os.path.exists(CM.MODEL) or rt.dvmh_amd_model_load_C(CM.MODEL.encode()) < 0:
+        raise SystemExit(f'bench.py: cannot load the trained cost model {CM.MODEL}')
     st = D.stats()
     if not st['gpu_present']:
         raise SystemExit('bench.py: no GPU visible to libdvmh_amd')
-    has_model = os.path.exists(MODEL_FILE)
-    if has_model:
-        os.environ['DVMH_AMD_MODEL'] = MODEL_FILE
-    bench.mode_chosen = 2 if has_model else 1
     log('device', D.device_name(), 'CUs', st['cu_count'])
-    plain = bench.run_form('plain', 0, args.steps, args.warmup)
-    log('plain', plain)
-    chosen = bench.run_form('chosen', bench.mode_chosen, args.steps, args.warmup)
+    if args.calibrate_copy:
+        calibrate_copy()
+    bench = Jac3d(rt)
+    plain = exhaustive = None
+    if not args.chosen_only:
+        plain = bench.run_form(0, 0, args.steps, args.warmup)
+        log('plain', plain)
+        rt.dvmh_amd_tuner_clear_C()
+        exhaustive = bench.run_form(1, 1, args.steps, args.warmup)
+        log('exhaustive', exhaustive)
+        rt.dvmh_amd_tuner_clear_C()
+    chosen = bench.run_form(2, 1, args.steps, args.warmup)
     log('chosen', chosen)
-    parity = None if args.skip_parity else parity_check(bench)
+    parity = None if (args.skip_parity or args.chosen_only) else bench.parity(2)
     log('parity', parity)
-    cpu = None if args.skip_cpu else cpu_baseline()
+    batched = others = None
+    if not (args.skip_others or args.chosen_only):
+        batched = batched_leg(rt, 20)
+        log('batched', batched)
+        others = other_configs(rt)
+        log('others', others)
+    cpu = None if (args.skip_cpu or args.chosen_only) else cpu_baseline()
     log('cpu', cpu)
-    st = D.stats()
     kernel_s = chosen['kernel_us'] * 1e-6
     achieved = PTS * B_ALG / kernel_s
     out = {
@@ -212,16 +345,19 @@ def main():
         'dtype': 'f64',
         'data': 'synthetic (seeded SplitMix64 interior, zero boundary)',
         'config': {'workload': '3-D 7-point Jacobi, FP64 384^3, 50 iterations per step, REDUCTION(MAX)+REDUCTION(SUM)',
-                   'form': 'chosen', 'tuner': 'cost-model' if has_model else 'exhaustive',
-                   'graph_replays': st['graph_replays']},
+                   'form': 'chosen (learned cost model: top-4 + default timed)'},
         'roofline': {'bound': 'hbm', 'achieved': achieved / 1e9, 'peak': HBM_PEAK / 1e9, 'unit': 'GB/s',
-                     'frac': achieved / HBM_PEAK, 'traffic': None,
+                     'frac': achieved / HBM_PEAK, 'traffic': measured_traffic(),
                      'kernel': 'jac3d stencil loop (loop_jac3d_41/54_kernel)', 'kernel_us': chosen['kernel_us'],
-                     'bytes_per_point_update': B_ALG},
-        'plain': {'value': plain['value'], 'ms_per_step': plain['ms_per_step'], 'kernel_us': plain['kernel_us'],
-                  'frac': (PTS * B_ALG / (plain['kernel_us'] * 1e-6)) / HBM_PEAK if plain['kernel_us'] else None},
-        'speedup_vs_plain': chosen['value'] / plain['value'],
+                     'bytes_per_point_update': B_ALG, 'point_updates_per_launch': PTS},
+        'plain': plain,
+        'exhaustive': exhaustive,
+        'speedup_vs_plain': chosen['value'] / plain['value'] if plain else None,
+        'kernel_speedup_vs_plain': plain['kernel_us'] / chosen['kernel_us'] if plain else None,
+        'cost_model_vs_exhaustive_kernel': chosen['kernel_us'] / exhaustive['kernel_us'] if exhaustive else None,
         'parity': parity,
+        'batched_grids': batched,
+        'other_configs': others,
         'cpu_baseline': cpu,
     }
     print(json.dumps(out), flush=True)

@@ -49,6 +49,23 @@ void dvmh_hip_diagonal(const void *src, void *dst, long typeSize, long Rx, long 
 void dvmh_hip_copy_block(const void *src, const long *srcSizes, const long *srcOrigin, void *dst, const long *dstSizes,
         const long *dstOrigin, const long *lo, const long *hi, int rank, long typeSize, hipStream_t s);
 
+// Cost-model perceptron (mlp.hip): F inputs, two ReLU layers of H units, one
+// output; parameters W1[H][F] b1[H] W2[H][H] b2[H] W3[H] b3 as one FP32 vector.
+#define DVMH_MLP_F 24
+#define DVMH_MLP_H 64
+#define DVMH_MLP_NPARAM (DVMH_MLP_H * DVMH_MLP_F + DVMH_MLP_H + DVMH_MLP_H * DVMH_MLP_H + DVMH_MLP_H + DVMH_MLP_H + 1)
+// Gradient of mean((out - y)^2) over n samples (X row-major n x F) into grad
+// (NPARAM floats) and the loss into *loss; scratch of
+// dvmh_hip_mlp_grad_scratch_bytes(n) bytes.  Deterministic.
+long dvmh_hip_mlp_grad_scratch_bytes(long n);
+void dvmh_hip_mlp_grad(const float *P, const float *X, const float *Y, long n, float *grad, float *loss, void *scratch,
+        hipStream_t s);
+// One Adam step; c1 = 1 - beta1^t, c2 = 1 - beta2^t for step t (from 1).
+void dvmh_hip_mlp_adam(float *P, const float *G, float *M, float *V, float lr, float beta1, float beta2, float eps, float c1,
+        float c2, hipStream_t s);
+// out[i] = network(X[i]) for n samples.
+void dvmh_hip_mlp_forward(const float *P, const float *X, long n, float *out, hipStream_t s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -135,6 +135,9 @@ bool alignToRoot(const Data *d, const Object *root, long tAxisOfArrayAxis[], lon
     }
     return false;
 }
+
+// Lazy initialisation for entry points that may be the first call into the library.
+void runtimeInit() { ensureInit(); }
 }  // namespace dvmh
 
 // ============================================================ init / exit

@@ -262,22 +262,16 @@ void performLoopNow(Loop *loop) {
     loop->spec = &sl;
     DvmType ref = (DvmType)&sl;
     if (onGpu) {
-        hipEvent_t ev0 = nullptr, ev1 = nullptr;
-        bool timing = settings.tuneMode != 0 && !graphCapturing();
-        if (timing) {
-            HIP_CHECK(hipEventCreate(&ev0));
-            HIP_CHECK(hipEventCreate(&ev1));
-            HIP_CHECK(hipEventRecord(ev0, sl.stream));
-        }
         callHandler(loop->gpuH, &ref);
         rt.loopsOnGpu++;
-        if (timing) {
+        // executions the tuner explores are timed from its pick to here; settled
+        // sites run without events or synchronisation
+        if (sl.explore && sl.tuneStart && !graphCapturing()) {
+            hipEvent_t ev1 = tunerEvent(1);
             HIP_CHECK(hipEventRecord(ev1, sl.stream));
             HIP_CHECK(hipEventSynchronize(ev1));
             float ms = 0;
-            HIP_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
-            HIP_CHECK(hipEventDestroy(ev0));
-            HIP_CHECK(hipEventDestroy(ev1));
+            HIP_CHECK(hipEventElapsedTime(&ms, sl.tuneStart, ev1));
             rt.gpuLoopSeconds += ms * 1e-3;
             if (sl.candidate >= 0) tunerRecord(&sl, ms * 1e-3);
         }

@@ -52,6 +52,7 @@ struct Settings {
     int syncMode = 0;           // DVMH_AMD_SYNC: 0 synchronise after every loop, 1 only when results are needed
     bool graph = false;         // DVMH_AMD_GRAPH: capture repeating regions into HIP graphs
     int tuneRepeats = 1;        // DVMH_AMD_TUNE_REPEATS: timed executions per candidate
+    bool tuneTransforms = false;   // DVMH_AMD_TUNE_TRANSFORMS: candidates may re-lay out arrays of ACROSS loops
     std::string tuneDb;         // DVMH_AMD_TUNEDB: file of recorded timings
     std::string modelFile;      // DVMH_AMD_MODEL: trained cost-model weights
     void load();
@@ -211,6 +212,8 @@ struct SpecLoop : Object {
     DvmType tune[DVMH_AMD_TUNE_N] = {0};
     int candidate = -1;             // index in the tuner's candidate list
     std::string siteKey;            // tuner site of this execution
+    bool explore = false;           // the tuner times this execution (candidate still lacks repeats)
+    hipEvent_t tuneStart = nullptr; // recorded at the pick when exploring
     std::vector<void *> toFree;
     SpecLoop(Loop *l, int dev): Object(Kind::SpecLoop), loop(l), device(dev) {}
 };
@@ -286,6 +289,7 @@ struct TuneChoice {
 };
 void tunerPick(SpecLoop *sl, long kind, TuneChoice &out);
 void tunerRecord(SpecLoop *sl, double seconds);
+hipEvent_t tunerEvent(int which);   // 0 start, 1 stop of an explored execution
 void tunerShutdown();
 
 // batching and graphs (batch.cpp)
@@ -306,6 +310,7 @@ char *graphPinnedSlot(size_t bytes);
 void graphNotePost(const std::shared_ptr<Reduction> &r, char *val, char *loc);
 void graphRunRegion(Region *region);
 void graphReleaseAll();
+void runtimeInit();   // settings, GPU and tuner set-up on first use
 void addPendingPost(const std::shared_ptr<Reduction> &r, char *val, char *loc, bool addInitial = false);
 
 }  // namespace dvmh

@@ -58,6 +58,7 @@ bool kernelTimingOn() { return timingOn; }
 using namespace dvmh;
 
 extern "C" void dvmh_amd_mark_kernel_C(DvmType curLoop, DvmType phase) {
+    runtimeInit();
     if (!timingOn) return;
     SpecLoop *sl = asSpec(curLoop, "dvmh_amd_mark_kernel_C");
     if (sl->device != 1) return;
@@ -77,11 +78,13 @@ extern "C" void dvmh_amd_mark_kernel_C(DvmType curLoop, DvmType phase) {
 }
 
 extern "C" void dvmh_amd_set_kernel_timing_C(DvmType enable) {
+    runtimeInit();
     if (!enable) drainKernelTimes();
     timingOn = enable != 0;
 }
 
 extern "C" DvmType dvmh_amd_kernel_stats_C(const char *file, DvmType line, DvmType out[2]) {
+    runtimeInit();
     drainKernelTimes();
     auto it = siteTimes.find(std::string(file) + ":" + std::to_string(line));
     if (it == siteTimes.end()) return 0;
@@ -91,11 +94,13 @@ extern "C" DvmType dvmh_amd_kernel_stats_C(const char *file, DvmType line, DvmTy
 }
 
 extern "C" void dvmh_amd_reset_kernel_stats_C(void) {
+    runtimeInit();
     drainKernelTimes();
     siteTimes.clear();
 }
 
 extern "C" void dvmh_amd_set_sync_mode_C(DvmType mode) {
+    runtimeInit();
     flushDeferredPosts();
     settings.syncMode = (int)mode;
 }

@@ -44,6 +44,7 @@ static void addCand(std::vector<Candidate> &v, int bx, int by, int bz, std::init
 std::vector<Candidate> makeCandidates(int kind, int loopRank, int depCount, const long extents[]) {
     std::vector<Candidate> v;
     int mapped = loopRank - (kind == DVMH_AMD_KIND_PARALLEL ? 0 : depCount);   // axes spread over threads
+    const int tfMax = settings.tuneTransforms ? 1 : 0;   // also try device layout transforms
     if (kind == DVMH_AMD_KIND_PARALLEL) {
         if (loopRank >= 3) addCand(v, 32, 4, 2, {});
         else if (loopRank == 2) addCand(v, 32, 8, 1, {});
@@ -69,31 +70,30 @@ std::vector<Candidate> makeCandidates(int kind, int loopRank, int depCount, cons
         // threads over the independent axes, each walks its line
         if (mapped >= 2) addCand(v, 32, 8, 1, {});
         else addCand(v, 256, 1, 1, {});
-        const int xs[] = {64, 128, 256};
-        const int ys[] = {1, 2, 4};
-        const int chunks[] = {0, 8, 16, 32};
+        const int xs[] = {32, 64, 128, 256};
+        const int ys[] = {1, 2, 4, 8};
         for (int bx : xs)
             for (int by : ys) {
                 if (mapped < 2 && by > 1) continue;
                 int n = bx * by;
                 if (n < 64 || n > 1024) continue;
-                for (int ch : chunks)
-                    for (int tf = 0; tf <= 1; tf++)
-                        addCand(v, bx, by, 1, {{DVMH_AMD_T_CHUNK, ch}, {DVMH_AMD_T_TRANSFORM, tf}});
+                for (int xcd = 0; xcd <= 1; xcd++)
+                    for (int tf = 0; tf <= tfMax; tf++)
+                        addCand(v, bx, by, 1, {{DVMH_AMD_T_XCD, xcd}, {DVMH_AMD_T_TRANSFORM, tf}});
             }
     } else {
-        // wavefront over tiles: tile extents on the two innermost dependent axes
-        // (A: second innermost, B: innermost) and along the outer dependent axis (CHUNK)
-        addCand(v, 64, 1, 1, {{DVMH_AMD_T_TILE_A, 4}, {DVMH_AMD_T_TILE_B, 64}, {DVMH_AMD_T_CHUNK, 4}});
-        const int tas[] = {2, 4, 8, 16};
-        const int tbs[] = {32, 64, 128};
-        const int tcs[] = {2, 4, 8};
-        for (int ta : tas)
-            for (int tb : tbs)
-                for (int tc : tcs)
-                    for (int var = 0; var <= 1; var++)
-                        addCand(v, tb < 64 ? 64 : tb, 1, 1,
-                                {{DVMH_AMD_T_TILE_A, ta}, {DVMH_AMD_T_TILE_B, tb}, {DVMH_AMD_T_CHUNK, tc}, {DVMH_AMD_T_VARIANT, var}});
+        // one launch per hyperplane: threads over the free dependent axes (y only
+        // with three or more dependent axes), optional diagonal layout
+        addCand(v, 64, 1, 1, {});
+        const int xs[] = {32, 64, 128, 256};
+        const int ys[] = {1, 2, 4, 8};
+        for (int bx : xs)
+            for (int by : ys) {
+                if (depCount < 3 && by > 1) continue;
+                int n = bx * by;
+                if (n < 64 || n > 512) continue;
+                for (int tf = 0; tf <= tfMax; tf++) addCand(v, bx, by, 1, {{DVMH_AMD_T_TRANSFORM, tf}});
+            }
     }
     (void)extents;
     return v;
@@ -147,6 +147,29 @@ int SiteInfo::bestMeasured() const {
 
 extern int modelRank(SiteInfo &s, std::vector<int> &order);   // cost model (model.cpp)
 
+// Candidates the cost-model mode times: the default and the model's best K.
+std::vector<int> SiteInfo::trialSet() const {
+    const int K = 4;
+    std::vector<int> trial{0};
+    for (int i = 0; i < (int)modelOrder.size() && (int)trial.size() < K + 1; i++)
+        if (modelOrder[i] != 0) trial.push_back(modelOrder[i]);
+    return trial;
+}
+
+// Candidates of the site still lacking their timed repeats in the current mode.
+long SiteInfo::pending() const {
+    long p = 0;
+    if (settings.tuneMode == 2 && modelRanked) {
+        for (int c : trialSet())
+            if ((int)times[c].size() < settings.tuneRepeats) p++;
+        return p;
+    }
+    if (settings.tuneMode != 1 && settings.tuneMode != 2) return 0;
+    for (size_t c = 0; c < cands.size(); c++)
+        if ((int)times[c].size() < settings.tuneRepeats) p++;
+    return p;
+}
+
 void tunerPick(SpecLoop *sl, long kind, TuneChoice &out) {
     Loop *l = sl->loop;
     int chosen = 0;
@@ -180,12 +203,8 @@ void tunerPick(SpecLoop *sl, long kind, TuneChoice &out) {
         if (chosen < 0) chosen = s->bestMeasured();
     } else {
         // model: measure the top-K predictions (plus the default), then keep the best
-        const int K = 4;
         chosen = -1;
-        std::vector<int> trial;
-        trial.push_back(0);
-        for (int i = 0; i < (int)s->modelOrder.size() && (int)trial.size() < K + 1; i++)
-            if (s->modelOrder[i] != 0) trial.push_back(s->modelOrder[i]);
+        std::vector<int> trial = s->trialSet();
         for (int c : trial)
             if ((int)s->times[c].size() < settings.tuneRepeats) { chosen = c; break; }
         if (chosen < 0) {
@@ -196,6 +215,14 @@ void tunerPick(SpecLoop *sl, long kind, TuneChoice &out) {
             }
         }
     }
+    // a candidate still lacking its timed repeats is being explored: time this
+    // execution from here (before the handler launches anything) to its end
+    if ((settings.tuneMode == 1 || settings.tuneMode == 2) && (int)s->times[chosen].size() < settings.tuneRepeats &&
+            !graphCapturing() && sl->stream) {
+        sl->explore = true;
+        sl->tuneStart = tunerEvent(0);
+        HIP_CHECK(hipEventRecord(sl->tuneStart, sl->stream));
+    }
     const Candidate &c = s->cands[chosen];
     for (int i = 0; i < 3; i++) out.block[i] = c.block[i];
     for (int i = 0; i < DVMH_AMD_TUNE_N; i++) out.tune[i] = c.tune[i];
@@ -203,6 +230,13 @@ void tunerPick(SpecLoop *sl, long kind, TuneChoice &out) {
     s->lastPicked = chosen;
     sl->candidate = chosen;
     sl->siteKey = s->key;
+}
+
+// Two reusable events for timing explored executions (one execution at a time).
+hipEvent_t tunerEvent(int which) {
+    static hipEvent_t ev[2] = {nullptr, nullptr};
+    if (!ev[which]) HIP_CHECK(hipEventCreate(&ev[which]));
+    return ev[which];
 }
 
 void tunerRecord(SpecLoop *sl, double seconds) {
@@ -271,17 +305,28 @@ long tunerLoad(const std::string &path) {
             s.times.assign(s.cands.size(), std::vector<double>());
             it = sites.emplace(key, s).first;
         }
-        int c = atoi(cs.c_str());
-        if (c >= 0 && c < (int)it->second.times.size()) {
-            it->second.times[c].push_back(atof(secs.c_str()));
-            n++;
-        }
+        // match the candidate by its configuration (the list depends on settings)
+        Candidate want;
+        std::stringstream bss(bs), tss(ts);
+        for (int i = 0; i < 3; i++) bss >> want.block[i];
+        for (int i = 0; i < DVMH_AMD_TUNE_N; i++) tss >> want.tune[i];
+        if (!bss || !tss) continue;
+        const auto &cands = it->second.cands;
+        for (size_t c = 0; c < cands.size(); c++)
+            if (cands[c].same(want)) {
+                it->second.times[c].push_back(atof(secs.c_str()));
+                n++;
+                break;
+            }
     }
     return n;
 }
 
+void modelInit();   // model.cpp
+
 void tunerInit() {
     if (!settings.tuneDb.empty()) tunerLoad(settings.tuneDb);
+    modelInit();
 }
 
 void tunerShutdown() {
@@ -293,19 +338,25 @@ void tunerShutdown() {
 using namespace dvmh;
 
 extern "C" void dvmh_amd_tuner_set_mode_C(DvmType mode) {
+    runtimeInit();
     DVMH_CHECK(mode >= 0 && mode <= 3, "Tuner mode must be 0..3");
     settings.tuneMode = (int)mode;
 }
 
-extern "C" DvmType dvmh_amd_tuner_save_C(const char *path) { return tunerSave(path); }
-extern "C" DvmType dvmh_amd_tuner_load_C(const char *path) { return tunerLoad(path); }
-extern "C" void dvmh_amd_tuner_clear_C(void) { sites.clear(); }
+extern "C" void dvmh_amd_tuner_set_repeats_C(DvmType repeats) {
+    runtimeInit();
+    DVMH_CHECK(repeats >= 1, "Tuner repeats must be >= 1");
+    settings.tuneRepeats = (int)repeats;
+}
+
+extern "C" DvmType dvmh_amd_tuner_save_C(const char *path) { runtimeInit(); return tunerSave(path); }
+extern "C" DvmType dvmh_amd_tuner_load_C(const char *path) { runtimeInit(); return tunerLoad(path); }
+extern "C" void dvmh_amd_tuner_clear_C(void) { runtimeInit(); sites.clear(); }
 
 extern "C" DvmType dvmh_amd_tuner_pending_C(void) {
+    runtimeInit();
     long pending = 0;
-    for (auto &kv : sites)
-        for (size_t c = 0; c < kv.second.cands.size(); c++)
-            if ((int)kv.second.times[c].size() < settings.tuneRepeats) pending++;
+    for (auto &kv : sites) pending += kv.second.pending();
     return pending;
 }
 
@@ -316,16 +367,16 @@ long tunerSitePending(const std::string &file, long line) {
     long pending = 0;
     for (auto &kv : sites) {
         if (kv.first.compare(0, prefix.size(), prefix) != 0) continue;
-        for (size_t c = 0; c < kv.second.cands.size(); c++)
-            if ((int)kv.second.times[c].size() < settings.tuneRepeats) pending++;
+        pending += kv.second.pending();
     }
     return pending;
 }
 }  // namespace dvmh
 
-extern "C" DvmType dvmh_amd_tuner_site_pending_C(const char *file, DvmType line) { return tunerSitePending(file, line); }
+extern "C" DvmType dvmh_amd_tuner_site_pending_C(const char *file, DvmType line) { runtimeInit(); return tunerSitePending(file, line); }
 
 extern "C" DvmType dvmh_amd_tuner_query_C(const char *file, DvmType line, DvmType out[10]) {
+    runtimeInit();
     std::string prefix = std::string(file) + ":" + std::to_string(line) + "|";
     for (auto &kv : sites) {
         if (kv.first.compare(0, prefix.size(), prefix) != 0) continue;

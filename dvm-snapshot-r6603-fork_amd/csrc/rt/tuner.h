@@ -30,6 +30,8 @@ struct SiteInfo {
     int lastPicked = -1;
     double measured(int c) const;
     int bestMeasured() const;
+    std::vector<int> trialSet() const;   // cost-model mode: default + the model's best few
+    long pending() const;                // candidates still to time in the current mode
 };
 
 std::vector<Candidate> makeCandidates(int kind, int loopRank, int depCount, const long extents[]);

@@ -1,0 +1,82 @@
+"""The converted workload programs as Python objects: setup / run / teardown
+through their C entry points, plus the source positions (file, line) of the
+loops their iteration phase executes, which the tuner keys its sites by."""
+import ctypes
+
+from . import DEVICE_TYPE_CUDA, DEVICE_TYPE_HOST, SEED_BASE, Program
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_lp = ctypes.POINTER(ctypes.c_long)
+
+
+class Workload:
+    name = ''
+    cdv = b''
+    lines = ()
+    seed_offset = 0
+
+    def __init__(self, size, targets=DEVICE_TYPE_HOST | DEVICE_TYPE_CUDA, flags=0):
+        self.size = size
+        self.P = Program(self.name)
+        self.P.set_targets(targets, flags)
+
+    def setup(self):
+        raise NotImplementedError
+
+    def run(self, iters):
+        raise NotImplementedError
+
+    def teardown(self):
+        self.P.fn('teardown', [])()
+
+    def points_per_iter(self):
+        return (self.size - 2) ** 3
+
+
+class Jac2d(Workload):
+    name, cdv, lines, seed_offset = 'jac2d', b'jac2d.cdv', (38, 43), 1
+
+    def setup(self):
+        self.P.fn('setup', [ctypes.c_long, ctypes.c_ulonglong])(self.size, SEED_BASE + self.seed_offset)
+
+    def run(self, iters):
+        return self.P.fn('iterate', [ctypes.c_int, _dp], ctypes.c_double)(iters, None)
+
+    def points_per_iter(self):
+        return (self.size - 2) ** 2
+
+
+class Jac3d(Workload):
+    name, cdv, lines, seed_offset = 'jac3d', b'jac3d.cdv', (41, 54), 2
+
+    def setup(self):
+        self.P.fn('setup', [ctypes.c_long, ctypes.c_ulonglong])(self.size, SEED_BASE + self.seed_offset)
+
+    def run(self, iters):
+        return self.P.fn('iterate', [ctypes.c_int, _dp, _dp], ctypes.c_double)(iters, None, None)
+
+
+class Sor3d(Workload):
+    name, cdv, lines, seed_offset = 'sor3d', b'sor3d.cdv', (32,), 3
+
+    def setup(self):
+        self.P.fn('setup', [ctypes.c_long, ctypes.c_double, ctypes.c_ulonglong])(self.size, 1.5, SEED_BASE + self.seed_offset)
+
+    def run(self, iters):
+        return self.P.fn('iterate', [ctypes.c_int, _dp], ctypes.c_double)(iters, None)
+
+
+class Adi3d(Workload):
+    name, cdv, lines, seed_offset = 'adi3d', b'adi3d.cdv', (34, 39, 44, 49, 54, 59, 64), 4
+
+    def setup(self):
+        self.P.fn('setup', [ctypes.c_long, ctypes.c_double, ctypes.c_ulonglong])(self.size, 0.3, SEED_BASE + self.seed_offset)
+
+    def run(self, iters):
+        return self.P.fn('iterate', [ctypes.c_int, _dp, _lp], ctypes.c_double)(iters, None, None)
+
+    def points_per_iter(self):
+        return self.size ** 3
+
+
+ALL = {'jac2d': Jac2d, 'jac3d': Jac3d, 'sor3d': Sor3d, 'adi3d': Adi3d}

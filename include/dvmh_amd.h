@@ -56,8 +56,8 @@ void dvmh_amd_set_graph_C(DvmType enable);
 
 /* ---- statistics ------------------------------------------------------------
  * out[0] loops run on the host, out[1] loops run on the GPU,
- * out[2] GPU loop time in nanoseconds (sum of handler stream time when the
- * tuner is timing), out[3] GPU present (0/1), out[4] compute units,
+ * out[2] GPU loop time in nanoseconds (stream time of the executions the
+ * tuner timed while exploring candidates), out[3] GPU present (0/1), out[4] compute units,
  * out[5] graph replays, out[6] batched groups, out[7] batched loops. */
 void dvmh_amd_get_stats_C(DvmType out[8]);
 void dvmh_amd_reset_stats_C(void);
@@ -100,16 +100,54 @@ void *dvmh_amd_batch_scratch_C(DvmType batchRef, DvmType bytes);
  * (falls back to the sweep for a loop the model has not seen), 3 fixed best
  * from the records.  Records: one line per (loop signature, candidate, seconds). */
 void dvmh_amd_tuner_set_mode_C(DvmType mode);
+/* Timed executions per candidate before the sweep moves on (the site's time
+ * for a candidate is the minimum of its repeats); DVMH_AMD_TUNE_REPEATS. */
+void dvmh_amd_tuner_set_repeats_C(DvmType repeats);
 DvmType dvmh_amd_tuner_save_C(const char *path);
 DvmType dvmh_amd_tuner_load_C(const char *path);
 void dvmh_amd_tuner_clear_C(void);
-/* Candidates still to be measured at the sites seen so far (exhaustive sweep). */
+/* Candidates still to be timed at the sites seen so far in the current mode
+ * (every candidate for the exhaustive sweep; the default and the model's best
+ * four for the cost model; none for modes 0 and 3). */
 DvmType dvmh_amd_tuner_pending_C(void);
 /* The same for the sites of one source loop (file, line). */
 DvmType dvmh_amd_tuner_site_pending_C(const char *file, DvmType line);
 /* Best candidate so far for the loop site (file, line): out[0] candidate index,
  * out[1] time in ns, out[2..9] block x,y,z and tune[0..4]; returns 0 if unknown. */
 DvmType dvmh_amd_tuner_query_C(const char *file, DvmType line, DvmType out[10]);
+/* Keys of the known loop sites, one per line, into buf (truncated to bufLen);
+ * returns the length needed including the terminating zero. */
+DvmType dvmh_amd_tuner_sites_C(char *buf, DvmType bufLen);
+
+/* ---- learned cost model -------------------------------------------------------
+ * A perceptron with DVMH_AMD_MODEL_F inputs, two ReLU layers of
+ * DVMH_AMD_MODEL_H units and one output predicts log(seconds per iteration
+ * point) of a loop site under a candidate configuration.  Training and
+ * inference run on the GPU.  Parameters are one FP32 vector of
+ * dvmh_amd_model_param_count_C() values: W1[H][F] b1[H] W2[H][H] b2[H] W3[H] b3. */
+#define DVMH_AMD_MODEL_F 24
+#define DVMH_AMD_MODEL_H 64
+DvmType dvmh_amd_model_param_count_C(void);
+/* Trains a fresh network (Adam, `epochs` full-batch steps, cosine-decayed step
+ * size from lr, weights seeded by seed) on every recorded timing except those
+ * of sites whose key contains `holdout` (NULL or "": none).  Returns the
+ * number of samples; *finalLoss (may be NULL) gets the standardized MSE. */
+DvmType dvmh_amd_model_train_C(DvmType epochs, double lr, DvmType seed, const char *holdout, double *finalLoss);
+DvmType dvmh_amd_model_save_C(const char *path);
+DvmType dvmh_amd_model_load_C(const char *path);
+DvmType dvmh_amd_model_ready_C(void);
+/* For the first site whose key contains keyPart: predicted seconds (pred) and
+ * recorded seconds (measured, -1 if none) of every candidate.  Returns the
+ * candidate count, -1 if no site matches. */
+DvmType dvmh_amd_model_site_predict_C(const char *keyPart, double *pred, double *measured, DvmType maxCands);
+/* Raw (unstandardized) feature rows of that site's candidates. */
+DvmType dvmh_amd_model_site_features_C(const char *keyPart, float *X, DvmType maxCands);
+/* Direct access to the kernels (host pointers): gradient and loss of
+ * mean((net(X) - y)^2), network outputs, one Adam step (step counts from 1). */
+DvmType dvmh_amd_model_grad_C(const float *params, const float *X, const float *y, DvmType n, float *grad, float *loss);
+DvmType dvmh_amd_model_forward_C(const float *params, const float *X, DvmType n, float *out);
+DvmType dvmh_amd_model_adam_C(float *params, const float *grad, float *m, float *v, DvmType step, double lr, double beta1,
+        double beta2, double eps);
 
 #ifdef __cplusplus
 }
