@@ -29,6 +29,27 @@ void dvmh_hip_reduce_finish(int func, int rtType, long n, const void *vals, cons
 void dvmh_hip_reduce_segments(int func, int rtType, long groups, const long *offsets, const void *vals,
         const void *locs, long locSize, void *outVals, void *outLocs, hipStream_t s);
 
+// Finish of up to DVMH_MULTI_MAX reductions of one element type (rtType) in
+// one launch, deterministic like dvmh_hip_reduce_finish.  Partials are folded
+// in blocks of 2048 into per-reduction slots (`slots`:
+// dvmh_hip_multi_slot_bytes(n) bytes of device memory); the last block of a
+// reduction to finish, found with counters[r] (zero before the launch, zero
+// again after it), folds the slots and writes the value and the winner's loc
+// record to outVal / outLoc, which may point into mapped pinned host memory.
+#define DVMH_MULTI_MAX 8
+struct DvmhMultiRed {
+    int func;
+    long n;
+    const void *vals;
+    const void *locs;
+    long locSize;
+    void *outVal;
+    void *outLoc;
+    void *slots;
+};
+long dvmh_hip_multi_slot_bytes(long n);
+void dvmh_hip_reduce_finish_multi(int rtType, int count, const struct DvmhMultiRed *reds, unsigned *counters, hipStream_t s);
+
 // Axis permutation copy.  src is stored with storage-position sizes srcSizes[0..rank-1]
 // (position 0 outermost); dst storage position p holds src storage position
 // dstFromSrc[p].  (rts/dvmh/cuda_transform.cu:273-652 semantics)

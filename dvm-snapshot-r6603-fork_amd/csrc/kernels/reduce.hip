@@ -185,6 +185,77 @@ void launchFinish(int f, long n, const void *vals, const void *locs, long locSiz
             (const char *)locs, locSize, (T *)outVal, (char *)outLoc);
 }
 
+// ---- several reductions of one element type in one launch
+// Block b belongs to reduction r with first[r] <= b < first[r + 1] and folds
+// partials [c * MULTI_CHUNK, (c + 1) * MULTI_CHUNK) of it (c = b - first[r])
+// into slot c.  The block that arrives last at r's counter folds r's slots in
+// slot order (ties: lower partial index), writes the result and the winner's
+// loc record, and resets the counter for the next launch.
+constexpr long MULTI_CHUNK = 2 * FIN_THREADS;
+
+struct MultiArgs {
+    int count;
+    int func[DVMH_MULTI_MAX];
+    long n[DVMH_MULTI_MAX];
+    const void *vals[DVMH_MULTI_MAX];
+    const char *locs[DVMH_MULTI_MAX];
+    long locSize[DVMH_MULTI_MAX];
+    void *outVal[DVMH_MULTI_MAX];
+    char *outLoc[DVMH_MULTI_MAX];
+    void *slots[DVMH_MULTI_MAX];
+    long first[DVMH_MULTI_MAX + 1];
+    unsigned *counters;
+};
+
+template <typename T, bool BITS>
+__global__ __launch_bounds__(FIN_THREADS) void multiFinishKernel(MultiArgs a) {
+    __shared__ long win;
+    __shared__ int isLast;
+    const long b = blockIdx.x;
+    int r = 0;
+    while (r + 1 < a.count && b >= a.first[r + 1]) r++;
+    const long c = b - a.first[r];
+    const long nb = a.first[r + 1] - a.first[r];
+    T *slotVal = (T *)a.slots[r];
+    long *slotIdx = (long *)((char *)a.slots[r] + ((nb * (long)sizeof(T) + 15) & ~15L));
+    const long lo = c * MULTI_CHUNK;
+    const long hi = lo + MULTI_CHUNK < a.n[r] ? lo + MULTI_CHUNK : a.n[r];
+    foldRange<T, BITS>(a.func[r], lo, hi, (const T *)a.vals[r], nullptr, slotVal + c, slotIdx + c);
+    if (threadIdx.x == 0) {
+        __threadfence();   // the slot is visible before the arrival is counted
+        const unsigned before = atomicAdd(&a.counters[r], 1u);
+        isLast = before == (unsigned)(nb - 1);
+    }
+    __syncthreads();
+    if (!isLast) return;
+    __threadfence();
+    foldRange<T, BITS>(a.func[r], 0, nb, slotVal, slotIdx, (T *)a.outVal[r], &win);
+    copyLoc<T>(a.locs[r], a.locSize[r], &win, a.outLoc[r]);
+    if (threadIdx.x == 0) a.counters[r] = 0u;
+}
+
+template <typename T, bool BITS>
+void launchMulti(int count, const DvmhMultiRed *reds, unsigned *counters, hipStream_t s) {
+    MultiArgs a;
+    a.count = count;
+    a.counters = counters;
+    long blocks = 0;
+    for (int r = 0; r < count; r++) {
+        a.func[r] = reds[r].func;
+        a.n[r] = reds[r].n;
+        a.vals[r] = reds[r].vals;
+        a.locs[r] = (const char *)reds[r].locs;
+        a.locSize[r] = reds[r].locSize;
+        a.outVal[r] = reds[r].outVal;
+        a.outLoc[r] = (char *)reds[r].outLoc;
+        a.slots[r] = reds[r].slots;
+        a.first[r] = blocks;
+        blocks += reds[r].n > 0 ? (reds[r].n + MULTI_CHUNK - 1) / MULTI_CHUNK : 1;
+    }
+    a.first[count] = blocks;
+    hipLaunchKernelGGL((multiFinishKernel<T, BITS>), dim3((unsigned)blocks), dim3(FIN_THREADS), 0, s, a);
+}
+
 template <typename T, bool BITS>
 void launchSegments(int f, long groups, const long *offsets, const void *vals, const void *locs, long locSize,
         void *outVals, void *outLocs, hipStream_t s) {
@@ -218,6 +289,16 @@ extern "C" void dvmh_hip_reduce_finish(int func, int rtType, long n, const void 
 extern "C" void dvmh_hip_reduce_segments(int func, int rtType, long groups, const long *offsets, const void *vals,
         const void *locs, long locSize, void *outVals, void *outLocs, hipStream_t s) {
     DISPATCH_TYPES(launchSegments, func, groups, offsets, vals, locs, locSize, outVals, outLocs, s)
+}
+
+extern "C" long dvmh_hip_multi_slot_bytes(long n) {
+    const long nb = n > 0 ? (n + MULTI_CHUNK - 1) / MULTI_CHUNK : 1;
+    return ((nb * 8 + 15) & ~15L) + nb * 8 + 64;
+}
+
+extern "C" void dvmh_hip_reduce_finish_multi(int rtType, int count, const DvmhMultiRed *reds, unsigned *counters, hipStream_t s) {
+    if (count <= 0 || count > DVMH_MULTI_MAX) return;
+    DISPATCH_TYPES(launchMulti, count, reds, counters, s)
 }
 
 extern "C" void dvmh_hip_fill(void *dst, const void *value, long valueBytes, long count, hipStream_t s) {

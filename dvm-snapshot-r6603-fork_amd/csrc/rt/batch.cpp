@@ -95,6 +95,7 @@ static void runBatchGroup(std::vector<Loop *> &group) {
     }
     DvmType ref = (DvmType)&ctx;
     callHandler(group[0]->batchH, &ref);
+    for (SpecLoop *sl : ctx.specs) gpuRedFinishFlush(sl);   // finishes requested per loop, if any
     rt.loopsOnGpu += (long)group.size();
     batchGroups++;
     batchLoops += (long)group.size();

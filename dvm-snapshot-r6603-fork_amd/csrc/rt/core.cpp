@@ -93,6 +93,11 @@ void Gpu::init() {
     snprintf(props.arch, sizeof(props.arch), "%.63s", p.gcnArchName);
     // Default-priority, non-blocking stream (never a prioritised one).
     HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    // arrival counters of the multi-reduction finish: zeroed once here, every
+    // launch leaves them zero again
+    HIP_CHECK(hipMalloc((void **)&finishCounters, 64 * sizeof(unsigned)));
+    HIP_CHECK(hipMemset(finishCounters, 0, 64 * sizeof(unsigned)));
+    HIP_CHECK(hipDeviceSynchronize());
     present = true;
     logMsg(LOG_INFO, "GPU %s (%s): %d CUs, wave%d", props.name, props.arch, props.cuCount, props.waveSize);
 }

@@ -113,25 +113,40 @@ struct PendingPost {
 static std::vector<PendingPost> pending;
 
 struct PinnedPool {
-    std::vector<char *> blocks;
+    std::vector<char *> blocks, devBlocks;   // host addresses and their device-side aliases
     size_t used = 0, blockSize = 1 << 20;
     char *take(size_t bytes) {
         bytes = (bytes + 63) & ~(size_t)63;
         size_t b = used / blockSize, off = used % blockSize;
         if (off + bytes > blockSize) { b++; off = 0; used = b * blockSize; }
         while (b >= blocks.size()) {
-            void *p = nullptr;
+            void *p = nullptr, *d = nullptr;
             HIP_CHECK(hipHostMalloc(&p, blockSize, hipHostMallocDefault));
+            HIP_CHECK(hipHostGetDevicePointer(&d, p, 0));
             blocks.push_back((char *)p);
+            devBlocks.push_back((char *)d);
         }
         used += bytes;
         return blocks[b] + off;
+    }
+    char *device(char *h) const {
+        for (size_t b = 0; b < blocks.size(); b++)
+            if (h >= blocks[b] && h < blocks[b] + blockSize) return devBlocks[b] + (h - blocks[b]);
+        return nullptr;
     }
     void reset() { used = 0; }
 };
 static PinnedPool pinnedPool;
 
 char *pinnedTake(size_t bytes) { return pinnedPool.take(bytes); }
+
+// Device-side address of mapped pinned host memory (pool blocks or graph slots).
+static char *mappedDevicePtr(char *h) {
+    if (char *d = pinnedPool.device(h)) return d;
+    void *d = nullptr;
+    HIP_CHECK(hipHostGetDevicePointer(&d, h, 0));
+    return (char *)d;
+}
 
 void flushDeferredPosts() {
     if (pending.empty()) return;
@@ -168,28 +183,65 @@ void gpuRedPrepare(SpecLoop *sl, Reduction *r, long count, bool fill) {
 }
 
 // DvmhReductionCuda::finish (rts/dvmh/loop.cpp:301-408): fold the partials on
-// the device, bring back one record, merge it into the variable.
+// the device and merge the result into the variable.  The finish requests of a
+// handler are collected and run together when the handler returns: one launch
+// folds every reduction of an element type and writes the results straight
+// into mapped pinned host memory (no separate copy).
 void gpuRedFinish(SpecLoop *sl, const std::shared_ptr<Reduction> &r) {
     DVMH_CHECK(r->gpuArr != nullptr, "dvmh_loop_cuda_red_finish_C without dvmh_loop_cuda_red_prepare_C");
     DVMH_CHECK(r->elemCount == 1, "Reduction arrays are not supported on the GPU");
-    void *dRes = gpu.alloc((size_t)(r->elemSize + (r->isLoc() ? r->locSize : 0)));
-    void *dLoc = r->isLoc() ? (char *)dRes + r->elemSize : nullptr;
-    long scratchBytes = dvmh_hip_reduce_scratch_bytes(r->items);
-    void *scratch = scratchBytes ? gpu.alloc((size_t)scratchBytes) : nullptr;
-    dvmh_hip_reduce_finish(r->func, r->rtType, r->items, r->gpuArr, r->gpuLoc, r->locSize, dRes, dLoc, scratch, sl->stream);
-    if (scratch) gpu.release(scratch);
-    size_t recBytes = (size_t)(r->elemSize + (r->isLoc() ? r->locSize : 0));
-    // a captured finish writes into a slot owned by the graph
-    char *host = graphCapturing() ? graphPinnedSlot(recBytes) : pinnedPool.take(recBytes);
-    if (graphCapturing()) graphNotePost(r, host, r->isLoc() ? host + r->elemSize : nullptr);
-    HIP_CHECK(hipMemcpyAsync(host, dRes, recBytes, hipMemcpyDeviceToHost, sl->stream));
-    gpu.release(dRes);
-    gpu.release(r->gpuArr);
-    r->gpuArr = nullptr;
-    if (r->gpuLoc) { gpu.release(r->gpuLoc); r->gpuLoc = nullptr; }
+    for (auto &q : sl->finishQueue)
+        DVMH_CHECK(q != r, "dvmh_loop_cuda_red_finish_C called twice for one reduction");
+    // the pointers registered by dvmh_loop_cuda_register_red_C are the handler's
+    // locals: clear them now, while the handler is still running
     if (r->gpuArrPtr) *r->gpuArrPtr = nullptr;
     if (r->gpuLocPtr) *r->gpuLocPtr = nullptr;
-    addPendingPost(r, host, r->isLoc() ? host + r->elemSize : nullptr);
+    r->gpuArrPtr = nullptr;
+    r->gpuLocPtr = nullptr;
+    sl->finishQueue.push_back(r);
+}
+
+void gpuRedFinishFlush(SpecLoop *sl) {
+    auto &q = sl->finishQueue;
+    while (!q.empty()) {
+        // the first pending reduction's element type, up to DVMH_MULTI_MAX of that type
+        const int type = q[0]->rtType;
+        std::vector<std::shared_ptr<Reduction>> group, rest;
+        for (auto &r : q) {
+            if (r->rtType == type && (int)group.size() < DVMH_MULTI_MAX) group.push_back(r);
+            else rest.push_back(r);
+        }
+        DvmhMultiRed d[DVMH_MULTI_MAX];
+        std::vector<void *> slots;
+        for (size_t i = 0; i < group.size(); i++) {
+            auto &r = group[i];
+            size_t recBytes = (size_t)(r->elemSize + (r->isLoc() ? r->locSize : 0));
+            // a captured finish writes into a slot owned by the graph
+            char *host = graphCapturing() ? graphPinnedSlot(recBytes) : pinnedPool.take(recBytes);
+            char *hostLoc = r->isLoc() ? host + r->elemSize : nullptr;
+            if (graphCapturing()) graphNotePost(r, host, hostLoc);
+            char *dev = mappedDevicePtr(host);
+            void *sv = gpu.alloc((size_t)dvmh_hip_multi_slot_bytes(r->items));
+            slots.push_back(sv);
+            d[i].func = r->func;
+            d[i].n = r->items;
+            d[i].vals = r->gpuArr;
+            d[i].locs = r->gpuLoc;
+            d[i].locSize = r->isLoc() ? r->locSize : 0;
+            d[i].outVal = dev;
+            d[i].outLoc = r->isLoc() ? dev + r->elemSize : nullptr;
+            d[i].slots = sv;
+            addPendingPost(r, host, hostLoc);
+        }
+        dvmh_hip_reduce_finish_multi(type, (int)group.size(), d, gpu.finishCounters, sl->stream);
+        for (void *p : slots) gpu.release(p);
+        for (auto &r : group) {
+            gpu.release(r->gpuArr);
+            r->gpuArr = nullptr;
+            if (r->gpuLoc) { gpu.release(r->gpuLoc); r->gpuLoc = nullptr; }
+        }
+        q.swap(rest);
+    }
 }
 
 // ------------------------------------------------------------ layout policy
@@ -263,6 +315,7 @@ void performLoopNow(Loop *loop) {
     DvmType ref = (DvmType)&sl;
     if (onGpu) {
         callHandler(loop->gpuH, &ref);
+        gpuRedFinishFlush(&sl);
         rt.loopsOnGpu++;
         // executions the tuner explores are timed from its pick to here; settled
         // sites run without events or synchronisation

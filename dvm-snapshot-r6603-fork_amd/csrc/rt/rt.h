@@ -73,6 +73,7 @@ public:
     int ordinal = 0;
     DeviceProps props;
     hipStream_t stream = nullptr;
+    unsigned *finishCounters = nullptr;   // arrival counters of the multi-reduction finish (kept zero between launches)
     void init();
     void *alloc(size_t bytes);          // size-keyed cache in front of hipMalloc
     void release(void *p);              // back to the cache
@@ -214,6 +215,7 @@ struct SpecLoop : Object {
     std::string siteKey;            // tuner site of this execution
     bool explore = false;           // the tuner times this execution (candidate still lacks repeats)
     hipEvent_t tuneStart = nullptr; // recorded at the pick when exploring
+    std::vector<std::shared_ptr<Reduction>> finishQueue;   // red_finish requests, run when the handler returns
     std::vector<void *> toFree;
     SpecLoop(Loop *l, int dev): Object(Kind::SpecLoop), loop(l), device(dev) {}
 };
@@ -279,6 +281,7 @@ void performLoop(Loop *loop);
 // reductions on the GPU (kernels in csrc/kernels/reduce.hip)
 void gpuRedPrepare(SpecLoop *sl, Reduction *r, long count, bool fill);
 void gpuRedFinish(SpecLoop *sl, const std::shared_ptr<Reduction> &r);
+void gpuRedFinishFlush(SpecLoop *sl);
 void flushDeferredPosts();
 
 // tuner
