@@ -9,6 +9,7 @@
 #include <cstdint>
 
 #include "launch.h"
+#include "dvmh_hip_device.h"
 
 namespace {
 
@@ -67,42 +68,6 @@ __global__ __launch_bounds__(256) void transposeTiles(const T *__restrict__ src,
     }
 }
 
-__device__ inline long convertXY(long x, long y, long Rx, long Ry, int slash) {
-    long idx;
-    if (!slash) {
-        long d = x + y;
-        if (Rx == Ry) {
-            if (d < Rx) idx = y + (1 + d) * d / 2;
-            else idx = Rx * (Rx - 1) + x - (2 * Rx - d - 1) * (2 * Rx - d - 2) / 2;
-        } else if (Rx < Ry) {
-            if (d < Rx) idx = y + (1 + d) * d / 2;
-            else if (d < Ry) idx = (1 + Rx) * Rx / 2 + Rx - x - 1 + Rx * (d - Rx);
-            else idx = Rx * Ry - Ry + y - (Rx + Ry - d - 1) * (Rx + Ry - d - 2) / 2;
-        } else {
-            if (d < Ry) idx = x + (1 + d) * d / 2;
-            else if (d < Rx) idx = (1 + Ry) * Ry / 2 + (Ry - y - 1) + Ry * (d - Ry);
-            else idx = Rx * Ry - Rx + x - (Rx + Ry - d - 1) * (Rx + Ry - d - 2) / 2;
-        }
-    } else {
-        if (Rx == Ry) {
-            long e = x + Rx - 1 - y;
-            if (e < Rx) idx = Rx - 1 - y + (e + 1) * e / 2;
-            else idx = Rx * (Rx - 1) + x - (Rx - x + y) * (Rx - x + y - 1) / 2;
-        } else if (Rx < Ry) {
-            long e = x + Ry - 1 - y;
-            if (e < Rx) idx = Ry - 1 - y + (e + 1) * e / 2;
-            else if (e < Ry) idx = (1 + Rx) * Rx / 2 + Rx - x - 1 + Rx * (e - Rx);
-            else idx = Rx * Ry - 1 - y - (Rx + y - x) * (Rx + y - x - 1) / 2;
-        } else {
-            long e = x + Ry - 1 - y;
-            if (e < Ry) idx = x + (1 + e) * e / 2;
-            else if (e < Rx) idx = (1 + Ry) * Ry / 2 + y + Ry * (x - y - 1);
-            else idx = Rx * Ry - Rx + x - (Rx + y - x) * (Rx + y - x - 1) / 2;
-        }
-    }
-    return idx;
-}
-
 template <typename T>
 __global__ void diagonalKernel(const T *__restrict__ src, T *__restrict__ dst, long Rx, long Ry, long Rz, int back,
         int slash) {
@@ -111,7 +76,7 @@ __global__ void diagonalKernel(const T *__restrict__ src, T *__restrict__ dst, l
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
         long z = i / plane, r = i % plane;
         long y = r / Rx, x = r % Rx;
-        long d = z * plane + convertXY(x, y, Rx, Ry, slash);
+        long d = z * plane + dvmh_dev::convertXY<long>(x, y, Rx, Ry, slash);
         if (!back) dst[d] = src[i];
         else dst[i] = src[d];
     }
@@ -125,6 +90,11 @@ long gridFor(long total) {
 }
 
 }  // namespace
+
+// The diagonal index map the kernels use, callable from the host (tests, tools).
+extern "C" long dvmh_amd_convert_xy_C(long x, long y, long Rx, long Ry, long slash) {
+    return dvmh_dev::convertXY<long>(x, y, Rx, Ry, (int)slash);
+}
 
 extern "C" void dvmh_hip_permute(const void *src, void *dst, long typeSize, int rank, const long *srcSizes,
         const int *dstFromSrc, hipStream_t s) {

@@ -63,6 +63,11 @@ void dvmh_amd_get_stats_C(DvmType out[8]);
 void dvmh_amd_reset_stats_C(void);
 /* Device name / architecture string of the GPU in use ("" without one). */
 const char *dvmh_amd_device_name_C(void);
+/* Position of (x, y) in an Rx x Ry plane stored diagonal by diagonal (slash 0:
+ * diagonals of constant x + y; 1: of constant x - y), the index map of the
+ * diagonal device layout (rts/dvmh/include/dvmhlib_device.h:56-98).  The same
+ * function the layout kernels and the handlers' DiagView use. */
+long dvmh_amd_convert_xy_C(long x, long y, long Rx, long Ry, long slash);
 
 /* ---- batching of many small loops -------------------------------------------
  * Between begin and end, dvmh_loop_perform_C of loops that registered a batch

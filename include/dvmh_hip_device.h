@@ -57,9 +57,11 @@ __device__ __forceinline__ View<T, I, R, UNIT> makeView(T *base, const I *stride
     return v;
 }
 
-// Diagonal position inside an Rx x Ry plane (x fastest).
+// Diagonal position inside an Rx x Ry plane (x fastest).  Also compiled for
+// the host, so the layout-transform kernels, the handlers and host-side checks
+// share one definition.
 template <typename I>
-__device__ __forceinline__ I convertXY(I x, I y, I Rx, I Ry, int slash) {
+__host__ __device__ __forceinline__ I convertXY(I x, I y, I Rx, I Ry, int slash) {
     I idx;
     if (!slash) {
         I d = x + y;
