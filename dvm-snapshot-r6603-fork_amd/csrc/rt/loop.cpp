@@ -190,6 +190,10 @@ void gpuRedPrepare(SpecLoop *sl, Reduction *r, long count, bool fill) {
 void gpuRedFinish(SpecLoop *sl, const std::shared_ptr<Reduction> &r) {
     DVMH_CHECK(r->gpuArr != nullptr, "dvmh_loop_cuda_red_finish_C without dvmh_loop_cuda_red_prepare_C");
     DVMH_CHECK(r->elemCount == 1, "Reduction arrays are not supported on the GPU");
+    DVMH_CHECK(r->rtType == rt_CHAR || r->rtType == rt_UCHAR || r->rtType == rt_INT || r->rtType == rt_UINT ||
+                    r->rtType == rt_LOGICAL || r->rtType == rt_LONG || r->rtType == rt_ULONG || r->rtType == rt_LLONG ||
+                    r->rtType == rt_ULLONG || r->rtType == rt_FLOAT || r->rtType == rt_DOUBLE,
+            "Reduction element type %d is not supported on the GPU", r->rtType);
     for (auto &q : sl->finishQueue)
         DVMH_CHECK(q != r, "dvmh_loop_cuda_red_finish_C called twice for one reduction");
     // the pointers registered by dvmh_loop_cuda_register_red_C are the handler's
