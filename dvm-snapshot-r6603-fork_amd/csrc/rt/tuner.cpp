@@ -45,7 +45,7 @@ std::vector<Candidate> makeCandidates(int kind, int loopRank, int depCount, cons
     std::vector<Candidate> v;
     int mapped = loopRank - (kind == DVMH_AMD_KIND_PARALLEL ? 0 : depCount);   // axes spread over threads
     const int tfMax = settings.tuneTransforms ? 1 : 0;   // also try device layout transforms
-    if (kind == DVMH_AMD_KIND_PARALLEL) {
+    if (kind == DVMH_AMD_KIND_PARALLEL || kind == DVMH_AMD_KIND_STENCIL) {
         if (loopRank >= 3) addCand(v, 32, 4, 2, {});
         else if (loopRank == 2) addCand(v, 32, 8, 1, {});
         else addCand(v, 256, 1, 1, {});
@@ -66,6 +66,30 @@ std::vector<Candidate> makeCandidates(int kind, int loopRank, int depCount, cons
                             addCand(v, bx, by, bz, {{DVMH_AMD_T_DEPTH, dp}, {DVMH_AMD_T_XCD, xcd}});
                     }
                 }
+        if (kind == DVMH_AMD_KIND_STENCIL && (loopRank == 2 || loopRank == 3)) {
+            // LDS march variant: block (bx, by), TILE_A points per thread on the
+            // innermost axis, CHUNK planes per workgroup; three planes of
+            // (by + 2) x (bx * TILE_A + 2) FP64 elements must fit 64 KB of LDS
+            const int mxs[] = {64, 128, 256, 512, 1024};
+            const int mys[] = {1, 2, 4, 8};
+            const int mcs[] = {1, 2, 3, 4};
+            const int marches[] = {8, 16, 32};
+            for (int bx : mxs)
+                for (int by : mys) {
+                    if (loopRank == 2 && by > 1) continue;
+                    if (loopRank == 3 && bx > 256) continue;
+                    int n = bx * by;
+                    if (n < 64 || n > 1024) continue;
+                    for (int c : mcs) {
+                        long rows = loopRank == 3 ? by + 2 : 1;
+                        if (3 * rows * ((long)bx * c + 2) * 8 > 64512) continue;
+                        for (int m : marches)
+                            for (int xcd = 0; xcd <= 1; xcd++)
+                                addCand(v, bx, by, 1, {{DVMH_AMD_T_VARIANT, 1}, {DVMH_AMD_T_TILE_A, c}, {DVMH_AMD_T_CHUNK, m},
+                                                       {DVMH_AMD_T_XCD, xcd}});
+                    }
+                }
+        }
     } else if (kind == DVMH_AMD_KIND_LINE) {
         // threads over the independent axes, each walks its line
         if (mapped >= 2) addCand(v, 32, 8, 1, {});
@@ -147,6 +171,10 @@ int SiteInfo::bestMeasured() const {
 
 extern int modelRank(SiteInfo &s, std::vector<int> &order);   // cost model (model.cpp)
 
+// Configurations forced per loop kind (tests, experiments): used instead of the
+// mode's choice at every site of that kind; never timed or recorded.
+static std::map<long, Candidate> forced;
+
 // Candidates the cost-model mode times: the default and the model's best K.
 std::vector<int> SiteInfo::trialSet() const {
     const int K = 4;
@@ -180,6 +208,13 @@ void tunerPick(SpecLoop *sl, long kind, TuneChoice &out) {
         Candidate d = c[0];
         for (int i = 0; i < 3; i++) out.block[i] = l->userBlock[i];
         for (int i = 0; i < DVMH_AMD_TUNE_N; i++) out.tune[i] = d.tune[i];
+        out.candidate = -1;
+        return;
+    }
+    auto fit = forced.find(kind);
+    if (fit != forced.end()) {
+        for (int i = 0; i < 3; i++) out.block[i] = fit->second.block[i];
+        for (int i = 0; i < DVMH_AMD_TUNE_N; i++) out.tune[i] = fit->second.tune[i];
         out.candidate = -1;
         return;
     }
@@ -347,6 +382,18 @@ extern "C" void dvmh_amd_tuner_set_repeats_C(DvmType repeats) {
     runtimeInit();
     DVMH_CHECK(repeats >= 1, "Tuner repeats must be >= 1");
     settings.tuneRepeats = (int)repeats;
+}
+
+extern "C" void dvmh_amd_tuner_force_C(DvmType kind, const DvmType block[3], const DvmType tune[DVMH_AMD_TUNE_N]) {
+    runtimeInit();
+    if (!block) {
+        forced.erase(kind);
+        return;
+    }
+    Candidate c;
+    for (int i = 0; i < 3; i++) c.block[i] = (int)block[i];
+    for (int i = 0; i < DVMH_AMD_TUNE_N; i++) c.tune[i] = tune ? tune[i] : 0;
+    forced[kind] = c;
 }
 
 extern "C" DvmType dvmh_amd_tuner_save_C(const char *path) { runtimeInit(); return tunerSave(path); }

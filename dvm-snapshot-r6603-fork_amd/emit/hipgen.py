@@ -21,18 +21,22 @@ Output, per program:
                      for wave64/LDS:
                        PARALLEL   one launch; x <- innermost axis, y, z next;
                                   DEPTH consecutive iterations of the slowest
-                                  mapped axis per thread; XCD-aware tile order;
-                                  in-kernel wave+LDS reduction, one partial
-                                  per workgroup.
+                                  mapped axis per thread (an unguarded copy for
+                                  threads whose iterations are all inside the
+                                  loop); XCD-aware tile order; in-kernel
+                                  wave+LDS reduction, one partial per workgroup.
+                       MARCH      variant of PARALLEL for loops that read some
+                                  arrays as a radius-1 stencil: a workgroup
+                                  marches a tile along the outermost axis with a
+                                  rolling three-plane LDS window per stencil
+                                  array, the next plane's loads in flight while
+                                  the current plane is computed.
                        LINE       ACROSS with one dependent axis: one launch,
                                   threads over the independent axes, each
-                                  thread walks its line in dependency order;
-                                  optionally through an LDS window of CHUNK
-                                  steps that is loaded and stored coalesced.
-                       WAVEFRONT  ACROSS with >= 2 dependent axes: hyperplane
-                                  wavefront (one launch per hyperplane of
-                                  tiles; inside a tile, local hyperplanes
-                                  separated by workgroup barriers).
+                                  thread walks its line in dependency order.
+                       WAVEFRONT  ACROSS with >= 2 dependent axes: one launch
+                                  per hyperplane of the dependent axes; arrays
+                                  may be in the diagonal layout.
 The loop body is inserted verbatim; every schedule respects the dependency
 order, so results are bit-identical to the sequential host order.
 """
@@ -72,6 +76,33 @@ class Loop:
         self.batch = d.get('batch', False)         # also emit a batch handler
         self.arrays = [p for p in self.params if p[0] == 'array']
         self.scalars = [p for p in self.params if p[0] == 'scalar']
+        # arrays read as a radius-1 stencil around the loop point: candidates for
+        # the LDS march variant (checked against the body below)
+        self.stencil = [a for a in d.get('stencil', []) if self._is_stencil(a)]
+
+    def _is_stencil(self, name):
+        """True when the body only reads `name` and every access is
+        name[v0 +- 0/1][v1 +- 0/1]... with the loop variables in order."""
+        if self.across or self.rank not in (2, 3):
+            return False
+        arr = [p for p in self.arrays if p[1] == name]
+        if not arr or arr[0][3] != self.rank:
+            return False
+        if re.search(r'\b' + name + r'\s*(\[[^\[\]]*\]\s*)+(=(?!=)|\+=|-=|\*=|/=|\+\+|--)', self.body):
+            return False
+        if re.search(r'(\+\+|--)\s*' + name + r'\b', self.body):
+            return False
+        for m in re.finditer(r'\b' + name + r'\s*((?:\[[^\[\]]*\]\s*)+)', self.body):
+            idx = re.findall(r'\[([^\[\]]*)\]', m.group(1))
+            if len(idx) != self.rank:
+                return False
+            for (v, _), e in zip(self.vars, idx):
+                if not re.fullmatch(r'\s*' + v + r'\s*(?:[+-]\s*[01]\s*)?', e):
+                    return False
+        # the array must not be reachable any other way (name used only as A[...])
+        if re.search(r'\b' + name + r'\b(?!\s*\[)', self.body):
+            return False
+        return True
 
     # static dependency masks the ACROSS clause can produce (identity correspondence)
     def dep_axes(self):
@@ -426,8 +457,24 @@ def parallel_kernel(L):
             b.append(f'const IndexT t{a} = t{a}_0;')
             conds.append(f't{a} < cnt{a}')
     if r >= 2:
-        b.append('#pragma unroll')
-        b.append('for (int dd = 0; dd < DEPTH; dd++) {')
+        # Threads whose DEPTH iterations are all inside the loop take an
+        # unguarded copy: one basic block, so the loads of every iteration can
+        # be issued before the first result is stored (iterations of a
+        # PARALLEL loop are independent and array pointers are restrict).
+        # Threads on the edges take the guarded copy.
+        whole = ' && '.join([f't{dep_ax}_b + DEPTH <= cnt{dep_ax}'] + conds)
+        b.append(f'if ({whole}) {{')
+        fast = [f'const IndexT t{dep_ax} = t{dep_ax}_b + dd;']
+        for i in range(r):
+            fast.append(var_assign(L, i, f't{i}', f'lo{i}', f'st{i}'))
+        fast.append('{\n' + indent(L.body + '\n', 1) + '}')
+        b.append(IND + '#pragma unroll')
+        b.append(IND + 'for (int dd = 0; dd < DEPTH; dd++) {')
+        b += [IND * 2 + ln for ln in '\n'.join(fast).split('\n')]
+        b.append(IND + '}')
+        b.append('} else {')
+        b.append(IND + '#pragma unroll')
+        b.append(IND + 'for (int dd = 0; dd < DEPTH; dd++) {')
         inner = [f'const IndexT t{dep_ax} = t{dep_ax}_b + dd;']
         conds2 = conds + [f't{dep_ax} < cnt{dep_ax}']
     else:
@@ -439,7 +486,8 @@ def parallel_kernel(L):
     inner.append(indent('{\n' + indent(L.body + '\n', 1) + '}', 1))
     inner.append('}')
     if r >= 2:
-        b += [IND + ln for ln in '\n'.join(inner).split('\n')]
+        b += [IND * 2 + ln for ln in '\n'.join(inner).split('\n')]
+        b.append(IND + '}')
         b.append('}')
     else:
         b += inner
@@ -449,10 +497,154 @@ def parallel_kernel(L):
     return '\n'.join(k) + '\n'
 
 
+def march_kernel(L):
+    """LDS march variant of a PARALLEL loop with stencil arrays (rank 2 or 3,
+    unit steps).  A workgroup owns a tile of the inner axes -- blockDim.x * COLS
+    points of the innermost axis (x), blockDim.y of the middle one for rank 3 --
+    and marches `march` planes along the outermost axis.  Each stencil array is
+    staged in LDS as a rolling window of three planes (tile plus a halo of one),
+    every element loaded from HBM once per tile; the body reads it through a
+    MarchView with the same A[i][j][k] syntax, other arrays stay in HBM."""
+    r = L.rank
+    ax_x, ax_y = r - 1, (r - 2 if r == 3 else None)
+    fp, setup = [], []
+    for p in L.arrays:
+        _, nm, ct, ar = p
+        fp.append(f'{ct} *__restrict__ {nm}_base')
+        fp += [f'IndexT {nm}_s{k}' for k in range(ar)]
+        strides = ', '.join(f'{nm}_s{k}' for k in range(ar))
+        setup.append(f'const IndexT {nm}_str[{ar}] = {{{strides}}};')
+        vn = f'{nm}_g' if nm in L.stencil else nm
+        setup.append(f'const auto {vn} = dvmh_dev::makeView<{ct}, IndexT, {ar}, true>({nm}_base, {nm}_str);')
+    for p in L.scalars:
+        fp.append(f'{p[2]} *__restrict__ {p[1]}_ptr')
+        setup.append(f'{p[2]} &{p[1]} = *{p[1]}_ptr;')
+    for i in range(r):
+        fp += [f'IndexT lo{i}', f'IndexT cnt{i}', f'IndexT st{i}']
+    fp += ['IndexT nbx', 'IndexT nby', 'IndexT nbz', 'long nblocks', 'int xcd', 'IndexT march']
+    fp += red_kernel_params(L)
+    k = ['template <typename IndexT, int COLS>',
+         f'__global__ __launch_bounds__(1024) void {L.name}_march_kernel({", ".join(fp)}) {{']
+    b = ['extern __shared__ __align__(16) unsigned char dvmh_lds[];']
+    b += setup
+    for v, ct in L.vars:
+        b.append(f'{ct} {v};')
+    for pr in L.privates:
+        b.append(pr + ';')
+    b += red_kernel_locals(L)
+    b.append('long tile = dvmh_dev::xcdTile((long)blockIdx.x, nblocks, xcd);')
+    b.append('const IndexT bx = (IndexT)(tile % nbx); tile /= nbx;')
+    b.append('const IndexT by = (IndexT)(tile % nby); tile /= nby;')
+    b.append('const IndexT bz = (IndexT)tile;')
+    b.append('const IndexT tileX = (IndexT)blockDim.x * COLS;')
+    b.append(f'const IndexT tx0 = bx * tileX;')
+    if ax_y is not None:
+        b.append('const IndexT tileY = (IndexT)blockDim.y;')
+        b.append('const IndexT ty0 = by * tileY;')
+        b.append('const IndexT rows = tileY + 2;')
+    else:
+        b.append('const IndexT rows = 1;')
+    b.append('const IndexT cols = tileX + 2;')
+    b.append('const IndexT plane = rows * cols;')
+    # LDS: three planes per stencil array, 16-byte aligned
+    off = '0'
+    for nm in L.stencil:
+        ct = [p for p in L.arrays if p[1] == nm][0][2]
+        b.append(f'{ct} *{nm}_lds = ({ct} *)(dvmh_lds + {off});')
+        off = f'((({off}) + 3 * plane * (IndexT)sizeof({ct}) + 15) & ~(IndexT)15)'
+    b.append(f'const IndexT kx0 = lo{ax_x} + tx0 - 1;   // array index held by LDS column 0')
+    if ax_y is not None:
+        b.append(f'const IndexT jy0 = lo{ax_y} + ty0 - 1;   // array index held by LDS row 0')
+    b.append('const IndexT tz0 = bz * march;')
+    b.append('const IndexT nMarch = cnt0 - tz0 < march ? cnt0 - tz0 : march;')
+    b.append('const IndexT i0 = lo0 + tz0;')
+    b.append('const int tid = (int)(threadIdx.x + blockDim.x * threadIdx.y);')
+    b.append('const int nthr = (int)(blockDim.x * blockDim.y);')
+    # Plane loads are software-pipelined through registers: each thread owns
+    # plane elements e = tid + m * nthr (m < NPF, NPF = 3 * COLS + 2 covers
+    # every shape the handler accepts); issue() loads a plane from HBM into
+    # registers, commit() stores them into an LDS slot.  The next plane's loads
+    # are issued before the current plane is computed, so their latency is
+    # hidden behind the compute.  Only indices some point of the loop reads
+    # (the loop box plus the halo) are loaded.
+    b.append('constexpr int NPF = 3 * COLS + 2;')
+    b.append('const IndexT rBase = (IndexT)tid / cols, cBase = (IndexT)tid - rBase * cols;')
+    b.append('const IndexT dR = (IndexT)nthr / cols, dC = (IndexT)nthr - dR * cols;')
+    for nm in L.stencil:
+        ct = [p for p in L.arrays if p[1] == nm][0][2]
+        b.append(f'{ct} {nm}_pf[NPF];')
+    lines = ['auto issue = [&](IndexT ip) {',
+             IND + 'const bool okI = ip >= lo0 - 1 && ip <= lo0 + cnt0;',
+             IND + 'IndexT rr = rBase, cc = cBase;',
+             IND + '#pragma unroll',
+             IND + 'for (int m = 0; m < NPF; m++) {',
+             IND * 2 + 'const IndexT kk = kx0 + cc;']
+    conds = ['okI', '(IndexT)tid + (IndexT)m * nthr < plane', f'kk >= lo{ax_x} - 1', f'kk <= lo{ax_x} + cnt{ax_x}']
+    if ax_y is not None:
+        lines.append(IND * 2 + 'const IndexT jj = jy0 + rr;')
+        conds += [f'jj >= lo{ax_y} - 1', f'jj <= lo{ax_y} + cnt{ax_y}']
+    lines.append(IND * 2 + f'const bool ok = {" && ".join(conds)};')
+    for nm in L.stencil:
+        ct = [p for p in L.arrays if p[1] == nm][0][2]
+        acc = f'{nm}_g[ip][jj][kk]' if ax_y is not None else f'{nm}_g[ip][kk]'
+        lines.append(IND * 2 + f'{nm}_pf[m] = ok ? {acc} : ({ct})0;')
+    lines += [IND * 2 + 'cc += dC;',
+              IND * 2 + 'rr += dR;',
+              IND * 2 + 'if (cc >= cols) { cc -= cols; rr++; }',
+              IND + '}',
+              '};',
+              'auto commit = [&](int slot) {',
+              IND + '#pragma unroll',
+              IND + 'for (int m = 0; m < NPF; m++) {',
+              IND * 2 + 'const IndexT e = (IndexT)tid + (IndexT)m * nthr;',
+              IND * 2 + 'if (e < plane) {']
+    for nm in L.stencil:
+        lines.append(IND * 3 + f'{nm}_lds[(IndexT)slot * plane + e] = {nm}_pf[m];')
+    lines += [IND * 2 + '}', IND + '}', '};']
+    b += lines
+    b.append('issue(i0 - 1);')
+    b.append('commit(0);')
+    b.append('issue(i0);')
+    b.append('commit(1);')
+    b.append('issue(i0 + 1);')
+    b.append('for (IndexT d = 0; d < nMarch; d++) {')
+    inner = ['commit((int)((d + 2) % 3));   // plane i0 + d + 1',
+             '__syncthreads();',
+             'if (d + 1 < nMarch) issue(i0 + d + 2);   // in flight while plane d is computed',
+             'const int sCur = (int)((d + 1) % 3);']
+    for nm in L.stencil:
+        ct = [p for p in L.arrays if p[1] == nm][0][2]
+        if ax_y is not None:
+            inner.append(f'const dvmh_dev::MarchView3<{ct}, IndexT> {nm}{{{nm}_lds, rows, cols, jy0, kx0, i0 + d, sCur}};')
+        else:
+            inner.append(f'const dvmh_dev::MarchView2<{ct}, IndexT> {nm}{{{nm}_lds, cols, kx0, i0 + d, sCur}};')
+    inner.append('const IndexT t0 = tz0 + d;')
+    if ax_y is not None:
+        inner.append(f'const IndexT t{ax_y} = ty0 + (IndexT)threadIdx.y;')
+    inner.append('#pragma unroll')
+    inner.append('for (int c = 0; c < COLS; c++) {')
+    pt = [f'const IndexT t{ax_x} = tx0 + (IndexT)threadIdx.x + (IndexT)c * (IndexT)blockDim.x;']
+    cc = [f't{ax_x} < cnt{ax_x}'] + ([f't{ax_y} < cnt{ax_y}'] if ax_y is not None else [])
+    pt.append(f'if ({" && ".join(cc)}) {{')
+    for i in range(r):
+        pt.append(IND + var_assign(L, i, f't{i}', f'lo{i}', f'st{i}'))
+    pt.append(indent('{\n' + indent(L.body + '\n', 1) + '}', 1))
+    pt.append('}')
+    inner += [IND + ln for ln in '\n'.join(pt).split('\n')]
+    inner.append('}')
+    inner.append('__syncthreads();')
+    b += [IND + ln for ln in '\n'.join(inner).split('\n')]
+    b.append('}')
+    b += red_kernel_tail(L, 'blockIdx.x', accumulate=False)
+    k.append(indent('\n'.join(b) + '\n', 1).rstrip('\n'))
+    k.append('}')
+    return '\n'.join(k) + '\n'
+
+
 def parallel_handler(L):
     r = L.rank
     h = [f'extern "C" void {L.name}_cuda(DvmType *pLoopRef{"".join(", DvmType " + p[1] + "_hdr[]" if p[0] == "array" else ", " + p[2] + " *" + p[1] + "_ptr" for p in L.params)}) {{']
-    b = handler_prologue(L, 'DVMH_AMD_KIND_PARALLEL')
+    b = handler_prologue(L, 'DVMH_AMD_KIND_STENCIL' if L.stencil else 'DVMH_AMD_KIND_PARALLEL')
     for p in L.arrays:
         b.append(f'if ({p[1]}_tfm != 0) {{ fprintf(stderr, "{L.name}: array {p[1]} is not in natural layout\\n"); abort(); }}')
     b.append('int depth = (int)tune[DVMH_AMD_T_DEPTH];')
@@ -461,16 +653,48 @@ def parallel_handler(L):
         b.append(f'DvmType cnt{i} = (boundsHigh[{i}] - boundsLow[{i}]) / loopSteps[{i}] + 1;')
     ax_x, ax_y, ax_z = r - 1, (r - 2 if r >= 2 else None), (r - 3 if r >= 3 else None)
     dep_ax = ax_z if ax_z is not None else (ax_y if ax_y is not None else ax_x)
-    b.append(f'DvmType nbx = (cnt{ax_x} + threads.x - 1) / threads.x;')
+    if L.stencil:
+        # LDS march variant: COLS points per thread on x, `march` planes per workgroup
+        b.append('int variant = (int)tune[DVMH_AMD_T_VARIANT];')
+        b.append('int mcols = (int)tune[DVMH_AMD_T_TILE_A];')
+        b.append('DvmType march = tune[DVMH_AMD_T_CHUNK];')
+        b.append('size_t ldsBytes = 0;')
+        unit = ' && '.join(f'loopSteps[{i}] == 1' for i in range(r))
+        shape_ok = 'threads.z == 1' + (' && threads.y == 1' if r == 2 else '')
+        b.append(f'if (variant == 1 && ({unit}) && {shape_ok} && mcols >= 1 && mcols <= 4 && march >= 1) {{')
+        rows = '((size_t)threads.y + 2)' if r == 3 else '(size_t)1'
+        b.append(IND + f'size_t mplane = {rows} * ((size_t)threads.x * mcols + 2);')
+        for nm in L.stencil:
+            ct = [p for p in L.arrays if p[1] == nm][0][2]
+            b.append(IND + f'ldsBytes += (3 * mplane * sizeof({ct}) + 15) & ~(size_t)15;')
+        b.append(IND + 'if (ldsBytes > 64512) variant = 0;   // leave room for the reductions\' static LDS')
+        b.append(IND + 'size_t mthreads = (size_t)threads.x * threads.y;')
+        b.append(IND + 'if ((mplane + mthreads - 1) / mthreads > (size_t)(3 * mcols + 2)) variant = 0;   // register share of a plane')
+        b.append('} else {')
+        b.append(IND + 'variant = 0;')
+        b.append('}')
+        b.append('DvmType nbx, nby, nbz;')
+        b.append('if (variant == 1) {')
+        b.append(IND + f'nbx = (cnt{ax_x} + (DvmType)threads.x * mcols - 1) / ((DvmType)threads.x * mcols);')
+        b.append(IND + (f'nby = (cnt{ax_y} + threads.y - 1) / threads.y;' if r == 3 else 'nby = 1;'))
+        b.append(IND + 'nbz = (cnt0 + march - 1) / march;')
+        b.append('} else {')
+        pre = IND
+    else:
+        b.append('DvmType nbx, nby, nbz;')
+        pre = ''
+    b.append(pre + f'nbx = (cnt{ax_x} + threads.x - 1) / threads.x;')
     if ax_y is not None:
         dy = ' * depth' if dep_ax == ax_y else ''
-        b.append(f'DvmType nby = (cnt{ax_y} + threads.y{dy} - 1) / (threads.y{dy});')
+        b.append(pre + f'nby = (cnt{ax_y} + threads.y{dy} - 1) / (threads.y{dy});')
     else:
-        b.append('DvmType nby = 1;')
+        b.append(pre + 'nby = 1;')
     if ax_z is not None:
-        b.append(f'DvmType nbz = (cnt{ax_z} + threads.z * depth - 1) / (threads.z * depth);')
+        b.append(pre + f'nbz = (cnt{ax_z} + threads.z * depth - 1) / (threads.z * depth);')
     else:
-        b.append('DvmType nbz = 1;')
+        b.append(pre + 'nbz = 1;')
+    if L.stencil:
+        b.append('}')
     nb = 'nbx * nby * nbz' + ''.join(f' * cnt{a}' for a in range(0, r - 3))
     b.append(f'DvmType nblocks = {nb};')
     b += handler_reds_prepare(L, 'nblocks', 0)
@@ -481,6 +705,22 @@ def parallel_handler(L):
             kernels.append((cond, f'{L.name}_kernel<{it}, {dp}>'))
     b.append('if (nblocks > 0) {')
     b.append(IND + 'dvmh_amd_mark_kernel_C(loop_ref, 0);')
+    if L.stencil:
+        b.append(IND + 'if (variant == 1) {')
+        for it in ['int', 'long long']:
+            args = array_fact_args(L, it)
+            for i in range(r):
+                args += [f'({it})boundsLow[{i}]', f'({it})cnt{i}', f'({it})loopSteps[{i}]']
+            args += [f'({it})nbx', f'({it})nby', f'({it})nbz', 'nblocks', '(int)tune[DVMH_AMD_T_XCD]', f'({it})march']
+            args += red_fact_args(L)
+            for cols in (1, 2, 3, 4):
+                cond = f'kernelIndexT == {"rt_INT" if it == "int" else "rt_LLONG"} && mcols == {cols}'
+                b.append(IND * 2 + f'if ({cond}) hipLaunchKernelGGL(({L.name}_march_kernel<{it}, {cols}>), '
+                         f'dim3((unsigned)nblocks), threads, ldsBytes, stream, {", ".join(args)});')
+        b.append(IND + '} else {')
+        ind2 = IND * 2
+    else:
+        ind2 = IND
     for it in ['int', 'long long']:
         args = array_fact_args(L, it)
         for i in range(r):
@@ -488,7 +728,9 @@ def parallel_handler(L):
         args += [f'({it})nbx', f'({it})nby', f'({it})nbz', 'nblocks', '(int)tune[DVMH_AMD_T_XCD]']
         args += red_fact_args(L)
         ks = [(c, k) for c, k in kernels if (('rt_INT' in c) == (it == 'int'))]
-        b += [IND + s for s in launch_dispatch(ks, args, 'dim3((unsigned)nblocks)')]
+        b += [ind2 + s for s in launch_dispatch(ks, args, 'dim3((unsigned)nblocks)')]
+    if L.stencil:
+        b.append(IND + '}')
     b.append(IND + 'dvmh_amd_mark_kernel_C(loop_ref, 1);')
     b.append('}')
     b += handler_reds_finish(L)
@@ -939,6 +1181,8 @@ def gpu_loop(L):
         text.append(across_handler(L))
     else:
         text.append(parallel_kernel(L))
+        if L.stencil:
+            text.append(march_kernel(L))
         text.append(parallel_handler(L))
         if L.batch:
             text.append(batch_kernel(L))

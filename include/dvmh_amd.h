@@ -23,7 +23,10 @@ extern "C" {
 #define DVMH_AMD_TUNE_N 8
 #define DVMH_AMD_KIND_PARALLEL 0   /* independent iterations, one kernel */
 #define DVMH_AMD_KIND_LINE 1       /* ACROSS with one dependent axis: line sweeps */
-#define DVMH_AMD_KIND_WAVEFRONT 2  /* ACROSS with >= 2 dependent axes: tiled wavefront */
+#define DVMH_AMD_KIND_WAVEFRONT 2  /* ACROSS with >= 2 dependent axes: hyperplane wavefront */
+#define DVMH_AMD_KIND_STENCIL 3    /* PARALLEL whose handler also has the LDS march variant
+                                      (VARIANT 1: TILE_A = points per thread on the innermost
+                                      axis, CHUNK = planes per workgroup on the outermost) */
 /* tune[] slots */
 #define DVMH_AMD_T_DEPTH 0     /* iterations of the slowest mapped axis per thread */
 #define DVMH_AMD_T_XCD 1       /* 1: XCD-aware block-to-tile order */
@@ -120,6 +123,10 @@ DvmType dvmh_amd_tuner_site_pending_C(const char *file, DvmType line);
 /* Best candidate so far for the loop site (file, line): out[0] candidate index,
  * out[1] time in ns, out[2..9] block x,y,z and tune[0..4]; returns 0 if unknown. */
 DvmType dvmh_amd_tuner_query_C(const char *file, DvmType line, DvmType out[10]);
+/* Forces the launch configuration (block[3], tune[DVMH_AMD_TUNE_N]) of every
+ * loop of the given kind, over whatever the mode would choose; forced
+ * executions are not timed.  block == NULL removes the forcing for the kind. */
+void dvmh_amd_tuner_force_C(DvmType kind, const DvmType block[3], const DvmType tune[DVMH_AMD_TUNE_N]);
 /* Keys of the known loop sites, one per line, into buf (truncated to bufLen);
  * returns the length needed including the terminating zero. */
 DvmType dvmh_amd_tuner_sites_C(char *buf, DvmType bufLen);

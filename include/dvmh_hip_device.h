@@ -57,6 +57,56 @@ __device__ __forceinline__ View<T, I, R, UNIT> makeView(T *base, const I *stride
     return v;
 }
 
+// ------------------------------------------------------------------ LDS march views
+// A read-only stencil array staged in LDS as a rolling window of three planes
+// of the outermost loop axis (planes i-1, i, i+1 of the point being computed).
+// A plane holds `rows` x `cols` elements: the workgroup's tile of the inner
+// axes plus a halo of one on each side; row 0 / column 0 hold array indices
+// j0 / k0.  `A[i'][j'][k']` with |i' - i| <= 1 reads the LDS copy, so a loop
+// body written against the global array runs unchanged.
+template <typename T, typename I>
+struct MarchRow {
+    const T *p;
+    I k0;
+    __device__ __forceinline__ const T &operator[](I k) const { return p[k - k0]; }
+};
+
+template <typename T, typename I>
+struct MarchPlane {
+    const T *p;
+    I cols, j0, k0;
+    __device__ __forceinline__ MarchRow<T, I> operator[](I j) const { return MarchRow<T, I>{p + (j - j0) * cols, k0}; }
+};
+
+__device__ __forceinline__ int marchSlot(int cur, int delta) {
+    int s = cur + delta;
+    return s < 0 ? s + 3 : (s > 2 ? s - 3 : s);
+}
+
+// rank 3: planes of rows x cols
+template <typename T, typename I>
+struct MarchView3 {
+    const T *lds;
+    I rows, cols, j0, k0;
+    I iCur;
+    int sCur;
+    __device__ __forceinline__ MarchPlane<T, I> operator[](I i) const {
+        return MarchPlane<T, I>{lds + (I)marchSlot(sCur, (int)(i - iCur)) * rows * cols, cols, j0, k0};
+    }
+};
+
+// rank 2: planes are single rows of cols elements
+template <typename T, typename I>
+struct MarchView2 {
+    const T *lds;
+    I cols, k0;
+    I iCur;
+    int sCur;
+    __device__ __forceinline__ MarchRow<T, I> operator[](I i) const {
+        return MarchRow<T, I>{lds + (I)marchSlot(sCur, (int)(i - iCur)) * cols, k0};
+    }
+};
+
 // Diagonal position inside an Rx x Ry plane (x fastest).  Also compiled for
 // the host, so the layout-transform kernels, the handlers and host-side checks
 // share one definition.

@@ -1,0 +1,86 @@
+"""Kernel variants the tuner can choose, forced one at a time and compared with
+the oracle bit for bit: the LDS march variant of PARALLEL stencil loops
+(rolling three-plane window, several tile shapes, points per thread and march
+lengths, including tiles that overhang the loop on every axis)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import dvmh_amd as D
+import helpers as H
+
+KIND_STENCIL = 3
+T_DEPTH, T_XCD, T_VARIANT, T_CHUNK, T_TILE_A = 0, 1, 2, 3, 4
+
+
+def _force(kind, block, tune):
+    r = D.runtime()
+    r.dvmh_amd_tuner_force_C.argtypes = [ctypes.c_long, ctypes.c_void_p, ctypes.c_void_p]
+    if block is None:
+        r.dvmh_amd_tuner_force_C(kind, None, None)
+        return
+    b = (ctypes.c_long * 3)(*block)
+    t = (ctypes.c_long * 8)(*tune)
+    r.dvmh_amd_tuner_force_C(kind, ctypes.cast(b, ctypes.c_void_p), ctypes.cast(t, ctypes.c_void_p))
+
+
+def _march(bx, by, cols, march, xcd=0):
+    tune = [0] * 8
+    tune[T_DEPTH] = 1
+    tune[T_VARIANT] = 1
+    tune[T_TILE_A] = cols
+    tune[T_CHUNK] = march
+    tune[T_XCD] = xcd
+    return (bx, by, 1), tune
+
+
+MARCH3 = [(64, 1, 1, 8, 0), (128, 4, 3, 16, 1), (256, 2, 2, 32, 0), (64, 8, 4, 8, 1), (128, 2, 1, 64, 1)]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('cfg', MARCH3)
+@pytest.mark.parametrize('n,iters', [(10, 3), (66, 4), (131, 3)])
+def test_jac3d_march_variant_bit_exact(cfg, n, iters):
+    block, tune = _march(*cfg)
+    _force(KIND_STENCIL, block, tune)
+    try:
+        res = H.run_jac3d(n, iters)
+    finally:
+        _force(KIND_STENCIL, None, None)
+    ref = H.oracle_jac3d(n, iters)
+    assert H.bits_equal(res['A'], ref['A']) and H.bits_equal(res['B'], ref['B'])
+    assert np.array_equal(res['eps'], ref['eps'])
+    assert H.sum_close(res['sum'], ref['sum'])
+
+
+MARCH2 = [(64, 1, 1, 8, 0), (256, 1, 4, 32, 1), (1024, 1, 2, 16, 0), (128, 1, 3, 64, 1)]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('cfg', MARCH2)
+@pytest.mark.parametrize('n,iters', [(5, 3), (70, 5), (1000, 3)])
+def test_jac2d_march_variant_bit_exact(cfg, n, iters):
+    block, tune = _march(*cfg)
+    _force(KIND_STENCIL, block, tune)
+    try:
+        res = H.run_jac2d(n, iters)
+    finally:
+        _force(KIND_STENCIL, None, None)
+    ref = H.oracle_jac2d(n, iters)
+    assert H.bits_equal(res['A'], ref['A']) and H.bits_equal(res['B'], ref['B'])
+    assert np.array_equal(res['eps'], ref['eps'])
+
+
+@pytest.mark.gpu
+def test_march_variant_full_size_jac3d():
+    """384^3 x 4 iterations with the variant's likely-best shape."""
+    block, tune = _march(128, 2, 3, 16, 1)
+    _force(KIND_STENCIL, block, tune)
+    try:
+        res = H.run_jac3d(384, 4)
+    finally:
+        _force(KIND_STENCIL, None, None)
+    ref = H.oracle_jac3d(384, 4)
+    assert H.bits_equal(res['A'], ref['A']) and np.array_equal(res['eps'], ref['eps'])
+    assert H.sum_close(res['sum'], ref['sum'])
