@@ -73,7 +73,7 @@ std::vector<Candidate> makeCandidates(int kind, int loopRank, int depCount, cons
             const int mxs[] = {64, 128, 256, 512, 1024};
             const int mys[] = {1, 2, 4, 8};
             const int mcs[] = {1, 2, 3, 4};
-            const int marches[] = {8, 16, 32};
+            const int marches[] = {8, 16, 32, 64};
             for (int bx : mxs)
                 for (int by : mys) {
                     if (loopRank == 2 && by > 1) continue;

@@ -10,7 +10,7 @@
 
     python -m dvmh_amd.costmodel sweep --out records.tsv
     python -m dvmh_amd.costmodel train --records records.tsv --out costmodel.bin --holdout '|e382x382x382'
-    python -m dvmh_amd.costmodel eval --records records.tsv --model costmodel.bin --site 'jac3d.cdv:41|k0|r3|m0|e382x382x382'
+    python -m dvmh_amd.costmodel eval --records records.tsv --model costmodel.bin --site 'jac3d.cdv:41|k3|r3|m0|e382x382x382'
 """
 import argparse
 import ctypes
@@ -37,7 +37,7 @@ DEFAULT_SPECS = [
     ('sor3d', [48, 64, 96, 128]),
 ]
 HOLDOUT = '|e382x382x382'
-HELD_OUT_SITE = 'jac3d.cdv:41|k0|r3|m0|e382x382x382'
+HELD_OUT_SITE = 'jac3d.cdv:41|k3|r3|m0|e382x382x382'
 
 _bound = None
 
