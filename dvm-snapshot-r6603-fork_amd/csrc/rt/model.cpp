@@ -87,9 +87,12 @@ void features(const SiteInfo &s, const Candidate &c, float x[F]) {
         double dep = 1;
         for (int i = 0; i < s.rank; i++) dep = std::max(dep, (double)s.extents[i]);
         perWg = threads * dep;   // each thread walks one line
+    } else if (c.tune[DVMH_AMD_T_VARIANT] == 1) {
+        // tiled wavefront: TILE_A x TILE_A x TILE_B iterations per workgroup
+        perWg = (double)c.tune[DVMH_AMD_T_TILE_A] * (double)c.tune[DVMH_AMD_T_TILE_A] * (double)c.tune[DVMH_AMD_T_TILE_B];
     } else {
-        perWg = (double)std::max(1L, (long)c.tune[DVMH_AMD_T_TILE_A]) * std::max(1L, (long)c.tune[DVMH_AMD_T_TILE_B]) *
-                std::max(1L, (long)c.tune[DVMH_AMD_T_CHUNK]);
+        // one hyperplane launch: a block of threads over the free dependent axes
+        perWg = threads;
     }
     x[20] = (float)lg(perWg);
     x[21] = (float)lg(std::max(1.0, pts / perWg));

@@ -118,6 +118,20 @@ std::vector<Candidate> makeCandidates(int kind, int loopRank, int depCount, cons
                 if (n < 64 || n > 512) continue;
                 for (int tf = 0; tf <= tfMax; tf++) addCand(v, bx, by, 1, {{DVMH_AMD_T_TRANSFORM, tf}});
             }
+        if (loopRank == 3 && depCount == 3) {
+            // tiled variant: TILE_A x TILE_A x TILE_B tiles staged in LDS with a
+            // halo of one, one launch per tile-plane
+            const int nts[] = {64, 128, 256, 512};
+            const int tas[] = {4, 8, 12, 16};
+            const int tcs[] = {8, 16, 32, 64};
+            for (int nt : nts)
+                for (int ta : tas)
+                    for (int tc : tcs) {
+                        if ((long)(ta + 2) * (ta + 2) * (tc + 2) * 8 > 64512) continue;
+                        if (nt > 4 * ta * ta) continue;   // threads cover a tile's (a, b) columns
+                        addCand(v, nt, 1, 1, {{DVMH_AMD_T_VARIANT, 1}, {DVMH_AMD_T_TILE_A, ta}, {DVMH_AMD_T_TILE_B, tc}});
+                    }
+        }
     }
     (void)extents;
     return v;

@@ -104,6 +104,33 @@ class Loop:
             return False
         return True
 
+    def tile_eligible(self):
+        """Rank 3, every loop axis in the ACROSS clause with widths <= 1, and
+        every access of the ACROSS array in the body of the form
+        X[i + d0][j + d1][k + d2] with at most one offset, |offset| <= 1."""
+        if not self.across or self.rank != 3:
+            return False
+        w = self.across['widths']
+        if len(w) != 3 or any(lo > 1 or hi > 1 or (lo == 0 and hi == 0) for lo, hi in w):
+            return False
+        name = self.across['array']
+        if re.search(r'\b' + name + r'\b(?!\s*\[)', self.body):
+            return False
+        for m in re.finditer(r'\b' + name + r'\s*((?:\[[^\[\]]*\]\s*)+)', self.body):
+            idx = re.findall(r'\[([^\[\]]*)\]', m.group(1))
+            if len(idx) != 3:
+                return False
+            moved = 0
+            for (v, _), e in zip(self.vars, idx):
+                mm = re.fullmatch(r'\s*' + v + r'\s*(?:([+-])\s*([01])\s*)?', e)
+                if not mm:
+                    return False
+                if mm.group(2) == '1':
+                    moved += 1
+            if moved > 1:
+                return False
+        return True
+
     # static dependency masks the ACROSS clause can produce (identity correspondence)
     def dep_axes(self):
         if not self.across:
@@ -1121,6 +1148,123 @@ def wave_launch(L, deps):
     return b
 
 
+def tile_kernel(L):
+    """Tiled wavefront for an ACROSS loop whose three axes all depend: tiles of
+    TA x TA x TC iterations are run one tile-plane (a + b + c = const) per
+    launch.  A workgroup stages its tile of the ACROSS array plus a halo of one
+    in LDS (lower halo: final values of earlier tiles; upper halo: values no
+    tile has updated yet), runs the tile's local hyperplanes in order with a
+    workgroup barrier between them, and writes the tile back.  The body reads
+    and writes the ACROSS array through a TileView3 with the usual X[i][j][k]
+    syntax; every dependency is met in the sequential order's sense, so the
+    results are those of the lexicographic sweep."""
+    X = L.across['array']
+    xct = [p for p in L.arrays if p[1] == X][0][2]
+    fp, setup = [], []
+    for p in L.arrays:
+        _, nm, ct, ar = p
+        fp.append(f'{ct} *__restrict__ {nm}_base')
+        fp += [f'IndexT {nm}_s{k}' for k in range(ar)]
+        strides = ', '.join(f'{nm}_s{k}' for k in range(ar))
+        setup.append(f'const IndexT {nm}_str[{ar}] = {{{strides}}};')
+        vn = f'{nm}_g' if nm == X else nm
+        setup.append(f'const auto {vn} = dvmh_dev::makeView<{ct}, IndexT, {ar}, true>({nm}_base, {nm}_str);')
+    for p in L.scalars:
+        fp.append(f'{p[2]} *__restrict__ {p[1]}_ptr')
+        setup.append(f'{p[2]} &{p[1]} = *{p[1]}_ptr;')
+    for i in range(3):
+        fp += [f'IndexT lo{i}', f'IndexT cnt{i}', f'IndexT st{i}']
+    fp += ['IndexT TA', 'IndexT TC', 'IndexT ntB', 'IndexT ntC', 'IndexT aLo', 'IndexT tplane']
+    fp += red_kernel_params(L)
+    k = ['template <typename IndexT>',
+         f'__global__ __launch_bounds__(1024) void {L.name}_tile_kernel({", ".join(fp)}) {{']
+    b = ['extern __shared__ __align__(16) unsigned char dvmh_lds[];']
+    b += setup
+    for v, ct in L.vars:
+        b.append(f'{ct} {v};')
+    for pr in L.privates:
+        b.append(pr + ';')
+    b += red_kernel_locals(L)
+    b += ['const IndexT ta = aLo + (IndexT)blockIdx.x / ntB;',
+          'const IndexT tb = (IndexT)blockIdx.x - ((IndexT)blockIdx.x / ntB) * ntB;',
+          'const IndexT tc = tplane - ta - tb;',
+          'if (tc < 0 || tc >= ntC) return;   // not on this tile-plane (uniform per workgroup)',
+          'const IndexT a0 = ta * TA, b0 = tb * TA, c0 = tc * TC;',
+          'const IndexT na = cnt0 - a0 < TA ? cnt0 - a0 : TA;',
+          'const IndexT nb = cnt1 - b0 < TA ? cnt1 - b0 : TA;',
+          'const IndexT nc = cnt2 - c0 < TC ? cnt2 - c0 : TC;',
+          'const IndexT R1 = nb + 2, R2 = nc + 2, R12 = R1 * R2;',
+          'const IndexT elems = (na + 2) * R12;',
+          f'{xct} *tile = ({xct} *)dvmh_lds;',
+          'const IndexT oi = lo0 + a0 - 1, oj = lo1 + b0 - 1, ok = lo2 + c0 - 1;   // array index of tile (0, 0, 0)',
+          'const IndexT tid = (IndexT)threadIdx.x, nthr = (IndexT)blockDim.x;',
+          '// stage the tile and its halo (only indices the loop can read: loop box plus one)',
+          'for (IndexT e = tid; e < elems; e += nthr) {',
+          IND + 'const IndexT p = e / R12, rem = e - (e / R12) * R12;',
+          IND + 'const IndexT q = rem / R2, r = rem - (rem / R2) * R2;',
+          IND + 'const IndexT ii = oi + p, jj = oj + q, kk = ok + r;',
+          IND + 'if (ii >= lo0 - 1 && ii <= lo0 + cnt0 && jj >= lo1 - 1 && jj <= lo1 + cnt1 && kk >= lo2 - 1 && kk <= lo2 + cnt2)',
+          IND * 2 + f'tile[e] = {X}_g[ii][jj][kk];',
+          '}',
+          '__syncthreads();',
+          f'const dvmh_dev::TileView3<{xct}, IndexT> {X}{{tile, R1, R2, oi, oj, ok}};',
+          '// local hyperplanes p + q + r = s, in order',
+          'const IndexT nab = na * nb;',
+          'for (IndexT s = 0; s < na + nb + nc - 2; s++) {',
+          IND + 'for (IndexT idx = tid; idx < nab; idx += nthr) {',
+          IND * 2 + 'const IndexT p = idx / nb, q = idx - (idx / nb) * nb;',
+          IND * 2 + 'const IndexT r = s - p - q;',
+          IND * 2 + 'if (r >= 0 && r < nc) {',
+          IND * 3 + 'const IndexT t0 = a0 + p, t1 = b0 + q, t2 = c0 + r;']
+    for i in range(3):
+        b.append(IND * 3 + var_assign(L, i, f't{i}', f'lo{i}', f'st{i}'))
+    b.append(indent('{\n' + indent(L.body + '\n', 1) + '}', 3))
+    b += [IND * 2 + '}',
+          IND + '}',
+          IND + '__syncthreads();',
+          '}',
+          '// write back the tile (not the halo)',
+          'const IndexT nbc = nb * nc;',
+          'for (IndexT e = tid; e < na * nbc; e += nthr) {',
+          IND + 'const IndexT p = e / nbc, rem = e - (e / nbc) * nbc;',
+          IND + 'const IndexT q = rem / nc, r = rem - (rem / nc) * nc;',
+          IND + f'{X}_g[oi + 1 + p][oj + 1 + q][ok + 1 + r] = tile[((p + 1) * R1 + (q + 1)) * R2 + (r + 1)];',
+          '}']
+    b += red_kernel_tail(L, '((long)(ta * ntB + tb) * ntC + tc)', accumulate=False)
+    k.append(indent('\n'.join(b) + '\n', 1).rstrip('\n'))
+    k.append('}')
+    return '\n'.join(k) + '\n'
+
+
+def tile_launch(L):
+    X = L.across['array']
+    xct = [p for p in L.arrays if p[1] == X][0][2]
+    b = ['DvmType TA = tune[DVMH_AMD_T_TILE_A], TC = tune[DVMH_AMD_T_TILE_B];',
+         'DvmType ntA = (cnt0 + TA - 1) / TA, ntB = (cnt1 + TA - 1) / TA, ntC = (cnt2 + TC - 1) / TC;',
+         f'size_t ldsBytes = (size_t)(TA + 2) * (TA + 2) * (TC + 2) * sizeof({xct});',
+         'DvmType tiles = ntA * ntB * ntC;']
+    b += handler_reds_prepare(L, 'tiles', 0)
+    b.append('dvmh_amd_mark_kernel_C(loop_ref, 0);')
+    b.append('for (DvmType tplane = 0; tplane <= (ntA - 1) + (ntB - 1) + (ntC - 1); tplane++) {')
+    b.append(IND + 'DvmType aLo = tplane - (ntB - 1) - (ntC - 1);')
+    b.append(IND + 'if (aLo < 0) aLo = 0;')
+    b.append(IND + 'DvmType aHi = tplane < ntA - 1 ? tplane : ntA - 1;')
+    b.append(IND + 'DvmType grid = (aHi - aLo + 1) * ntB;')
+    for it in ['int', 'long long']:
+        args = array_fact_args(L, it)
+        for i in range(3):
+            args += [f'({it})boundsLow[{i}]', f'({it})cnt{i}', f'({it})loopSteps[{i}]']
+        args += [f'({it})TA', f'({it})TC', f'({it})ntB', f'({it})ntC', f'({it})aLo', f'({it})tplane']
+        args += red_fact_args(L)
+        cond = f'kernelIndexT == {"rt_INT" if it == "int" else "rt_LLONG"}'
+        b.append(IND + f'if ({cond}) hipLaunchKernelGGL(({L.name}_tile_kernel<{it}>), dim3((unsigned)grid), threads, ldsBytes, '
+                 f'stream, {", ".join(args)});')
+    b.append('}')
+    b.append('dvmh_amd_mark_kernel_C(loop_ref, 1);')
+    b += handler_reds_finish(L)
+    return b
+
+
 def across_handler(L):
     r = L.rank
     ndep = len(L.dep_axes())
@@ -1159,7 +1303,21 @@ def across_handler(L):
         for deps in itertools.combinations(range(r), m):
             mask = sum(1 << (r - 1 - a) for a in deps)
             b.append(f'}} else if (dependency_mask == {mask}) {{')
-            b += [IND + s for s in wave_launch(L, list(deps))]
+            if m == 3 and L.tile_eligible():
+                X = L.across['array']
+                xct = [p for p in L.arrays if p[1] == X][0][2]
+                unit = ' && '.join(f'loopSteps[{i}] == 1' for i in range(3))
+                b.append(IND + '// tiled variant: unit forward steps, natural layout, tile fits 64 KB of LDS')
+                b.append(IND + f'bool tiled = tune[DVMH_AMD_T_VARIANT] == 1 && {unit} && {X}_tfm == 0 && threads.y == 1 && '
+                         'threads.z == 1 && tune[DVMH_AMD_T_TILE_A] >= 2 && tune[DVMH_AMD_T_TILE_B] >= 2 && '
+                         f'(tune[DVMH_AMD_T_TILE_A] + 2) * (tune[DVMH_AMD_T_TILE_A] + 2) * (tune[DVMH_AMD_T_TILE_B] + 2) * (DvmType)sizeof({xct}) <= 64512;')
+                b.append(IND + 'if (tiled) {')
+                b += [IND * 2 + s for s in tile_launch(L)]
+                b.append(IND + '} else {')
+                b += [IND * 2 + s for s in wave_launch(L, list(deps))]
+                b.append(IND + '}')
+            else:
+                b += [IND + s for s in wave_launch(L, list(deps))]
     b.append('} else {')
     b.append(IND + f'fprintf(stderr, "{L.name}: unsupported dependency mask %ld\\n", (long)dependency_mask);')
     b.append(IND + 'abort();')
@@ -1178,6 +1336,8 @@ def gpu_loop(L):
         for m in range(2, r + 1):
             for deps in itertools.combinations(range(r), m):
                 text.append(wave_kernel(L, list(deps)))
+        if L.tile_eligible():
+            text.append(tile_kernel(L))
         text.append(across_handler(L))
     else:
         text.append(parallel_kernel(L))

@@ -107,6 +107,31 @@ struct MarchView2 {
     }
 };
 
+// A tile of a rank-3 array staged in LDS (read and written): element (0,0,0)
+// holds array index (i0, j0, k0), planes of rows x cols elements.
+template <typename T, typename I>
+struct TileRow {
+    T *p;
+    I k0;
+    __device__ __forceinline__ T &operator[](I k) const { return p[k - k0]; }
+};
+
+template <typename T, typename I>
+struct TilePlane {
+    T *p;
+    I cols, j0, k0;
+    __device__ __forceinline__ TileRow<T, I> operator[](I j) const { return TileRow<T, I>{p + (j - j0) * cols, k0}; }
+};
+
+template <typename T, typename I>
+struct TileView3 {
+    T *p;
+    I rows, cols, i0, j0, k0;
+    __device__ __forceinline__ TilePlane<T, I> operator[](I i) const {
+        return TilePlane<T, I>{p + (i - i0) * rows * cols, cols, j0, k0};
+    }
+};
+
 // Diagonal position inside an Rx x Ry plane (x fastest).  Also compiled for
 // the host, so the layout-transform kernels, the handlers and host-side checks
 // share one definition.

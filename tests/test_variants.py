@@ -84,3 +84,49 @@ def test_march_variant_full_size_jac3d():
     ref = H.oracle_jac3d(384, 4)
     assert H.bits_equal(res['A'], ref['A']) and np.array_equal(res['eps'], ref['eps'])
     assert H.sum_close(res['sum'], ref['sum'])
+
+
+# ---------------------------------------------------------------- tiled wavefront (ACROSS on three axes)
+KIND_WAVEFRONT = 2
+
+
+def _tile(nt, ta, tc):
+    tune = [0] * 8
+    tune[T_DEPTH] = 1
+    tune[T_VARIANT] = 1
+    tune[T_TILE_A] = ta
+    tune[5] = tc          # TILE_B
+    return (nt, 1, 1), tune
+
+
+TILES = [(64, 4, 8), (256, 8, 16), (512, 16, 16), (128, 8, 64), (256, 12, 32)]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('cfg', TILES)
+@pytest.mark.parametrize('n,iters', [(5, 2), (23, 3), (70, 2)])
+def test_sor3d_tiled_wavefront_bit_exact(cfg, n, iters):
+    """Tiles that overhang the loop on every axis, and tiles larger than the
+    whole loop (n = 5: 3 x 3 x 3 iterations)."""
+    block, tune = _tile(*cfg)
+    _force(KIND_WAVEFRONT, block, tune)
+    try:
+        res = H.run_sor3d(n, iters)
+    finally:
+        _force(KIND_WAVEFRONT, None, None)
+    ref = H.oracle_sor3d(n, iters)
+    assert H.bits_equal(res['A'], ref['A'])
+    assert np.array_equal(res['eps'], ref['eps'])
+
+
+@pytest.mark.gpu
+def test_sor3d_tiled_full_size():
+    """320^3 (the SOR workload's size), 2 sweeps, against the oracle."""
+    block, tune = _tile(256, 16, 16)
+    _force(KIND_WAVEFRONT, block, tune)
+    try:
+        res = H.run_sor3d(320, 2)
+    finally:
+        _force(KIND_WAVEFRONT, None, None)
+    ref = H.oracle_sor3d(320, 2)
+    assert H.bits_equal(res['A'], ref['A']) and np.array_equal(res['eps'], ref['eps'])
