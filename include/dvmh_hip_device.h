@@ -7,6 +7,11 @@
  *   DiagView   the same for a layout whose two fastest storage axes are
  *              diagonalised (planes stored anti-diagonal by anti-diagonal), the
  *              index map of rts/dvmh/include/dvmhlib_device.h:56-98.
+ *   MarchView2/3, TileView3
+ *              the same syntax over copies staged in LDS: a rolling window of
+ *              three planes of a read-only stencil array (march variant of
+ *              PARALLEL loops) and a tile with halo of an ACROSS array (tiled
+ *              wavefront).
  *   reductions wave64 shuffle trees plus one LDS step across the waves of a
  *              workgroup; every thread of the workgroup must call them.
  *   xcdTile    XCD-aware renumbering of workgroups: the hardware deals
