@@ -7,6 +7,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstdio>
+#include <cstdlib>
 
 #include "launch.h"
 #include "dvmh_hip_device.h"
@@ -98,6 +100,16 @@ extern "C" long dvmh_amd_convert_xy_C(long x, long y, long Rx, long Ry, long sla
 
 extern "C" void dvmh_hip_permute(const void *src, void *dst, long typeSize, int rank, const long *srcSizes,
         const int *dstFromSrc, hipStream_t s) {
+    // the gather stays inside the buffer only for a true permutation
+    unsigned seen = 0;
+    for (int p = 0; p < rank; p++) {
+        const int q = dstFromSrc[p];
+        if (rank > MAXR || q < 0 || q >= rank || ((seen >> q) & 1u)) {
+            fprintf(stderr, "dvmh_hip_permute: not a permutation of %d axes\n", rank);
+            abort();
+        }
+        seen |= 1u << q;
+    }
     long srcStride[MAXR];
     long coll = 1;
     for (int q = rank - 1; q >= 0; q--) { srcStride[q] = coll; coll *= srcSizes[q]; }
@@ -168,6 +180,8 @@ extern "C" void dvmh_hip_diagonal(const void *src, void *dst, long typeSize, lon
         case 4: hipLaunchKernelGGL(diagonalKernel<uint32_t>, dim3(blocks), dim3(256), 0, s, (const uint32_t *)src, (uint32_t *)dst, Rx, Ry, Rz, back, slash); break;
         case 8: hipLaunchKernelGGL(diagonalKernel<uint64_t>, dim3(blocks), dim3(256), 0, s, (const uint64_t *)src, (uint64_t *)dst, Rx, Ry, Rz, back, slash); break;
         case 16: hipLaunchKernelGGL(diagonalKernel<Bytes16>, dim3(blocks), dim3(256), 0, s, (const Bytes16 *)src, (Bytes16 *)dst, Rx, Ry, Rz, back, slash); break;
-        default: break;
+        default:
+            fprintf(stderr, "dvmh_hip_diagonal: element size %ld has no diagonal kernel\n", typeSize);
+            abort();
     }
 }

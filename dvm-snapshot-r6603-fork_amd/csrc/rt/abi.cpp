@@ -1255,6 +1255,8 @@ extern "C" void dvmh_amd_get_stats_C(DvmType out[8]) {
     out[7] = batchLoops;
 }
 
+extern "C" DvmType dvmh_amd_layout_changes_C(void) { return layoutChanges(); }
+
 extern "C" void dvmh_amd_reset_stats_C(void) {
     rt.loopsOnHost = rt.loopsOnGpu = 0;
     rt.gpuLoopSeconds = 0;

@@ -293,6 +293,7 @@ void Data::setLayout(const int newPerm[], int newDiag) {
         storageSizes(this, sizes);
         long Rx = sizes[rank - 1], Ry = sizes[rank - 2], Rz = 1;
         for (int q = 0; q < rank - 2; q++) Rz *= sizes[q];
+        DVMH_INTERNAL((size_t)(Rx * Ry * Rz) * typeSize == nb, "diagonal planes do not cover the array");
         void *tmp = gpu.alloc(nb);
         dvmh_hip_diagonal(dev.ptr, tmp, typeSize, Rx, Ry, Rz, 1, dev.diagState == 2, gpu.stream);
         gpu.release(dev.ptr);
@@ -320,6 +321,7 @@ void Data::setLayout(const int newPerm[], int newDiag) {
         storageSizes(this, sizes);
         long Rx = sizes[rank - 1], Ry = sizes[rank - 2], Rz = 1;
         for (int q = 0; q < rank - 2; q++) Rz *= sizes[q];
+        DVMH_INTERNAL((size_t)(Rx * Ry * Rz) * typeSize == nb, "diagonal planes do not cover the array");
         void *tmp = gpu.alloc(nb);
         dvmh_hip_diagonal(dev.ptr, tmp, typeSize, Rx, Ry, Rz, 0, newDiag == 2, gpu.stream);
         gpu.release(dev.ptr);

@@ -131,6 +131,30 @@ void jac3d_fetch(double *outA, double *outB) {
     }
 }
 
+/* get_actual(X[lo0:hi0][lo1:hi1][lo2:hi2]) of A (which = 0) or B (1), then
+ * copies that block, row-major, into out. */
+void jac3d_fetch_block(int which, const long lo[3], const long hi[3], double *out) {
+    DvmType *X = which ? B : A;
+    dvmh_get_actual_subarray_C(X, 3, (DvmType)lo[0], (DvmType)hi[0], (DvmType)lo[1], (DvmType)hi[1], (DvmType)lo[2],
+            (DvmType)hi[2]);
+    const double *x = (const double *)dvmh_get_natural_base_C(0, X);
+    for (long i = lo[0]; i <= hi[0]; i++)
+        for (long j = lo[1]; j <= hi[1]; j++)
+            for (long k = lo[2]; k <= hi[2]; k++) *out++ = x[(i * n + j) * n + k];
+}
+
+/* Writes the block from in into the host copy of A (which = 0) or B (1), then
+ * actual(X[lo0:hi0][lo1:hi1][lo2:hi2]). */
+void jac3d_store_block(int which, const long lo[3], const long hi[3], const double *in) {
+    DvmType *X = which ? B : A;
+    double *x = (double *)dvmh_get_natural_base_C(0, X);
+    for (long i = lo[0]; i <= hi[0]; i++)
+        for (long j = lo[1]; j <= hi[1]; j++)
+            for (long k = lo[2]; k <= hi[2]; k++) x[(i * n + j) * n + k] = *in++;
+    dvmh_actual_subarray_C(X, 3, (DvmType)lo[0], (DvmType)hi[0], (DvmType)lo[1], (DvmType)hi[1], (DvmType)lo[2],
+            (DvmType)hi[2]);
+}
+
 void jac3d_teardown(void) {
     dvmh_delete_object_(B);
     dvmh_delete_object_(A);

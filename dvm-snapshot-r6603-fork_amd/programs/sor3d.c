@@ -93,4 +93,15 @@ void sor3d_fetch(double *outA) {
     for (long q = 0; q < total; q++) outA[q] = a[q];
 }
 
+/* get_actual(A[lo0:hi0][lo1:hi1][lo2:hi2]), then copies that block, row-major,
+ * into out. */
+void sor3d_fetch_block(const long lo[3], const long hi[3], double *out) {
+    dvmh_get_actual_subarray_C(A, 3, (DvmType)lo[0], (DvmType)hi[0], (DvmType)lo[1], (DvmType)hi[1], (DvmType)lo[2],
+            (DvmType)hi[2]);
+    const double *a = (const double *)dvmh_get_natural_base_C(0, A);
+    for (long i = lo[0]; i <= hi[0]; i++)
+        for (long j = lo[1]; j <= hi[1]; j++)
+            for (long k = lo[2]; k <= hi[2]; k++) *out++ = a[(i * n + j) * n + k];
+}
+
 void sor3d_teardown(void) { dvmh_delete_object_(A); }

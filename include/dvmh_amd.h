@@ -64,6 +64,9 @@ void dvmh_amd_set_graph_C(DvmType enable);
  * out[5] graph replays, out[6] batched groups, out[7] batched loops. */
 void dvmh_amd_get_stats_C(DvmType out[8]);
 void dvmh_amd_reset_stats_C(void);
+/* Device layout changes (axis permutations and diagonal re-layouts of arrays
+ * done for ACROSS loops, and their undoing) since the runtime started. */
+DvmType dvmh_amd_layout_changes_C(void);
 /* Device name / architecture string of the GPU in use ("" without one). */
 const char *dvmh_amd_device_name_C(void);
 /* Position of (x, y) in an Rx x Ry plane stored diagonal by diagonal (slash 0:
