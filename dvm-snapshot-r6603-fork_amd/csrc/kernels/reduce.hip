@@ -6,8 +6,8 @@
 // reduce inside each workgroup, so the runtime folds one partial per workgroup
 // (a few thousand values) in a single 1024-thread workgroup and only the final
 // record goes back to the host.  Order is fixed (thread t folds t, t+1024, ...
-// sequentially, then a fixed LDS tree), so the result is deterministic; for
-// MAX/MIN with loc the lowest partial index wins ties.
+// sequentially, then fixed wave64 shuffle trees), so the result is
+// deterministic; for MAX/MIN with loc the lowest partial index wins ties.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -63,29 +63,71 @@ __device__ inline void comb(int f, T &acc, long &accIdx, T v, long vIdx) {
     else combine<T>(f, acc, accIdx, v, vIdx);
 }
 
-// One workgroup folds vals[begin, end).  Element e of the range carries the
-// tie-break index idx ? idx[e] : e.  The result and the winner's index go to
-// outVal / outIdx (index -1 for an empty range).
+// Bit-exact lane shuffle of any element type used here (1, 4 or 8 bytes).
+template <typename T>
+__device__ inline T shflDownAny(T v, int o) {
+    if constexpr (sizeof(T) == 8) {
+        long long x;
+        __builtin_memcpy(&x, &v, 8);
+        x = __shfl_down(x, o, 64);
+        __builtin_memcpy(&v, &x, 8);
+        return v;
+    } else if constexpr (sizeof(T) == 4) {
+        int x;
+        __builtin_memcpy(&x, &v, 4);
+        x = __shfl_down(x, o, 64);
+        __builtin_memcpy(&v, &x, 4);
+        return v;
+    } else {
+        return (T)__shfl_down((int)v, o, 64);
+    }
+}
+
+// Tree over the 64 lanes of a wavefront; lane 0 ends with the fold of all.
+template <typename T, bool BITS>
+__device__ inline void waveFold(int f, T &acc, long &accIdx, int &has) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const T v = shflDownAny(acc, o);
+        const long vi = shflDownAny(accIdx, o);
+        const int vh = __shfl_down(has, o, 64);
+        if (vh && lane + o < 64) {
+            if (!has) { acc = v; accIdx = vi; has = 1; }
+            else comb<T, BITS>(f, acc, accIdx, v, vi);
+        }
+    }
+}
+
+// One workgroup (FIN_THREADS threads) folds vals[begin, end).  Element e of
+// the range carries the tie-break index idx ? idx[e] : e.  The result and the
+// winner's index go to outVal / outIdx (index -1 for an empty range).  Thread
+// t folds t, t + FIN_THREADS, ... in order (8 loads in flight), then a wave64
+// shuffle tree per wavefront and one more over the wavefront results: a fixed
+// order, so the result is deterministic.
 template <typename T, bool BITS>
 __device__ void foldRange(int f, long begin, long end, const T *__restrict__ vals, const long *__restrict__ idx, T *outVal,
         long *outIdx) {
-    __shared__ T sVal[FIN_THREADS];
-    __shared__ long sIdx[FIN_THREADS];
-    __shared__ int sHas[FIN_THREADS];
+    constexpr int NW = FIN_THREADS / 64;
+    __shared__ T sVal[NW];
+    __shared__ long sIdx[NW];
+    __shared__ int sHas[NW];
     const int t = threadIdx.x;
     T acc = identityOf<T>(f);
     long accIdx = -1;
     int has = 0;
-    // 4 independent loads in flight per thread, folded in index order
     long i = begin + t;
-    for (; i + 3 * FIN_THREADS < end; i += 4 * FIN_THREADS) {
-        T v0 = vals[i], v1 = vals[i + FIN_THREADS], v2 = vals[i + 2 * FIN_THREADS], v3 = vals[i + 3 * FIN_THREADS];
-        long i0 = idx ? idx[i] : i, i1 = idx ? idx[i + FIN_THREADS] : i + FIN_THREADS;
-        long i2 = idx ? idx[i + 2 * FIN_THREADS] : i + 2 * FIN_THREADS, i3 = idx ? idx[i + 3 * FIN_THREADS] : i + 3 * FIN_THREADS;
-        if (!has) { acc = v0; accIdx = i0; has = 1; } else comb<T, BITS>(f, acc, accIdx, v0, i0);
-        comb<T, BITS>(f, acc, accIdx, v1, i1);
-        comb<T, BITS>(f, acc, accIdx, v2, i2);
-        comb<T, BITS>(f, acc, accIdx, v3, i3);
+    for (; i + 7 * FIN_THREADS < end; i += 8 * FIN_THREADS) {
+        T v[8];
+        long vi[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            v[q] = vals[i + q * FIN_THREADS];
+            vi[q] = idx ? idx[i + q * FIN_THREADS] : i + q * FIN_THREADS;
+        }
+        if (!has) { acc = v[0]; accIdx = vi[0]; has = 1; } else comb<T, BITS>(f, acc, accIdx, v[0], vi[0]);
+#pragma unroll
+        for (int q = 1; q < 8; q++) comb<T, BITS>(f, acc, accIdx, v[q], vi[q]);
     }
     for (; i < end; i += FIN_THREADS) {
         T v = vals[i];
@@ -93,26 +135,23 @@ __device__ void foldRange(int f, long begin, long end, const T *__restrict__ val
         if (!has) { acc = v; accIdx = vi; has = 1; }
         else comb<T, BITS>(f, acc, accIdx, v, vi);
     }
-    sVal[t] = acc;
-    sIdx[t] = has ? accIdx : -1;
-    sHas[t] = has;
-    __syncthreads();
-    for (int w = FIN_THREADS / 2; w > 0; w >>= 1) {
-        if (t < w && sHas[t + w]) {
-            if (!sHas[t]) { sVal[t] = sVal[t + w]; sIdx[t] = sIdx[t + w]; sHas[t] = 1; }
-            else {
-                T a = sVal[t];
-                long ai = sIdx[t];
-                comb<T, BITS>(f, a, ai, sVal[t + w], sIdx[t + w]);
-                sVal[t] = a;
-                sIdx[t] = ai;
-            }
-        }
-        __syncthreads();
+    waveFold<T, BITS>(f, acc, accIdx, has);
+    const int w = t >> 6, lane = t & 63;
+    if (lane == 0) {
+        sVal[w] = acc;
+        sIdx[w] = has ? accIdx : -1;
+        sHas[w] = has;
     }
-    if (t == 0) {
-        *outVal = sHas[0] ? sVal[0] : identityOf<T>(f);
-        if (outIdx) *outIdx = sHas[0] ? sIdx[0] : -1;
+    __syncthreads();
+    if (w == 0) {
+        acc = lane < NW ? sVal[lane] : identityOf<T>(f);
+        accIdx = lane < NW ? sIdx[lane] : -1;
+        has = lane < NW ? sHas[lane] : 0;
+        waveFold<T, BITS>(f, acc, accIdx, has);
+        if (lane == 0) {
+            *outVal = has ? acc : identityOf<T>(f);
+            if (outIdx) *outIdx = has ? accIdx : -1;
+        }
     }
 }
 
@@ -186,12 +225,17 @@ void launchFinish(int f, long n, const void *vals, const void *locs, long locSiz
 }
 
 // ---- several reductions of one element type in one launch
-// Block b belongs to reduction r with first[r] <= b < first[r + 1] and folds
-// partials [c * MULTI_CHUNK, (c + 1) * MULTI_CHUNK) of it (c = b - first[r])
-// into slot c.  The block that arrives last at r's counter folds r's slots in
-// slot order (ties: lower partial index), writes the result and the winner's
-// loc record, and resets the counter for the next launch.
-constexpr long MULTI_CHUNK = 2 * FIN_THREADS;
+// Block b belongs to reduction r with first[r] <= b < first[r + 1].  A
+// reduction of at most MULTI_SINGLE partials gets one block, which folds them
+// all and writes the result.  A larger one gets a block per MULTI_CHUNK
+// partials: block c = b - first[r] folds [c * MULTI_CHUNK, (c + 1) * MULTI_CHUNK)
+// into slot c, and the block that arrives last at r's counter folds r's slots
+// in slot order (ties: lower partial index), writes the result and the
+// winner's loc record, and resets the counter for the next launch.
+constexpr long MULTI_CHUNK = 8 * FIN_THREADS;
+constexpr long MULTI_SINGLE = 16 * FIN_THREADS;
+
+__host__ __device__ inline long multiBlocks(long n) { return n <= MULTI_SINGLE ? 1 : (n + MULTI_CHUNK - 1) / MULTI_CHUNK; }
 
 struct MultiArgs {
     int count;
@@ -216,6 +260,11 @@ __global__ __launch_bounds__(FIN_THREADS) void multiFinishKernel(MultiArgs a) {
     while (r + 1 < a.count && b >= a.first[r + 1]) r++;
     const long c = b - a.first[r];
     const long nb = a.first[r + 1] - a.first[r];
+    if (nb == 1) {
+        foldRange<T, BITS>(a.func[r], 0, a.n[r], (const T *)a.vals[r], nullptr, (T *)a.outVal[r], &win);
+        copyLoc<T>(a.locs[r], a.locSize[r], &win, a.outLoc[r]);
+        return;
+    }
     T *slotVal = (T *)a.slots[r];
     long *slotIdx = (long *)((char *)a.slots[r] + ((nb * (long)sizeof(T) + 15) & ~15L));
     const long lo = c * MULTI_CHUNK;
@@ -250,7 +299,7 @@ void launchMulti(int count, const DvmhMultiRed *reds, unsigned *counters, hipStr
         a.outLoc[r] = (char *)reds[r].outLoc;
         a.slots[r] = reds[r].slots;
         a.first[r] = blocks;
-        blocks += reds[r].n > 0 ? (reds[r].n + MULTI_CHUNK - 1) / MULTI_CHUNK : 1;
+        blocks += multiBlocks(reds[r].n);
     }
     a.first[count] = blocks;
     hipLaunchKernelGGL((multiFinishKernel<T, BITS>), dim3((unsigned)blocks), dim3(FIN_THREADS), 0, s, a);
@@ -292,7 +341,7 @@ extern "C" void dvmh_hip_reduce_segments(int func, int rtType, long groups, cons
 }
 
 extern "C" long dvmh_hip_multi_slot_bytes(long n) {
-    const long nb = n > 0 ? (n + MULTI_CHUNK - 1) / MULTI_CHUNK : 1;
+    const long nb = multiBlocks(n);
     return ((nb * 8 + 15) & ~15L) + nb * 8 + 64;
 }
 

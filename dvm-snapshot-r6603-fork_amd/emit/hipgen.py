@@ -104,6 +104,22 @@ class Loop:
             return False
         return True
 
+    def stencil_range(self, name):
+        """Per loop axis, the (lowest, highest) offset at which the body reads
+        `name` (each in -1..1).  Every index of [lo + lowest, last + highest]
+        on an axis is read by some iteration, so a load clamped into that range
+        stays inside the array."""
+        lo = [None] * self.rank
+        hi = [None] * self.rank
+        for m in re.finditer(r'\b' + name + r'\s*((?:\[[^\[\]]*\]\s*)+)', self.body):
+            idx = re.findall(r'\[([^\[\]]*)\]', m.group(1))
+            for a, ((v, _), e) in enumerate(zip(self.vars, idx)):
+                mm = re.fullmatch(r'\s*' + v + r'\s*(?:([+-])\s*([01])\s*)?', e)
+                d = 0 if not mm.group(1) else (int(mm.group(2)) if mm.group(1) == '+' else -int(mm.group(2)))
+                lo[a] = d if lo[a] is None else min(lo[a], d)
+                hi[a] = d if hi[a] is None else max(hi[a], d)
+        return [(l or 0, h or 0) for l, h in zip(lo, hi)]
+
     def tile_eligible(self):
         """Rank 3, every loop axis in the ACROSS clause with widths <= 1, and
         every access of the ACROSS array in the body of the form
@@ -540,9 +556,10 @@ def march_kernel(L):
         fp.append(f'{ct} *__restrict__ {nm}_base')
         fp += [f'IndexT {nm}_s{k}' for k in range(ar)]
         strides = ', '.join(f'{nm}_s{k}' for k in range(ar))
+        if nm in L.stencil:
+            continue   # read through LDS; plane loads use the base and strides directly
         setup.append(f'const IndexT {nm}_str[{ar}] = {{{strides}}};')
-        vn = f'{nm}_g' if nm in L.stencil else nm
-        setup.append(f'const auto {vn} = dvmh_dev::makeView<{ct}, IndexT, {ar}, true>({nm}_base, {nm}_str);')
+        setup.append(f'const auto {nm} = dvmh_dev::makeView<{ct}, IndexT, {ar}, true>({nm}_base, {nm}_str);')
     for p in L.scalars:
         fp.append(f'{p[2]} *__restrict__ {p[1]}_ptr')
         setup.append(f'{p[2]} &{p[1]} = *{p[1]}_ptr;')
@@ -593,40 +610,67 @@ def march_kernel(L):
     # registers, commit() stores them into an LDS slot.  The next plane's loads
     # are issued before the current plane is computed, so their latency is
     # hidden behind the compute.  Only indices some point of the loop reads
-    # (the loop box plus the halo) are loaded.
+    # (the loop box plus the halo) are loaded.  Which elements a thread owns,
+    # their offsets inside a plane and whether they are loaded do not change
+    # along the march: they are worked out once, and a plane load is a base
+    # pointer plus those offsets.
     b.append('constexpr int NPF = 3 * COLS + 2;')
-    b.append('const IndexT rBase = (IndexT)tid / cols, cBase = (IndexT)tid - rBase * cols;')
-    b.append('const IndexT dR = (IndexT)nthr / cols, dC = (IndexT)nthr - dR * cols;')
+    b.append('unsigned inPlane = 0;   // bit m: element tid + m * nthr lies inside the plane')
     for nm in L.stencil:
-        ct = [p for p in L.arrays if p[1] == nm][0][2]
-        b.append(f'{ct} {nm}_pf[NPF];')
-    lines = ['auto issue = [&](IndexT ip) {',
-             IND + 'const bool okI = ip >= lo0 - 1 && ip <= lo0 + cnt0;',
-             IND + 'IndexT rr = rBase, cc = cBase;',
+        rng = L.stencil_range(nm)
+        b.append(f'// {nm}: indices the body reads, per axis (a load is clamped into them)')
+        for a in range(r):
+            dl, dh = rng[a]
+            b.append(f'const IndexT {nm}_lo{a} = lo{a} + ({dl}), {nm}_hi{a} = lo{a} + cnt{a} - 1 + ({dh});')
+        b.append(f'IndexT {nm}_off[NPF];')
+    lines = ['{',
+             IND + 'IndexT rr = (IndexT)tid / cols, cc = (IndexT)tid - rr * cols;',
+             IND + 'const IndexT dR = (IndexT)nthr / cols, dC = (IndexT)nthr - dR * cols;',
              IND + '#pragma unroll',
              IND + 'for (int m = 0; m < NPF; m++) {',
-             IND * 2 + 'const IndexT kk = kx0 + cc;']
-    conds = ['okI', '(IndexT)tid + (IndexT)m * nthr < plane', f'kk >= lo{ax_x} - 1', f'kk <= lo{ax_x} + cnt{ax_x}']
+             IND * 2 + 'const IndexT kk = kx0 + cc;',
+             IND * 2 + 'inPlane |= ((IndexT)tid + (IndexT)m * nthr < plane ? 1u : 0u) << m;']
     if ax_y is not None:
         lines.append(IND * 2 + 'const IndexT jj = jy0 + rr;')
-        conds += [f'jj >= lo{ax_y} - 1', f'jj <= lo{ax_y} + cnt{ax_y}']
-    lines.append(IND * 2 + f'const bool ok = {" && ".join(conds)};')
     for nm in L.stencil:
-        ct = [p for p in L.arrays if p[1] == nm][0][2]
-        acc = f'{nm}_g[ip][jj][kk]' if ax_y is not None else f'{nm}_g[ip][kk]'
-        lines.append(IND * 2 + f'{nm}_pf[m] = ok ? {acc} : ({ct})0;')
+        lines.append(IND * 2 + f'const IndexT {nm}_k = kk < {nm}_lo{ax_x} ? {nm}_lo{ax_x} : (kk > {nm}_hi{ax_x} ? {nm}_hi{ax_x} : kk);')
+        if ax_y is not None:
+            lines.append(IND * 2 + f'const IndexT {nm}_j = jj < {nm}_lo{ax_y} ? {nm}_lo{ax_y} : (jj > {nm}_hi{ax_y} ? {nm}_hi{ax_y} : jj);')
+            lines.append(IND * 2 + f'{nm}_off[m] = {nm}_j * {nm}_s1 + {nm}_k;')
+        else:
+            lines.append(IND * 2 + f'{nm}_off[m] = {nm}_k;')
     lines += [IND * 2 + 'cc += dC;',
               IND * 2 + 'rr += dR;',
               IND * 2 + 'if (cc >= cols) { cc -= cols; rr++; }',
               IND + '}',
-              '};',
-              'auto commit = [&](int slot) {',
-              IND + '#pragma unroll',
-              IND + 'for (int m = 0; m < NPF; m++) {',
-              IND * 2 + 'const IndexT e = (IndexT)tid + (IndexT)m * nthr;',
-              IND * 2 + 'if (e < plane) {']
+              '}']
     for nm in L.stencil:
-        lines.append(IND * 3 + f'{nm}_lds[(IndexT)slot * plane + e] = {nm}_pf[m];')
+        ct = [p for p in L.arrays if p[1] == nm][0][2]
+        lines.append(f'{ct} {nm}_pf[NPF];')
+    # Elements outside the indices the body reads are loaded from a clamped
+    # (valid) address; no iteration reads their LDS copies.
+    lines += ['auto issue = [&](IndexT ip) {']
+    for nm in L.stencil:
+        ct = [p for p in L.arrays if p[1] == nm][0][2]
+        lines.append(IND + f'const IndexT {nm}_i = ip < {nm}_lo0 ? {nm}_lo0 : (ip > {nm}_hi0 ? {nm}_hi0 : ip);')
+        lines.append(IND + f'const {ct} *{nm}_pl = {nm}_base + {nm}_i * {nm}_s0;')
+    lines += [IND + '#pragma unroll',
+              IND + 'for (int m = 0; m < NPF; m++) {']
+    lines.append(IND * 2 + 'const bool ld = (inPlane >> m) & 1u;')
+    for nm in L.stencil:
+        ct = [p for p in L.arrays if p[1] == nm][0][2]
+        lines.append(IND * 2 + f'{nm}_pf[m] = ld ? {nm}_pl[{nm}_off[m]] : ({ct})0;')
+    lines += [IND + '}',
+              '};',
+              'auto commit = [&](int slot) {']
+    for nm in L.stencil:
+        ct = [p for p in L.arrays if p[1] == nm][0][2]
+        lines.append(IND + f'{ct} *{nm}_dst = {nm}_lds + (IndexT)slot * plane + tid;')
+    lines += [IND + '#pragma unroll',
+              IND + 'for (int m = 0; m < NPF; m++) {',
+              IND * 2 + 'if ((inPlane >> m) & 1u) {']
+    for nm in L.stencil:
+        lines.append(IND * 3 + f'{nm}_dst[(IndexT)m * nthr] = {nm}_pf[m];')
     lines += [IND * 2 + '}', IND + '}', '};']
     b += lines
     b.append('issue(i0 - 1);')
@@ -654,7 +698,7 @@ def march_kernel(L):
     cc = [f't{ax_x} < cnt{ax_x}'] + ([f't{ax_y} < cnt{ax_y}'] if ax_y is not None else [])
     pt.append(f'if ({" && ".join(cc)}) {{')
     for i in range(r):
-        pt.append(IND + var_assign(L, i, f't{i}', f'lo{i}', f'st{i}'))
+        pt.append(IND + var_assign(L, i, f't{i}', f'lo{i}', '1'))   # the variant runs unit steps only
     pt.append(indent('{\n' + indent(L.body + '\n', 1) + '}', 1))
     pt.append('}')
     inner += [IND + ln for ln in '\n'.join(pt).split('\n')]
