@@ -48,7 +48,11 @@ struct DvmhMultiRed {
     void *slots;
 };
 long dvmh_hip_multi_slot_bytes(long n);
-void dvmh_hip_reduce_finish_multi(int rtType, int count, const struct DvmhMultiRed *reds, unsigned *counters, hipStream_t s);
+// counters: DVMH_MULTI_MAX + 1 words, zero between launches.  doneFlag (mapped
+// host memory, may be 0) is set to doneSeq once every result of the launch is
+// written, so the host can wait for the results without a stream sync.
+void dvmh_hip_reduce_finish_multi(int rtType, int count, const struct DvmhMultiRed *reds, unsigned *counters,
+        unsigned long long *doneFlag, unsigned long long doneSeq, hipStream_t s);
 
 // Axis permutation copy.  src is stored with storage-position sizes srcSizes[0..rank-1]
 // (position 0 outermost); dst storage position p holds src storage position

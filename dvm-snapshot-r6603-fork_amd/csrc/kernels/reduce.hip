@@ -249,7 +249,27 @@ struct MultiArgs {
     void *slots[DVMH_MULTI_MAX];
     long first[DVMH_MULTI_MAX + 1];
     unsigned *counters;
+    unsigned long long *doneFlag;   // 0, or a word the launch sets to doneSeq once every result is written
+    unsigned long long doneSeq;
 };
+
+// Called by every thread of a block that has just written reduction r's
+// result.  The block that completes the last of the launch's results
+// publishes doneSeq: every result (host memory) is visible to the host
+// before the word is.
+__device__ inline void multiResultWritten(const MultiArgs &a) {
+    if (!a.doneFlag) return;
+    __threadfence_system();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned before = atomicAdd(&a.counters[DVMH_MULTI_MAX], 1u);
+        if (before == (unsigned)(a.count - 1)) {
+            a.counters[DVMH_MULTI_MAX] = 0u;
+            __threadfence_system();
+            __hip_atomic_store(a.doneFlag, a.doneSeq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+    }
+}
 
 template <typename T, bool BITS>
 __global__ __launch_bounds__(FIN_THREADS) void multiFinishKernel(MultiArgs a) {
@@ -263,6 +283,7 @@ __global__ __launch_bounds__(FIN_THREADS) void multiFinishKernel(MultiArgs a) {
     if (nb == 1) {
         foldRange<T, BITS>(a.func[r], 0, a.n[r], (const T *)a.vals[r], nullptr, (T *)a.outVal[r], &win);
         copyLoc<T>(a.locs[r], a.locSize[r], &win, a.outLoc[r]);
+        multiResultWritten(a);
         return;
     }
     T *slotVal = (T *)a.slots[r];
@@ -281,13 +302,17 @@ __global__ __launch_bounds__(FIN_THREADS) void multiFinishKernel(MultiArgs a) {
     foldRange<T, BITS>(a.func[r], 0, nb, slotVal, slotIdx, (T *)a.outVal[r], &win);
     copyLoc<T>(a.locs[r], a.locSize[r], &win, a.outLoc[r]);
     if (threadIdx.x == 0) a.counters[r] = 0u;
+    multiResultWritten(a);
 }
 
 template <typename T, bool BITS>
-void launchMulti(int count, const DvmhMultiRed *reds, unsigned *counters, hipStream_t s) {
+void launchMulti(int count, const DvmhMultiRed *reds, unsigned *counters, unsigned long long *doneFlag,
+        unsigned long long doneSeq, hipStream_t s) {
     MultiArgs a;
     a.count = count;
     a.counters = counters;
+    a.doneFlag = doneFlag;
+    a.doneSeq = doneSeq;
     long blocks = 0;
     for (int r = 0; r < count; r++) {
         a.func[r] = reds[r].func;
@@ -345,9 +370,10 @@ extern "C" long dvmh_hip_multi_slot_bytes(long n) {
     return ((nb * 8 + 15) & ~15L) + nb * 8 + 64;
 }
 
-extern "C" void dvmh_hip_reduce_finish_multi(int rtType, int count, const DvmhMultiRed *reds, unsigned *counters, hipStream_t s) {
+extern "C" void dvmh_hip_reduce_finish_multi(int rtType, int count, const DvmhMultiRed *reds, unsigned *counters,
+        unsigned long long *doneFlag, unsigned long long doneSeq, hipStream_t s) {
     if (count <= 0 || count > DVMH_MULTI_MAX) return;
-    DISPATCH_TYPES(launchMulti, count, reds, counters, s)
+    DISPATCH_TYPES(launchMulti, count, reds, counters, doneFlag, doneSeq, s)
 }
 
 extern "C" void dvmh_hip_fill(void *dst, const void *value, long valueBytes, long count, hipStream_t s) {

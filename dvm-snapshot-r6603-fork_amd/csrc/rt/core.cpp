@@ -56,6 +56,7 @@ void Settings::load() {
     tuneMode = (int)envLong("DVMH_AMD_TUNE", envLong("DVMH_OPT_PARAMS", 0) ? 1 : 0);
     DVMH_CHECK(tuneMode >= 0 && tuneMode <= 3, "DVMH_AMD_TUNE must be 0..3");
     syncMode = (int)envLong("DVMH_AMD_SYNC", 0);
+    pollDone = envLong("DVMH_AMD_POLL", 1) != 0;
     graph = envLong("DVMH_AMD_GRAPH", 0) != 0;
     tuneRepeats = (int)envLong("DVMH_AMD_TUNE_REPEATS", 1);
     if (tuneRepeats < 1) tuneRepeats = 1;
@@ -97,9 +98,33 @@ void Gpu::init() {
     // launch leaves them zero again
     HIP_CHECK(hipMalloc((void **)&finishCounters, 64 * sizeof(unsigned)));
     HIP_CHECK(hipMemset(finishCounters, 0, 64 * sizeof(unsigned)));
+    void *done = nullptr;
+    HIP_CHECK(hipHostMalloc(&done, 64, hipHostMallocMapped));
+    doneHost = (volatile unsigned long long *)done;
+    *doneHost = 0;
+    HIP_CHECK(hipHostGetDevicePointer((void **)&doneDev, done, 0));
     HIP_CHECK(hipDeviceSynchronize());
     present = true;
     logMsg(LOG_INFO, "GPU %s (%s): %d CUs, wave%d", props.name, props.arch, props.cuCount, props.waveSize);
+}
+
+// Waiting on the completion word skips the runtime's signal wait (a stream
+// sync costs several microseconds more than a poll of host memory).  Every
+// 4096 polls the stream is queried: once it has drained the word must be set.
+void Gpu::waitDone(unsigned long long seq) {
+    for (unsigned long spins = 1;; spins++) {
+        if (__atomic_load_n(doneHost, __ATOMIC_ACQUIRE) >= seq) return;
+        if ((spins & 4095) == 0) {
+            hipError_t q = hipStreamQuery(stream);
+            if (q == hipSuccess) {
+                DVMH_INTERNAL(__atomic_load_n(doneHost, __ATOMIC_ACQUIRE) >= seq,
+                        "reduction finish %llu drained without setting its completion word", seq);
+                return;
+            }
+            if (q != hipErrorNotReady) HIP_CHECK(q);
+        }
+        __builtin_ia32_pause();
+    }
 }
 
 void *Gpu::alloc(size_t bytes) {

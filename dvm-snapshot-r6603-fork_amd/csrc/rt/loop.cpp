@@ -148,9 +148,16 @@ static char *mappedDevicePtr(char *h) {
     return (char *)d;
 }
 
+// Completion word value that covers every pending post, 0 when one of them
+// came from a path without it (batches, graph replays): then the stream is
+// synchronised.
+static unsigned long long pendingDone = 0;
+
 void flushDeferredPosts() {
     if (pending.empty()) return;
-    gpu.sync();
+    if (pendingDone && settings.pollDone) gpu.waitDone(pendingDone);
+    else gpu.sync();
+    pendingDone = 0;
     for (auto &p : pending) {
         p.red->postValues(p.val, p.loc);
         if (p.addInitial) p.red->postValues(p.red->initArr.data(), p.red->isLoc() ? p.red->initLoc.data() : nullptr);
@@ -161,6 +168,7 @@ void flushDeferredPosts() {
 
 void addPendingPost(const std::shared_ptr<Reduction> &r, char *val, char *loc, bool addInitial) {
     pending.push_back(PendingPost{r, val, loc, addInitial});
+    pendingDone = 0;   // until a completion word covers it
 }
 
 // ------------------------------------------------------------ GPU reductions
@@ -237,7 +245,13 @@ void gpuRedFinishFlush(SpecLoop *sl) {
             d[i].slots = sv;
             addPendingPost(r, host, hostLoc);
         }
-        dvmh_hip_reduce_finish_multi(type, (int)group.size(), d, gpu.finishCounters, sl->stream);
+        // the loop's last finish launch publishes a completion word (not in a
+        // captured graph, whose replays would all publish the same value)
+        const bool signal = rest.empty() && !graphCapturing();
+        const unsigned long long seq = signal ? ++gpu.doneSeq : 0;
+        dvmh_hip_reduce_finish_multi(type, (int)group.size(), d, gpu.finishCounters, signal ? gpu.doneDev : nullptr, seq,
+                sl->stream);
+        if (signal) pendingDone = seq;
         for (void *p : slots) gpu.release(p);
         for (auto &r : group) {
             gpu.release(r->gpuArr);

@@ -52,6 +52,7 @@ struct Settings {
     int syncMode = 0;           // DVMH_AMD_SYNC: 0 synchronise after every loop, 1 only when results are needed
     bool graph = false;         // DVMH_AMD_GRAPH: capture repeating regions into HIP graphs
     int tuneRepeats = 1;        // DVMH_AMD_TUNE_REPEATS: timed executions per candidate
+    bool pollDone = true;          // DVMH_AMD_POLL: wait for reduction results on the completion word, not a stream sync
     bool tuneTransforms = false;   // DVMH_AMD_TUNE_TRANSFORMS: candidates may re-lay out arrays of ACROSS loops
     std::string tuneDb;         // DVMH_AMD_TUNEDB: file of recorded timings
     std::string modelFile;      // DVMH_AMD_MODEL: trained cost-model weights
@@ -74,6 +75,12 @@ public:
     DeviceProps props;
     hipStream_t stream = nullptr;
     unsigned *finishCounters = nullptr;   // arrival counters of the multi-reduction finish (kept zero between launches)
+    // completion word in mapped pinned host memory: a reduction finish sets it
+    // to its sequence number once its results are written
+    volatile unsigned long long *doneHost = nullptr;
+    unsigned long long *doneDev = nullptr;
+    unsigned long long doneSeq = 0;
+    void waitDone(unsigned long long seq);   // until the word reaches seq (or the stream drains)
     void init();
     void *alloc(size_t bytes);          // size-keyed cache in front of hipMalloc
     void release(void *p);              // back to the cache

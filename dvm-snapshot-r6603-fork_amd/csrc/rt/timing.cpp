@@ -12,7 +12,9 @@ struct SiteTiming {
     double seconds = 0;
 };
 static bool timingOn = false;
+static long timingPeriod = 1;   // time one launch in this many, per site
 static std::map<std::string, SiteTiming> siteTimes;
+static std::map<std::string, long> siteMarks;   // phase-0 marks seen per site while timing
 // events recorded but not yet read back: (site, start, stop)
 struct PendingEvents {
     std::string site;
@@ -66,6 +68,7 @@ extern "C" void dvmh_amd_mark_kernel_C(DvmType curLoop, DvmType phase) {
     HIP_CHECK(hipStreamIsCapturing(sl->stream, &cs));
     if (cs != hipStreamCaptureStatusNone) return;   // events are not recorded into graphs
     if (phase == 0) {
+        if (timingPeriod > 1 && siteMarks[siteOf(sl->loop)]++ % timingPeriod != 0) return;   // not sampled
         openStart = takeEvent();
         HIP_CHECK(hipEventRecord(openStart, sl->stream));
     } else if (openStart) {
@@ -81,6 +84,8 @@ extern "C" void dvmh_amd_set_kernel_timing_C(DvmType enable) {
     runtimeInit();
     if (!enable) drainKernelTimes();
     timingOn = enable != 0;
+    timingPeriod = enable > 1 ? (long)enable : 1;
+    siteMarks.clear();
 }
 
 extern "C" DvmType dvmh_amd_kernel_stats_C(const char *file, DvmType line, DvmType out[2]) {

@@ -42,7 +42,9 @@ void dvmh_amd_loop_get_tuning_C(DvmType curLoop, DvmType kind, DvmType tune[]);
  * Emitted handlers bracket their main kernel launches with phase 0 / phase 1
  * marks.  With timing enabled the runtime records HIP events on the loop's
  * stream at the marks and accumulates, per loop site (file, line), the number
- * of bracketed launches and their total duration. */
+ * of bracketed launches and their total duration.  enable = 1 times every
+ * launch; enable = k > 1 times the first of every k launches of a site (the
+ * events cost host time and a few microseconds of stream time each). */
 void dvmh_amd_mark_kernel_C(DvmType curLoop, DvmType phase);
 void dvmh_amd_set_kernel_timing_C(DvmType enable);
 /* out[0] launches, out[1] total nanoseconds; returns 0 if the site is unknown. */
