@@ -540,6 +540,13 @@ def parallel_kernel(L):
     return '\n'.join(k) + '\n'
 
 
+# Register slots per thread for plane loads (NPF) instantiated for each COLS:
+# a launch needs ceil(plane / threads) slots, between COLS + 1 (rank 2, or a
+# tall tile) and 3 * COLS + 2 (rank 3, one row of threads), and takes the
+# smallest instantiated value that covers it.
+MARCH_NPF = {1: [2, 3, 4, 5], 2: [3, 4, 5, 7, 8], 3: [4, 5, 7, 10, 11], 4: [5, 7, 10, 14]}
+
+
 def march_kernel(L):
     """LDS march variant of a PARALLEL loop with stencil arrays (rank 2 or 3,
     unit steps).  A workgroup owns a tile of the inner axes -- blockDim.x * COLS
@@ -567,7 +574,7 @@ def march_kernel(L):
         fp += [f'IndexT lo{i}', f'IndexT cnt{i}', f'IndexT st{i}']
     fp += ['IndexT nbx', 'IndexT nby', 'IndexT nbz', 'long nblocks', 'int xcd', 'IndexT march']
     fp += red_kernel_params(L)
-    k = ['template <typename IndexT, int COLS>',
+    k = ['template <typename IndexT, int COLS, int NPF>',
          f'__global__ __launch_bounds__(1024) void {L.name}_march_kernel({", ".join(fp)}) {{']
     b = ['extern __shared__ __align__(16) unsigned char dvmh_lds[];']
     b += setup
@@ -605,8 +612,9 @@ def march_kernel(L):
     b.append('const int tid = (int)(threadIdx.x + blockDim.x * threadIdx.y);')
     b.append('const int nthr = (int)(blockDim.x * blockDim.y);')
     # Plane loads are software-pipelined through registers: each thread owns
-    # plane elements e = tid + m * nthr (m < NPF, NPF = 3 * COLS + 2 covers
-    # every shape the handler accepts); issue() loads a plane from HBM into
+    # plane elements e = tid + m * nthr (m < NPF; the handler instantiates NPF
+    # just above ceil(plane / threads), so no registers are held for elements
+    # a shape never has); issue() loads a plane from HBM into
     # registers, commit() stores them into an LDS slot.  The next plane's loads
     # are issued before the current plane is computed, so their latency is
     # hidden behind the compute.  Only indices some point of the loop reads
@@ -614,7 +622,6 @@ def march_kernel(L):
     # their offsets inside a plane and whether they are loaded do not change
     # along the march: they are worked out once, and a plane load is a base
     # pointer plus those offsets.
-    b.append('constexpr int NPF = 3 * COLS + 2;')
     b.append('unsigned inPlane = 0;   // bit m: element tid + m * nthr lies inside the plane')
     for nm in L.stencil:
         rng = L.stencil_range(nm)
@@ -730,6 +737,7 @@ def parallel_handler(L):
         b.append('int mcols = (int)tune[DVMH_AMD_T_TILE_A];')
         b.append('DvmType march = tune[DVMH_AMD_T_CHUNK];')
         b.append('size_t ldsBytes = 0;')
+        b.append('int npf = 0;')
         unit = ' && '.join(f'loopSteps[{i}] == 1' for i in range(r))
         shape_ok = 'threads.z == 1' + (' && threads.y == 1' if r == 2 else '')
         b.append(f'if (variant == 1 && ({unit}) && {shape_ok} && mcols >= 1 && mcols <= 4 && march >= 1) {{')
@@ -740,7 +748,11 @@ def parallel_handler(L):
             b.append(IND + f'ldsBytes += (3 * mplane * sizeof({ct}) + 15) & ~(size_t)15;')
         b.append(IND + 'if (ldsBytes > 64512) variant = 0;   // leave room for the reductions\' static LDS')
         b.append(IND + 'size_t mthreads = (size_t)threads.x * threads.y;')
-        b.append(IND + 'if ((mplane + mthreads - 1) / mthreads > (size_t)(3 * mcols + 2)) variant = 0;   // register share of a plane')
+        b.append(IND + '// register share of a plane: the smallest instantiated NPF that covers it')
+        b.append(IND + 'size_t need = (mplane + mthreads - 1) / mthreads;')
+        for cols, sets in MARCH_NPF.items():
+            b.append(IND + f'if (mcols == {cols}) {{ ' + ' '.join(f'if (npf == 0 && need <= {v}) npf = {v};' for v in sets) + ' }')
+        b.append(IND + 'if (npf == 0 || kernelIndexT != rt_INT) variant = 0;   // 64-bit indices: the direct kernel')
         b.append('} else {')
         b.append(IND + 'variant = 0;')
         b.append('}')
@@ -778,15 +790,15 @@ def parallel_handler(L):
     b.append(IND + 'dvmh_amd_mark_kernel_C(loop_ref, 0);')
     if L.stencil:
         b.append(IND + 'if (variant == 1) {')
-        for it in ['int', 'long long']:
-            args = array_fact_args(L, it)
-            for i in range(r):
-                args += [f'({it})boundsLow[{i}]', f'({it})cnt{i}', f'({it})loopSteps[{i}]']
-            args += [f'({it})nbx', f'({it})nby', f'({it})nbz', 'nblocks', '(int)tune[DVMH_AMD_T_XCD]', f'({it})march']
-            args += red_fact_args(L)
-            for cols in (1, 2, 3, 4):
-                cond = f'kernelIndexT == {"rt_INT" if it == "int" else "rt_LLONG"} && mcols == {cols}'
-                b.append(IND * 2 + f'if ({cond}) hipLaunchKernelGGL(({L.name}_march_kernel<{it}, {cols}>), '
+        it = 'int'
+        args = array_fact_args(L, it)
+        for i in range(r):
+            args += [f'({it})boundsLow[{i}]', f'({it})cnt{i}', f'({it})loopSteps[{i}]']
+        args += [f'({it})nbx', f'({it})nby', f'({it})nbz', 'nblocks', '(int)tune[DVMH_AMD_T_XCD]', f'({it})march']
+        args += red_fact_args(L)
+        for cols, sets in MARCH_NPF.items():
+            for v in sets:
+                b.append(IND * 2 + f'if (mcols == {cols} && npf == {v}) hipLaunchKernelGGL(({L.name}_march_kernel<{it}, {cols}, {v}>), '
                          f'dim3((unsigned)nblocks), threads, ldsBytes, stream, {", ".join(args)});')
         b.append(IND + '} else {')
         ind2 = IND * 2
