@@ -263,7 +263,18 @@ def cpu_baseline(target_seconds=12.0):
     sec = time.perf_counter() - t0
     return {'value': PTS * its / sec, 'unit': 'point-updates/s', 'cores': cores, 'kind': 'port',
             'sample': f'{its} iterations of the 384^3 sweep (oracle C/OpenMP host-handler restatement, '
-                      f'{sec:.1f} s on {cores} threads)'}
+                      f'{sec:.1f} s on {cores} threads)', 'cpu_model': cpu_model()}
+
+
+def cpu_model():
+    try:
+        with open('/proc/cpuinfo') as f:
+            for line in f:
+                if line.startswith('model name'):
+                    return line.split(':', 1)[1].strip()
+    except OSError:
+        pass
+    return 'unknown'
 
 
 def measured_traffic():
