@@ -1,0 +1,230 @@
+"""The C-DVMH front end (emit/cdvmh.py).
+
+- Its loop descriptions of the configuration programs equal the hand-written
+  ones (programs/*.loops, written from cdvmh's handler shape): loop variables,
+  steps, parameters, reductions, ACROSS widths, shadow_renew arrays, bodies.
+- The programs it converts from the .cdv sources (lib/libcdv_<prog>.so, the
+  source's own functions) give the oracle's results: on the host path here,
+  on the GPU in the `gpu` tests (bit-exact grids and MAX/MIN/MAXLOC histories,
+  SUM within 1e-12 relative).
+- relax2d.cdv (ACROSS on two axes, private, MAX + MIN, cuda_block, get_actual
+  of a sub-array, host element reads) against its sequential semantics, the
+  lexicographic order an ACROSS loop preserves.
+- Constructs outside the subset stop the conversion with the line.
+"""
+import ast
+import ctypes
+import os
+import re
+import sys
+
+import numpy as np
+import pytest
+
+import dvmh_amd as D
+import helpers as H
+
+PKG = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), 'dvm-snapshot-r6603-fork_amd')
+sys.path.insert(0, os.path.join(PKG, 'emit'))
+import cdvmh  # noqa: E402
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_lp = ctypes.POINTER(ctypes.c_long)
+
+
+def _norm(body):
+    return re.sub(r'\s+', ' ', body).strip()
+
+
+# ---------------------------------------------------------------- descriptions
+@pytest.mark.parametrize('prog', ['jac2d', 'jac3d', 'sor3d', 'adi3d'])
+def test_descriptions_equal_hand_written(prog, tmp_path):
+    cdvmh.convert_file(os.path.join(PKG, 'programs', prog + '.cdv'), str(tmp_path), prog=prog)
+    got = ast.literal_eval(open(tmp_path / f'{prog}.loops').read().split('\n', 1)[1])
+    want = ast.literal_eval(open(os.path.join(PKG, 'programs', prog + '.loops')).read())
+    assert got['include'] == want['include']
+    # loops in program order (the hand-written names of jac2d carry the pragma
+    # line numbers of an earlier revision of the source)
+    assert len(got['loops']) == len(want['loops'])
+    for g, w in zip(got['loops'], want['loops']):
+        assert g['vars'] == w['vars']
+        assert g['steps'] == w.get('steps', ['1'] * len(w['vars']))
+        assert sorted(g['params']) == sorted(w['params'])
+        assert g.get('reductions', []) == w.get('reductions', [])
+        assert g.get('across') == w.get('across')
+        assert sorted(g.get('stencil', [])) == sorted(w.get('stencil', []))
+        assert _norm(g['body']) == _norm(w['body'])
+
+
+def test_converted_text_calls(tmp_path):
+    """The converted relax2d: declaration, allocation, distribution, region
+    registrations with inferred intents, loop registration, sub-array
+    get_actual, element reads, release."""
+    cdvmh.convert_file(os.path.join(PKG, 'programs', 'relax2d.cdv'), str(tmp_path), prog='relax2d_cdv')
+    c = open(tmp_path / 'relax2d_cdv.c').read()
+    for call in ['dvmh_array_declare_C(A, 2, -rt_DOUBLE, (DvmType)(0), (DvmType)1, (DvmType)1, (DvmType)(n)',
+                 'dvmh_array_alloc_C(A, n * n * sizeof(double));',
+                 'dvmh_distribute_C(A, 2, dvmh_distribution_block_C(1), dvmh_distribution_block_C(2));',
+                 'dvmh_region_register_array_C(dvmh_region_1, INTENT_OUT, A, "A");',
+                 'dvmh_region_register_array_C(dvmh_region_2, INTENT_INOUT, A, "A");',
+                 'dvmh_region_register_scalar_C(dvmh_region_2, INTENT_IN, &w, sizeof(w), "w");',
+                 'dvmh_loop_set_cuda_block_C(dvmh_loop_1, 64, 2, 1);',
+                 'dvmh_loop_across_C(dvmh_loop_2, 0, A, 2, (DvmType)1, (DvmType)1, (DvmType)1, (DvmType)1);',
+                 'dvmh_loop_reduction_C(dvmh_loop_2, rf_MIN, &mn, rt_DOUBLE, 1, 0, 0);',
+                 'dvmh_get_actual_subarray_C(A, 2, (DvmType)(1), (DvmType)(n - 2), (DvmType)(0), (DvmType)(n - 1));',
+                 '(*(double *)dvmh_get_element_addr_C(A, 2, (DvmType)(i), (DvmType)(j)))',
+                 'dvmh_array_free_C(A);', 'dvmh_forget_header_(A);']:
+        assert call in c, call
+    spec = ast.literal_eval(open(tmp_path / 'relax2d_cdv.loops').read().split('\n', 1)[1])
+    sweep = spec['loops'][1]
+    assert sweep['privates'] == ['double t'] and sweep['across']['widths'] == [(1, 1), (1, 1)]
+    assert [p[1] for p in sweep['params']] == ['A', 'w']
+
+
+BAD = [
+    ('nested regions', '#pragma dvm region\n{\n#pragma dvm region\n{\n}\n}\n', 'nested regions'),
+    ('imperfect nest', '#pragma dvm region\n{\n#pragma dvm parallel([i][j] on A[i][j])\nfor (i = 0; i < n; i++) {\n'
+     'x = 1;\nfor (j = 0; j < n; j++) A[i][j] = 0;\n}\n}\n', 'perfect'),
+    ('unknown clause', '#pragma dvm region\n{\n#pragma dvm parallel([i][j] on A[i][j]) remote_access(A[i][j])\n'
+     'for (i = 0; i < n; i++) for (j = 0; j < n; j++) A[i][j] = 0;\n}\n', 'remote_access'),
+    ('non-unit subscript', '#pragma dvm region\n{\n#pragma dvm parallel([i][j] on A[2 * i][j])\n'
+     'for (i = 0; i < n; i++) for (j = 0; j < n; j++) A[i][j] = 0;\n}\n', 'subscript'),
+]
+
+
+@pytest.mark.parametrize('what,body,msg', BAD, ids=[b[0] for b in BAD])
+def test_unsupported_constructs_name_the_line(what, body, msg):
+    src = ('void f(long n)\n{\n    int i, j, x;\n    #pragma dvm array distribute[block][block]\n    double (*A)[n];\n'
+           '    A = malloc(n * n * sizeof(double));\n' + body + '}\n')
+    with pytest.raises(cdvmh.ConvertError) as e:
+        cv = cdvmh.Converter(src, 't.cdv', 't')
+        cv.convert()
+    assert 't.cdv:' in str(e.value) and msg in str(e.value)
+
+
+# ---------------------------------------------------------------- converted programs
+def _lib(prog):
+    return D.load(f'libcdv_{prog}.so')
+
+
+def run_cdv(prog, n, iters):
+    lib = _lib(prog)
+    if prog == 'jac2d':
+        f = lib.jac2d
+        f.argtypes = [ctypes.c_long, ctypes.c_int, ctypes.c_ulonglong, _dp]
+        eps = np.zeros(iters)
+        f(n, iters, D.SEED_BASE + 1, H.dp(eps))
+        return {'eps': eps}
+    if prog == 'jac3d':
+        f = lib.jac3d
+        f.argtypes = [ctypes.c_long, ctypes.c_int, ctypes.c_ulonglong, _dp, _dp]
+        eps, s = np.zeros(iters), np.zeros(iters)
+        f(n, iters, D.SEED_BASE + 2, H.dp(eps), H.dp(s))
+        return {'eps': eps, 'sum': s}
+    if prog == 'sor3d':
+        f = lib.sor3d
+        f.argtypes = [ctypes.c_long, ctypes.c_int, ctypes.c_double, ctypes.c_ulonglong, _dp]
+        eps = np.zeros(iters)
+        f(n, iters, 1.5, D.SEED_BASE + 3, H.dp(eps))
+        return {'eps': eps}
+    f = lib.adi3d
+    f.argtypes = [ctypes.c_long, ctypes.c_int, ctypes.c_double, ctypes.c_ulonglong, _dp, _lp]
+    mx, loc = np.zeros(iters), np.zeros(iters, dtype=np.int64)
+    f(n, iters, 0.3, D.SEED_BASE + 4, H.dp(mx), loc.ctypes.data_as(_lp))
+    return {'max': mx, 'loc': loc}
+
+
+def check_against_oracle(prog, n, iters):
+    got = run_cdv(prog, n, iters)
+    if prog == 'jac2d':
+        assert np.array_equal(got['eps'], H.oracle_jac2d(n, iters)['eps'])
+    elif prog == 'jac3d':
+        ref = H.oracle_jac3d(n, iters)
+        assert np.array_equal(got['eps'], ref['eps']) and H.sum_close(got['sum'], ref['sum'])
+    elif prog == 'sor3d':
+        assert np.array_equal(got['eps'], H.oracle_sor3d(n, iters)['eps'])
+    else:
+        ref = H.oracle_adi3d(n, iters)
+        assert np.array_equal(got['max'], ref['max']) and np.array_equal(got['loc'], ref['loc'])
+
+
+HOST_CASES = [('jac2d', 20, 3), ('jac3d', 12, 3), ('sor3d', 9, 2), ('adi3d', 8, 2)]
+
+
+@pytest.mark.parametrize('prog,n,iters', HOST_CASES)
+def test_converted_program_host_path(prog, n, iters):
+    if D.stats()['gpu_present']:
+        pytest.skip('the host path runs where no GPU is present; the GPU tests cover this case')
+    check_against_oracle(prog, n, iters)
+
+
+def _gpu_loops():
+    return D.stats()['loops_gpu']
+
+
+GPU_CASES = [('jac2d', 5, 3), ('jac2d', 70, 4), ('jac2d', 1000, 3), ('jac3d', 10, 3), ('jac3d', 66, 4),
+             ('jac3d', 200, 3), ('sor3d', 5, 2), ('sor3d', 23, 3), ('sor3d', 70, 2), ('adi3d', 6, 2), ('adi3d', 33, 2),
+             ('adi3d', 64, 3)]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('prog,n,iters', GPU_CASES)
+def test_converted_program_gpu(prog, n, iters):
+    before = _gpu_loops()
+    check_against_oracle(prog, n, iters)
+    assert _gpu_loops() > before
+
+
+# ---------------------------------------------------------------- relax2d
+def relax2d_reference(n, iters, w, seed):
+    """Sequential execution of relax2d.cdv: lexicographic Gauss-Seidel, the
+    order the ACROSS loop preserves."""
+    A = np.zeros(n * n)
+    H.oracle().orc_init_grid2d(H.vp(A), n, seed, 1.0)
+    a = A.reshape(n, n).tolist()
+    res_h, min_h = [], []
+    for _ in range(iters):
+        res, mn = 0.0, 1e300
+        for i in range(1, n - 1):
+            for j in range(1, n - 1):
+                t = 0.25 * (a[i - 1][j] + a[i + 1][j] + a[i][j - 1] + a[i][j + 1])
+                t = a[i][j] + w * (t - a[i][j])
+                d = abs(t - a[i][j])
+                res = res if res > d else d
+                mn = mn if mn < t else t
+                a[i][j] = t
+        res_h.append(res)
+        min_h.append(mn)
+    return np.array(res_h), np.array(min_h), np.array(a[1:n - 1]).ravel()
+
+
+def run_relax2d(n, iters, w, seed):
+    f = _lib('relax2d').relax2d
+    f.argtypes = [ctypes.c_long, ctypes.c_int, ctypes.c_double, ctypes.c_ulonglong, _dp, _dp, _dp]
+    res, mn, out = np.zeros(iters), np.zeros(iters), np.zeros((n - 2) * n)
+    f(n, iters, w, seed, H.dp(res), H.dp(mn), H.dp(out))
+    return res, mn, out
+
+
+def _check_relax2d(n, iters, w=1.25, seed=D.SEED_BASE + 7):
+    res, mn, out = run_relax2d(n, iters, w, seed)
+    rr, rm, ro = relax2d_reference(n, iters, w, seed)
+    assert np.array_equal(res, rr) and np.array_equal(mn, rm)
+    assert H.bits_equal(out, ro)
+
+
+@pytest.mark.parametrize('n,iters', [(3, 2), (16, 3)])
+def test_relax2d_host_path(n, iters):
+    if D.stats()['gpu_present']:
+        pytest.skip('the host path runs where no GPU is present; the GPU tests cover this case')
+    _check_relax2d(n, iters)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('n,iters', [(3, 2), (4, 2), (17, 3), (65, 2), (130, 2)])
+def test_relax2d_gpu(n, iters):
+    """ACROSS on both axes of a 2-D loop: one hyperplane launch per
+    anti-diagonal, MAX and MIN in one loop, and the sub-array fetch."""
+    before = _gpu_loops()
+    _check_relax2d(n, iters)
+    assert _gpu_loops() > before
