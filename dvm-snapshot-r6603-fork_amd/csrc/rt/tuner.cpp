@@ -41,7 +41,7 @@ static void addCand(std::vector<Candidate> &v, int bx, int by, int bz, std::init
 }
 
 // Candidate lists.  Index 0 is always the default configuration.
-std::vector<Candidate> makeCandidates(int kind, int loopRank, int depCount, const long extents[]) {
+std::vector<Candidate> makeCandidates(int kind, int loopRank, int depCount, const long extents[], long depMask) {
     std::vector<Candidate> v;
     int mapped = loopRank - (kind == DVMH_AMD_KIND_PARALLEL ? 0 : depCount);   // axes spread over threads
     const int tfMax = settings.tuneTransforms ? 1 : 0;   // also try device layout transforms
@@ -105,6 +105,18 @@ std::vector<Candidate> makeCandidates(int kind, int loopRank, int depCount, cons
                     for (int tf = 0; tf <= tfMax; tf++)
                         addCand(v, bx, by, 1, {{DVMH_AMD_T_XCD, xcd}, {DVMH_AMD_T_TRANSFORM, tf}});
             }
+        if (depMask == 1 && (loopRank == 2 || loopRank == 3)) {
+            // tiled line sweep (dependent axis innermost): blockDim.x lines per
+            // workgroup, TILE_A iterations per staged chunk; LDS of two chunks
+            // of lines x ((chunk + 2) | 1) FP64 slots plus a row offset per line
+            const int tls[] = {64, 128, 256};
+            const int cks[] = {8, 16, 32};
+            for (int tl : tls)
+                for (int ck : cks) {
+                    if ((long)tl * (2 * ((ck + 2) | 1) * 8 + 8) + 16 > 64512) continue;
+                    addCand(v, tl, 1, 1, {{DVMH_AMD_T_VARIANT, 1}, {DVMH_AMD_T_TILE_A, ck}});
+                }
+        }
     } else {
         // one launch per hyperplane: threads over the free dependent axes (y only
         // with three or more dependent axes), optional diagonal layout
@@ -156,11 +168,12 @@ static SiteInfo &siteFor(SpecLoop *sl, int kind) {
     s.kind = kind;
     s.rank = sl->loop->rank;
     s.depCount = __builtin_popcountl((unsigned long)sl->loop->depMask);
+    s.depMask = sl->loop->depMask;
     for (int i = 0; i < sl->loop->rank && i < 8; i++) {
         Loop *l = sl->loop;
         s.extents[i] = l->step[i] > 0 ? (l->hi[i] - l->lo[i]) / l->step[i] + 1 : (l->lo[i] - l->hi[i]) / (-l->step[i]) + 1;
     }
-    s.cands = makeCandidates(kind, s.rank, s.depCount, s.extents);
+    s.cands = makeCandidates(kind, s.rank, s.depCount, s.extents, s.depMask);
     s.times.assign(s.cands.size(), std::vector<double>());
     return s;
 }
@@ -218,7 +231,8 @@ void tunerPick(SpecLoop *sl, long kind, TuneChoice &out) {
     SiteInfo *s = nullptr;
     if (l->userBlockFlag) {
         // a user cuda_block wins over everything (rts/dvmh/loop.cpp:2141-2147)
-        std::vector<Candidate> c = makeCandidates((int)kind, l->rank, __builtin_popcountl((unsigned long)l->depMask), nullptr);
+        std::vector<Candidate> c = makeCandidates((int)kind, l->rank, __builtin_popcountl((unsigned long)l->depMask), nullptr,
+                l->depMask);
         Candidate d = c[0];
         for (int i = 0; i < 3; i++) out.block[i] = l->userBlock[i];
         for (int i = 0; i < DVMH_AMD_TUNE_N; i++) out.tune[i] = d.tune[i];
@@ -326,6 +340,7 @@ static bool parseKey(const std::string &key, SiteInfo &s) {
     s.rank = atoi(key.c_str() + r + 2);
     long mask = atol(key.c_str() + m + 2);
     s.depCount = __builtin_popcountl((unsigned long)mask);
+    s.depMask = mask;
     std::string ex = key.substr(e + 2);
     std::stringstream ss(ex);
     std::string part;
@@ -350,7 +365,7 @@ long tunerLoad(const std::string &path) {
             SiteInfo s;
             s.key = key;
             if (!parseKey(key, s)) continue;
-            s.cands = makeCandidates(s.kind, s.rank, s.depCount, s.extents);
+            s.cands = makeCandidates(s.kind, s.rank, s.depCount, s.extents, s.depMask);
             s.times.assign(s.cands.size(), std::vector<double>());
             it = sites.emplace(key, s).first;
         }

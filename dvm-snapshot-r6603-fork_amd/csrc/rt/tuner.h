@@ -22,6 +22,7 @@ struct Candidate {
 struct SiteInfo {
     std::string key;
     int kind = 0, rank = 0, depCount = 0;
+    long depMask = 0;
     long extents[8] = {0};
     std::vector<Candidate> cands;
     std::vector<std::vector<double>> times;   // seconds per measured execution
@@ -34,7 +35,8 @@ struct SiteInfo {
     long pending() const;                // candidates still to time in the current mode
 };
 
-std::vector<Candidate> makeCandidates(int kind, int loopRank, int depCount, const long extents[]);
+// depMask: the loop's dependency mask (bit rank-1-a for axis a), -1 if unknown
+std::vector<Candidate> makeCandidates(int kind, int loopRank, int depCount, const long extents[], long depMask = -1);
 std::map<std::string, SiteInfo> &tunerSites();
 long tunerSave(const std::string &path);
 long tunerLoad(const std::string &path);

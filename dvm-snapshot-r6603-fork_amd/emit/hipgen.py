@@ -147,6 +147,35 @@ class Loop:
                 return False
         return True
 
+    def line_tile_eligible(self):
+        """Rank 2 or 3, and every access of the ACROSS array in the body is
+        X[i][j][k + d] with the loop's own variables, offset 0 on all axes but
+        the innermost, |d| <= 1 there, and d = 0 where it is written: a line
+        along the innermost axis can then be staged in LDS and walked by one
+        thread (the tiled line sweep)."""
+        if not self.across or self.rank not in (2, 3):
+            return False
+        name = self.across['array']
+        arr = [p for p in self.arrays if p[1] == name]
+        if not arr or arr[0][3] != self.rank:
+            return False
+        if re.search(r'\b' + name + r'\b(?!\s*\[)', self.body):
+            return False
+        if re.search(r'(\+\+|--)\s*' + name + r'\b', self.body):
+            return False
+        for m in re.finditer(r'\b' + name + r'\s*((?:\[[^\[\]]*\]\s*)+)(\s*(?:=(?!=)|\+=|-=|\*=|/=|\+\+|--))?', self.body):
+            idx = re.findall(r'\[([^\[\]]*)\]', m.group(1))
+            if len(idx) != self.rank:
+                return False
+            for a, ((v, _), e) in enumerate(zip(self.vars, idx)):
+                mm = re.fullmatch(r'\s*' + v + r'\s*(?:([+-])\s*([01])\s*)?', e)
+                if not mm:
+                    return False
+                moved = mm.group(2) == '1'
+                if moved and (a != self.rank - 1 or m.group(2)):
+                    return False
+        return True
+
     # static dependency masks the ACROSS clause can produce (identity correspondence)
     def dep_axes(self):
         if not self.across:
@@ -1216,6 +1245,7 @@ def tile_kernel(L):
     results are those of the lexicographic sweep."""
     X = L.across['array']
     xct = [p for p in L.arrays if p[1] == X][0][2]
+    rng = L.stencil_range(X)   # per axis, the offsets at which the body reads X
     fp, setup = [], []
     for p in L.arrays:
         _, nm, ct, ar = p
@@ -1230,7 +1260,7 @@ def tile_kernel(L):
         setup.append(f'{p[2]} &{p[1]} = *{p[1]}_ptr;')
     for i in range(3):
         fp += [f'IndexT lo{i}', f'IndexT cnt{i}', f'IndexT st{i}']
-    fp += ['IndexT TA', 'IndexT TC', 'IndexT ntB', 'IndexT ntC', 'IndexT aLo', 'IndexT tplane']
+    fp += ['IndexT dt_TA', 'IndexT dt_TC', 'IndexT dt_ntB', 'IndexT dt_ntC', 'IndexT dt_aLo', 'IndexT dt_tplane']
     fp += red_kernel_params(L)
     k = ['template <typename IndexT>',
          f'__global__ __launch_bounds__(1024) void {L.name}_tile_kernel({", ".join(fp)}) {{']
@@ -1241,37 +1271,38 @@ def tile_kernel(L):
     for pr in L.privates:
         b.append(pr + ';')
     b += red_kernel_locals(L)
-    b += ['const IndexT ta = aLo + (IndexT)blockIdx.x / ntB;',
-          'const IndexT tb = (IndexT)blockIdx.x - ((IndexT)blockIdx.x / ntB) * ntB;',
-          'const IndexT tc = tplane - ta - tb;',
-          'if (tc < 0 || tc >= ntC) return;   // not on this tile-plane (uniform per workgroup)',
-          'const IndexT a0 = ta * TA, b0 = tb * TA, c0 = tc * TC;',
-          'const IndexT na = cnt0 - a0 < TA ? cnt0 - a0 : TA;',
-          'const IndexT nb = cnt1 - b0 < TA ? cnt1 - b0 : TA;',
-          'const IndexT nc = cnt2 - c0 < TC ? cnt2 - c0 : TC;',
-          'const IndexT R1 = nb + 2, R2 = nc + 2, R12 = R1 * R2;',
-          'const IndexT elems = (na + 2) * R12;',
-          f'{xct} *tile = ({xct} *)dvmh_lds;',
-          'const IndexT oi = lo0 + a0 - 1, oj = lo1 + b0 - 1, ok = lo2 + c0 - 1;   // array index of tile (0, 0, 0)',
-          'const IndexT tid = (IndexT)threadIdx.x, nthr = (IndexT)blockDim.x;',
-          '// stage the tile and its halo (only indices the loop can read: loop box plus one)',
-          'for (IndexT e = tid; e < elems; e += nthr) {',
-          IND + 'const IndexT p = e / R12, rem = e - (e / R12) * R12;',
-          IND + 'const IndexT q = rem / R2, r = rem - (rem / R2) * R2;',
-          IND + 'const IndexT ii = oi + p, jj = oj + q, kk = ok + r;',
-          IND + 'if (ii >= lo0 - 1 && ii <= lo0 + cnt0 && jj >= lo1 - 1 && jj <= lo1 + cnt1 && kk >= lo2 - 1 && kk <= lo2 + cnt2)',
-          IND * 2 + f'tile[e] = {X}_g[ii][jj][kk];',
+    b += ['const IndexT dt_ta = dt_aLo + (IndexT)blockIdx.x / dt_ntB;',
+          'const IndexT dt_tb = (IndexT)blockIdx.x - ((IndexT)blockIdx.x / dt_ntB) * dt_ntB;',
+          'const IndexT dt_tc = dt_tplane - dt_ta - dt_tb;',
+          'if (dt_tc < 0 || dt_tc >= dt_ntC) return;   // not on this tile-plane (uniform per workgroup)',
+          'const IndexT dt_a0 = dt_ta * dt_TA, dt_b0 = dt_tb * dt_TA, dt_c0 = dt_tc * dt_TC;',
+          'const IndexT dt_na = cnt0 - dt_a0 < dt_TA ? cnt0 - dt_a0 : dt_TA;',
+          'const IndexT dt_nb = cnt1 - dt_b0 < dt_TA ? cnt1 - dt_b0 : dt_TA;',
+          'const IndexT dt_nc = cnt2 - dt_c0 < dt_TC ? cnt2 - dt_c0 : dt_TC;',
+          'const IndexT dt_R1 = dt_nb + 2, dt_R2 = dt_nc + 2, dt_R12 = dt_R1 * dt_R2;',
+          'const IndexT dt_elems = (dt_na + 2) * dt_R12;',
+          f'{xct} *dt_tile = ({xct} *)dvmh_lds;',
+          'const IndexT dt_oi = lo0 + dt_a0 - 1, dt_oj = lo1 + dt_b0 - 1, dt_ok = lo2 + dt_c0 - 1;   // array index of tile slot (0, 0, 0)',
+          'const IndexT dt_tid = (IndexT)threadIdx.x, dt_nthr = (IndexT)blockDim.x;',
+          '// stage the tile and its halo: only indices the body reads, per axis',
+          'for (IndexT dt_e = dt_tid; dt_e < dt_elems; dt_e += dt_nthr) {',
+          IND + 'const IndexT dt_p = dt_e / dt_R12, dt_rem = dt_e - (dt_e / dt_R12) * dt_R12;',
+          IND + 'const IndexT dt_q = dt_rem / dt_R2, dt_r = dt_rem - (dt_rem / dt_R2) * dt_R2;',
+          IND + 'const IndexT dt_ii = dt_oi + dt_p, dt_jj = dt_oj + dt_q, dt_kk = dt_ok + dt_r;',
+          IND + 'if (' + ' && '.join(f'{v} >= lo{a} + ({rng[a][0]}) && {v} <= lo{a} + cnt{a} - 1 + ({rng[a][1]})'
+                                    for a, v in enumerate(('dt_ii', 'dt_jj', 'dt_kk'))) + ')',
+          IND * 2 + f'dt_tile[dt_e] = {X}_g[dt_ii][dt_jj][dt_kk];',
           '}',
           '__syncthreads();',
-          f'const dvmh_dev::TileView3<{xct}, IndexT> {X}{{tile, R1, R2, oi, oj, ok}};',
-          '// local hyperplanes p + q + r = s, in order',
-          'const IndexT nab = na * nb;',
-          'for (IndexT s = 0; s < na + nb + nc - 2; s++) {',
-          IND + 'for (IndexT idx = tid; idx < nab; idx += nthr) {',
-          IND * 2 + 'const IndexT p = idx / nb, q = idx - (idx / nb) * nb;',
-          IND * 2 + 'const IndexT r = s - p - q;',
-          IND * 2 + 'if (r >= 0 && r < nc) {',
-          IND * 3 + 'const IndexT t0 = a0 + p, t1 = b0 + q, t2 = c0 + r;']
+          f'const dvmh_dev::TileView3<{xct}, IndexT> {X}{{dt_tile, dt_R1, dt_R2, dt_oi, dt_oj, dt_ok}};',
+          '// local hyperplanes p + q + r = s of the tile, in order',
+          'const IndexT dt_nab = dt_na * dt_nb;',
+          'for (IndexT dt_s = 0; dt_s < dt_na + dt_nb + dt_nc - 2; dt_s++) {',
+          IND + 'for (IndexT dt_idx = dt_tid; dt_idx < dt_nab; dt_idx += dt_nthr) {',
+          IND * 2 + 'const IndexT dt_p = dt_idx / dt_nb, dt_q = dt_idx - (dt_idx / dt_nb) * dt_nb;',
+          IND * 2 + 'const IndexT dt_r = dt_s - dt_p - dt_q;',
+          IND * 2 + 'if (dt_r >= 0 && dt_r < dt_nc) {',
+          IND * 3 + 'const IndexT t0 = dt_a0 + dt_p, t1 = dt_b0 + dt_q, t2 = dt_c0 + dt_r;']
     for i in range(3):
         b.append(IND * 3 + var_assign(L, i, f't{i}', f'lo{i}', f'st{i}'))
     b.append(indent('{\n' + indent(L.body + '\n', 1) + '}', 3))
@@ -1280,16 +1311,173 @@ def tile_kernel(L):
           IND + '__syncthreads();',
           '}',
           '// write back the tile (not the halo)',
-          'const IndexT nbc = nb * nc;',
-          'for (IndexT e = tid; e < na * nbc; e += nthr) {',
-          IND + 'const IndexT p = e / nbc, rem = e - (e / nbc) * nbc;',
-          IND + 'const IndexT q = rem / nc, r = rem - (rem / nc) * nc;',
-          IND + f'{X}_g[oi + 1 + p][oj + 1 + q][ok + 1 + r] = tile[((p + 1) * R1 + (q + 1)) * R2 + (r + 1)];',
+          'const IndexT dt_nbc = dt_nb * dt_nc;',
+          'for (IndexT dt_e = dt_tid; dt_e < dt_na * dt_nbc; dt_e += dt_nthr) {',
+          IND + 'const IndexT dt_p = dt_e / dt_nbc, dt_rem = dt_e - (dt_e / dt_nbc) * dt_nbc;',
+          IND + 'const IndexT dt_q = dt_rem / dt_nc, dt_r = dt_rem - (dt_rem / dt_nc) * dt_nc;',
+          IND + f'{X}_g[dt_oi + 1 + dt_p][dt_oj + 1 + dt_q][dt_ok + 1 + dt_r] = dt_tile[((dt_p + 1) * dt_R1 + (dt_q + 1)) * dt_R2 + (dt_r + 1)];',
           '}']
-    b += red_kernel_tail(L, '((long)(ta * ntB + tb) * ntC + tc)', accumulate=False)
+    b += red_kernel_tail(L, '((long)(dt_ta * dt_ntB + dt_tb) * dt_ntC + dt_tc)', accumulate=False)
     k.append(indent('\n'.join(b) + '\n', 1).rstrip('\n'))
     k.append('}')
     return '\n'.join(k) + '\n'
+
+
+# Chunk lengths (iterations per staged chunk) the tiled line sweep is
+# instantiated for; a thread keeps its share of the next chunk, CK + 1
+# elements, in registers while the current one is computed.
+LINE_TILE_CK = [8, 16, 32]
+
+
+def line_tile_kernel(L):
+    """Tiled line sweep for an ACROSS loop whose only dependent axis is the
+    innermost one.  The direct line kernel gives each thread a line along
+    that axis, so neighbouring threads touch addresses a row apart and every
+    load is uncoalesced.  Here a workgroup owns blockDim.x lines and walks
+    them in chunks of CK iterations, double-buffered in LDS: while each
+    thread runs its line's iterations of chunk c in loop order against the
+    LDS copy (LineView), the loads of chunk c + 1 are in flight in registers
+    (consecutive threads read consecutive elements of a line); then chunk c
+    is written back the same coalesced way and chunk c + 1 committed to the
+    other buffer.  Slot 0 of a chunk holds the element before it in loop
+    order (carried, already updated), slot CK + 1 the element after it (not
+    yet updated).  Loads are clamped to the indices the body reads."""
+    r = L.rank
+    X = L.across['array']
+    xct = [p for p in L.arrays if p[1] == X][0][2]
+    dlo, dhi = L.stencil_range(X)[r - 1]
+    fp, setup = [], []
+    for p in L.arrays:
+        _, nm, ct, ar = p
+        fp.append(f'{ct} *__restrict__ {nm}_base')
+        fp += [f'IndexT {nm}_s{k}' for k in range(ar)]
+        if nm == X:
+            continue
+        strides = ', '.join(f'{nm}_s{k}' for k in range(ar))
+        setup.append(f'const IndexT {nm}_str[{ar}] = {{{strides}}};')
+        setup.append(f'const auto {nm} = dvmh_dev::makeView<{ct}, IndexT, {ar}, false>({nm}_base, {nm}_str);')
+    for p in L.scalars:
+        fp.append(f'{p[2]} *__restrict__ {p[1]}_ptr')
+        setup.append(f'{p[2]} &{p[1]} = *{p[1]}_ptr;')
+    for i in range(r):
+        fp += [f'IndexT lo{i}', f'IndexT cnt{i}', f'IndexT st{i}']
+    fp += ['IndexT dl_nlines']
+    fp += red_kernel_params(L)
+    kx = r - 1
+    # at most 256 threads (the handler checks): lines are few, so registers
+    # for deep prefetch matter more than occupancy
+    k = ['template <typename IndexT, int CK>',
+         f'__global__ __launch_bounds__(256) void {L.name}_ltile_kernel({", ".join(fp)}) {{']
+    b = ['extern __shared__ __align__(16) unsigned char dvmh_lds[];']
+    b += setup
+    for v, ct in L.vars:
+        b.append(f'{ct} {v};')
+    for pr in L.privates:
+        b.append(pr + ';')
+    b += red_kernel_locals(L)
+    b += ['constexpr IndexT dl_RS = (CK + 2) | 1;   // slots per line row (odd: spreads LDS banks)',
+          'const IndexT dl_TL = (IndexT)blockDim.x, dl_tid = (IndexT)threadIdx.x;',
+          f'{xct} *dl_buf0 = ({xct} *)dvmh_lds, *dl_buf1 = dl_buf0 + dl_TL * dl_RS;   // two chunks of dl_TL rows',
+          'IndexT *dl_rowOff = (IndexT *)(dvmh_lds + (((size_t)2 * dl_TL * dl_RS * sizeof(' + xct + ') + 15) & ~(size_t)15));',
+          'const IndexT dl_line0 = (IndexT)blockIdx.x * dl_TL;',
+          'const IndexT dl_nl = dl_nlines - dl_line0 < dl_TL ? dl_nlines - dl_line0 : dl_TL;',
+          'const bool dl_mine = dl_tid < dl_nl;',
+          '// the line this thread owns: its indices on the independent axes']
+    if r == 3:
+        b += ['const IndexT dl_myLine = dl_line0 + (dl_mine ? dl_tid : 0);',
+              'const IndexT t1 = dl_myLine % cnt1, t0 = dl_myLine / cnt1;',
+              f'dl_rowOff[dl_tid] = dl_mine ? (lo0 + t0 * st0) * {X}_s0 + (lo1 + t1 * st1) * {X}_s1 : (IndexT)0;']
+    else:
+        b += ['const IndexT t0 = dl_line0 + (dl_mine ? dl_tid : 0);',
+              f'dl_rowOff[dl_tid] = dl_mine ? (lo0 + t0 * st0) * {X}_s0 : (IndexT)0;']
+    b += ['__syncthreads();',
+          '// indices the body reads along the line: loads are clamped into them',
+          f'const IndexT dl_kA = lo{kx}, dl_kZ = lo{kx} + (cnt{kx} - 1) * st{kx};',
+          f'const IndexT dl_kMin = (dl_kA < dl_kZ ? dl_kA : dl_kZ) + ({dlo}), dl_kMax = (dl_kA < dl_kZ ? dl_kZ : dl_kA) + ({dhi});',
+          f'auto dl_load = [&](IndexT kk, IndexT row) -> {xct} {{',
+          IND + f'kk = kk < dl_kMin ? dl_kMin : (kk > dl_kMax ? dl_kMax : kk);',
+          IND + f'return {X}_base[dl_rowOff[row] + kk * {X}_s{kx}];',
+          '};',
+          f'const IndexT dl_nchunks = (cnt{kx} + CK - 1) / CK;',
+          '// chunk 0: slots 0 .. n + 1 straight into buffer 0',
+          '{',
+          IND + f'const IndexT n = cnt{kx} < CK ? cnt{kx} : CK, w = n + 2;',
+          IND + f'const IndexT kb = lo{kx} - st{kx};',
+          IND + 'for (IndexT e = dl_tid; e < dl_nl * w; e += dl_TL) {',
+          IND * 2 + 'const IndexT row = e / w, sl = e - (e / w) * w;',
+          IND * 2 + f'dl_buf0[row * dl_RS + sl] = dl_load(kb + sl * st{kx}, row);',
+          IND + '}',
+          '}',
+          '__syncthreads();',
+          f'{xct} dl_pre[CK + 1];   // this thread\'s share of the next chunk (slots 1 .. CK + 1)',
+          'for (IndexT dl_c = 0; dl_c < dl_nchunks; dl_c++) {',
+          IND + f'{xct} *dl_cur = (dl_c & 1) ? dl_buf1 : dl_buf0, *dl_nxt = (dl_c & 1) ? dl_buf0 : dl_buf1;',
+          IND + 'const IndexT dl_c0 = dl_c * CK;',
+          IND + f'const IndexT dl_n = cnt{kx} - dl_c0 < CK ? cnt{kx} - dl_c0 : CK;',
+          IND + f'const IndexT dl_kb = lo{kx} + (dl_c0 - 1) * st{kx};   // array index held by slot 0',
+          IND + '// issue the next chunk\'s loads (slots 1 .. n1 + 1): in flight during the compute',
+          IND + f'const IndexT dl_n1 = dl_c + 1 < dl_nchunks ? (cnt{kx} - dl_c0 - CK < CK ? cnt{kx} - dl_c0 - CK : CK) : 0;',
+          IND + 'const IndexT dl_w1 = dl_n1 + 1;',
+          IND + '// (unconditional: row and index are clamped, so every load is in bounds)',
+          IND + '#pragma unroll',
+          IND + 'for (int m = 0; m < CK + 1; m++) {',
+          IND * 2 + 'const IndexT e = dl_tid + (IndexT)m * dl_TL;',
+          IND * 2 + 'const IndexT row0 = e / dl_w1, sl = 1 + (e - row0 * dl_w1);',
+          IND * 2 + f'dl_pre[m] = dl_load(dl_kb + (CK + sl) * st{kx}, row0 < dl_nl ? row0 : dl_nl - 1);',
+          IND + '}',
+          IND + 'if (dl_mine) {',
+          IND * 2 + f'const dvmh_dev::LineView<{xct}, IndexT, {r}> {X}{{dl_cur + dl_tid * dl_RS, dl_kb, st{kx}}};',
+          IND * 2 + 'for (IndexT dl_i = 1; dl_i <= dl_n; dl_i++) {',
+          IND * 3 + f'const IndexT t{kx} = dl_c0 + dl_i - 1;']
+    for i in range(r):
+        b.append(IND * 3 + var_assign(L, i, f't{i}', f'lo{i}', f'st{i}'))
+    b.append(indent('{\n' + indent(L.body + '\n', 1) + '}', 3))
+    b += [IND * 2 + '}',
+          IND * 2 + 'dl_nxt[dl_tid * dl_RS] = dl_cur[dl_tid * dl_RS + dl_n];   // carried into the next chunk',
+          IND + '}',
+          IND + '__syncthreads();',
+          IND + '// write back this chunk (slots 1 .. n), commit the next one',
+          IND + 'for (IndexT e = dl_tid; e < dl_nl * dl_n; e += dl_TL) {',
+          IND * 2 + 'const IndexT row = e / dl_n, sl = 1 + (e - (e / dl_n) * dl_n);',
+          IND * 2 + f'{X}_base[dl_rowOff[row] + (dl_kb + sl * st{kx}) * {X}_s{kx}] = dl_cur[row * dl_RS + sl];',
+          IND + '}',
+          IND + '#pragma unroll',
+          IND + 'for (int m = 0; m < CK + 1; m++) {',
+          IND * 2 + 'const IndexT e = dl_tid + (IndexT)m * dl_TL;',
+          IND * 2 + 'if (dl_n1 > 0 && e < dl_nl * dl_w1) {',
+          IND * 3 + 'const IndexT row = e / dl_w1, sl = 1 + (e - (e / dl_w1) * dl_w1);',
+          IND * 3 + 'dl_nxt[row * dl_RS + sl] = dl_pre[m];',
+          IND * 2 + '}',
+          IND + '}',
+          IND + '__syncthreads();',
+          '}']
+    b += red_kernel_tail(L, 'blockIdx.x', accumulate=False)
+    k.append(indent('\n'.join(b) + '\n', 1).rstrip('\n'))
+    k.append('}')
+    return '\n'.join(k) + '\n'
+
+
+def line_tile_launch(L):
+    r = L.rank
+    b = ['DvmType nlines = 1' + ''.join(f' * cnt{a}' for a in range(r - 1)) + ';',
+         'DvmType nblocks = (nlines + threads.x - 1) / threads.x;']
+    b += handler_reds_prepare(L, 'nblocks', 0)
+    b.append('if (nblocks > 0) {')
+    b.append(IND + 'dvmh_amd_mark_kernel_C(loop_ref, 0);')
+    for it in ['int', 'long long']:
+        args = array_fact_args(L, it)
+        for i in range(r):
+            args += [f'({it})boundsLow[{i}]', f'({it})cnt{i}', f'({it})loopSteps[{i}]']
+        args += [f'({it})nlines']
+        args += red_fact_args(L)
+        for ck in LINE_TILE_CK:
+            cond = f'kernelIndexT == {"rt_INT" if it == "int" else "rt_LLONG"} && CK == {ck}'
+            b.append(IND + f'if ({cond}) hipLaunchKernelGGL(({L.name}_ltile_kernel<{it}, {ck}>), dim3((unsigned)nblocks), threads, '
+                     f'ldsBytes, stream, {", ".join(args)});')
+    b.append(IND + 'dvmh_amd_mark_kernel_C(loop_ref, 1);')
+    b.append('}')
+    b += handler_reds_finish(L)
+    return b
 
 
 def tile_launch(L):
@@ -1353,7 +1541,22 @@ def across_handler(L):
         b.append(f'{kw} (dependency_mask == {mask}) {{')
         for p in L.arrays:
             b.append(IND + f'if ({p[1]}_tfm == 2) {{ fprintf(stderr, "{L.name}: diagonal layout in a line sweep\\n"); abort(); }}')
-        b += [IND + s for s in line_launch(L, dep)]
+        if dep == r - 1 and L.line_tile_eligible():
+            X = L.across['array']
+            xct = [p for p in L.arrays if p[1] == X][0][2]
+            b.append(IND + '// tiled line sweep: unit steps along the line, natural layout, chunk fits 64 KB of LDS')
+            b.append(IND + 'DvmType CK = tune[DVMH_AMD_T_TILE_A], RS = (CK + 2) | 1;')
+            b.append(IND + f'size_t ldsBytes = (((size_t)2 * threads.x * RS * sizeof({xct}) + 15) & ~(size_t)15) + (size_t)threads.x * sizeof(DvmType);')
+            cks = ' || '.join(f'CK == {ck}' for ck in LINE_TILE_CK)
+            b.append(IND + f'bool ltile = tune[DVMH_AMD_T_VARIANT] == 1 && (loopSteps[{r - 1}] == 1 || loopSteps[{r - 1}] == -1) && '
+                     f'{X}_tfm == 0 && threads.x <= 256 && threads.y == 1 && threads.z == 1 && ({cks}) && ldsBytes <= 64512;')
+            b.append(IND + 'if (ltile) {')
+            b += [IND * 2 + s for s in line_tile_launch(L)]
+            b.append(IND + '} else {')
+            b += [IND * 2 + s for s in line_launch(L, dep)]
+            b.append(IND + '}')
+        else:
+            b += [IND + s for s in line_launch(L, dep)]
     # two or more dependent axes: hyperplane kernels for every mask with >= 2 bits
     for m in range(2, r + 1):
         for deps in itertools.combinations(range(r), m):
@@ -1394,6 +1597,8 @@ def gpu_loop(L):
                 text.append(wave_kernel(L, list(deps)))
         if L.tile_eligible():
             text.append(tile_kernel(L))
+        if L.line_tile_eligible():
+            text.append(line_tile_kernel(L))
         text.append(across_handler(L))
     else:
         text.append(parallel_kernel(L))

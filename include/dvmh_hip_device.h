@@ -7,11 +7,12 @@
  *   DiagView   the same for a layout whose two fastest storage axes are
  *              diagonalised (planes stored anti-diagonal by anti-diagonal), the
  *              index map of rts/dvmh/include/dvmhlib_device.h:56-98.
- *   MarchView2/3, TileView3
+ *   MarchView2/3, TileView3, LineView
  *              the same syntax over copies staged in LDS: a rolling window of
  *              three planes of a read-only stencil array (march variant of
- *              PARALLEL loops) and a tile with halo of an ACROSS array (tiled
- *              wavefront).
+ *              PARALLEL loops), a tile with halo of an ACROSS array (tiled
+ *              wavefront), and a chunk of one line of an ACROSS array (tiled
+ *              line sweep along the innermost axis).
  *   reductions wave64 shuffle trees plus one LDS step across the waves of a
  *              workgroup; every thread of the workgroup must call them.
  *   xcdTile    XCD-aware renumbering of workgroups: the hardware deals
@@ -135,6 +136,24 @@ struct TileView3 {
     __device__ __forceinline__ TilePlane<T, I> operator[](I i) const {
         return TilePlane<T, I>{p + (i - i0) * rows * cols, cols, j0, k0};
     }
+};
+
+// One line of an ACROSS array staged in LDS, for bodies that index it as
+// X[i][j][k + d] with the loop's own i, j and |d| <= 1 along the line: the
+// leading subscripts select nothing (they are the line the thread owns), the
+// last one maps array index k to slot (k - kb) * dir of the line's chunk.
+template <typename T, typename I, int D>
+struct LineView {
+    T *p;
+    I kb, dir;
+    __device__ __forceinline__ LineView<T, I, D - 1> operator[](I) const { return LineView<T, I, D - 1>{p, kb, dir}; }
+};
+
+template <typename T, typename I>
+struct LineView<T, I, 1> {
+    T *p;
+    I kb, dir;
+    __device__ __forceinline__ T &operator[](I k) const { return p[(k - kb) * dir]; }
 };
 
 // Diagonal position inside an Rx x Ry plane (x fastest).  Also compiled for

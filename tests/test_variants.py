@@ -1,7 +1,10 @@
 """Kernel variants the tuner can choose, forced one at a time and compared with
 the oracle bit for bit: the LDS march variant of PARALLEL stencil loops
 (rolling three-plane window, several tile shapes, points per thread and march
-lengths, including tiles that overhang the loop on every axis)."""
+lengths, including tiles that overhang the loop on every axis), the tiled
+wavefront of ACROSS loops on three axes, and the tiled line sweep of ACROSS
+loops along the innermost axis (forward and backward, chunks that do not
+divide the line, a single-iteration chunk)."""
 import ctypes
 
 import numpy as np
@@ -130,3 +133,48 @@ def test_sor3d_tiled_full_size():
         _force(KIND_WAVEFRONT, None, None)
     ref = H.oracle_sor3d(320, 2)
     assert H.bits_equal(res['A'], ref['A']) and np.array_equal(res['eps'], ref['eps'])
+
+
+# ---------------------------------------------------------------- tiled line sweep (ACROSS along the innermost axis)
+KIND_LINE = 1
+
+
+def _ltile(tl, ck):
+    tune = [0] * 8
+    tune[T_VARIANT] = 1
+    tune[T_TILE_A] = ck
+    return (tl, 1, 1), tune
+
+
+LTILES = [(64, 8), (128, 16), (256, 8), (64, 32), (128, 8)]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('cfg', LTILES)
+@pytest.mark.parametrize('n,steps', [(6, 2), (33, 2), (64, 3)])
+def test_adi3d_tiled_line_sweep_bit_exact(cfg, n, steps):
+    """The two sweeps along k use the tiled variant (the other four fall back
+    to the direct line kernel with the same block); n = 6 gives one partial
+    chunk, n = 33 and 64 several with a partial last one."""
+    block, tune = _ltile(*cfg)
+    _force(KIND_LINE, block, tune)
+    try:
+        res = H.run_adi3d(n, steps)
+    finally:
+        _force(KIND_LINE, None, None)
+    ref = H.oracle_adi3d(n, steps)
+    assert H.bits_equal(res['A'], ref['A'])
+    assert np.array_equal(res['max'], ref['max']) and np.array_equal(res['loc'], ref['loc'])
+
+
+@pytest.mark.gpu
+def test_adi3d_tiled_line_sweep_full_size():
+    """256^3 (the ADI workload's size), two steps."""
+    block, tune = _ltile(128, 16)
+    _force(KIND_LINE, block, tune)
+    try:
+        res = H.run_adi3d(256, 2)
+    finally:
+        _force(KIND_LINE, None, None)
+    ref = H.oracle_adi3d(256, 2)
+    assert H.bits_equal(res['A'], ref['A']) and np.array_equal(res['loc'], ref['loc'])
