@@ -203,11 +203,20 @@ extern int modelRank(SiteInfo &s, std::vector<int> &order);   // cost model (mod
 static std::map<long, Candidate> forced;
 
 // Candidates the cost-model mode times: the default and the model's best K.
+// The default, the model's best K, and the model's best candidate of every
+// kernel variant the site offers that the top K miss: a variant the records
+// hold little of (so the model ranks it poorly) is still timed once.
 std::vector<int> SiteInfo::trialSet() const {
     const int K = 4;
     std::vector<int> trial{0};
     for (int i = 0; i < (int)modelOrder.size() && (int)trial.size() < K + 1; i++)
         if (modelOrder[i] != 0) trial.push_back(modelOrder[i]);
+    for (int c : modelOrder) {
+        const long v = cands[c].tune[DVMH_AMD_T_VARIANT];
+        bool seen = false;
+        for (int t : trial) seen = seen || cands[t].tune[DVMH_AMD_T_VARIANT] == v;
+        if (!seen) trial.push_back(c);
+    }
     return trial;
 }
 

@@ -166,23 +166,43 @@ def expected_random_best(meas, k, trials=20000, seed=0):
     return float(best.mean())
 
 
+VARIANT_FEATURE = 14   # feature column of the kernel variant (model.cpp features())
+
+
+def site_variants(site, n):
+    x = np.zeros((n, 24), dtype=np.float32)
+    rt().dvmh_amd_model_site_features_C(site.encode(), fptr(x), n)
+    return x[:, VARIANT_FEATURE].astype(int)
+
+
+def trial_set(pred, variants, k=4):
+    """The candidates the tuner times in cost-model mode (tuner.cpp
+    SiteInfo::trialSet): the default, the model's best k, and the model's
+    best candidate of every kernel variant those miss."""
+    order = [int(c) for c in np.argsort(pred, kind='stable')]
+    top = [0] + [c for c in order if c != 0][:k]
+    for c in order:
+        if variants[c] not in {variants[t] for t in top}:
+            top.append(c)
+    return top
+
+
 def evaluate(site, k=4):
     """Quality of the model's ranking on a site with complete records."""
     pred, meas = site_table(site)
     if np.any(meas <= 0):
         raise ValueError(f'site {site} has unmeasured candidates')
     best = float(meas.min())
-    order = np.argsort(pred, kind='stable')
-    top = [0] + [int(c) for c in order if c != 0][:k]
+    top = trial_set(pred, site_variants(site, len(meas)), k)
     chosen = float(meas[top].min())
-    rand = expected_random_best(meas, k)
+    rand = expected_random_best(meas, len(top) - 1)   # the same number of timed candidates
     rp = np.argsort(np.argsort(pred))
     rm = np.argsort(np.argsort(meas))
     spearman = float(np.corrcoef(rp, rm)[0, 1])
     return {'site': site, 'candidates': int(len(meas)), 'k': k, 'exhaustive_s': best, 'default_s': float(meas[0]),
             'model_s': chosen, 'model_vs_exhaustive': chosen / best, 'default_vs_exhaustive': float(meas[0]) / best,
             'random_k_vs_exhaustive': rand / best, 'model_speedup_vs_default': float(meas[0]) / chosen,
-            'spearman': spearman, 'model_top': top}
+            'spearman': spearman, 'model_top': top, 'trials': len(top)}
 
 
 def main(argv=None):

@@ -112,10 +112,11 @@ needs_records = pytest.mark.skipif(not os.path.exists(CM.RECORDS), reason='no re
 @needs_records
 def test_model_gain_on_held_out_site():
     """Trained on the recorded sweep without the 384^3 Jacobi sites, the model's
-    choice at a 384^3 Jacobi site (best measured of its top 4 plus the default,
-    5 timed executions instead of every candidate) is within 5% of the
-    exhaustive optimum, beats the default configuration, and beats the same
-    measuring budget spent on random candidates (the method without the model)."""
+    choice at a 384^3 Jacobi site (best measured of the default, its top 4 and
+    its best candidate of any kernel variant those miss: a handful of timed
+    executions instead of every candidate) is within 5% of the exhaustive
+    optimum, beats the default configuration, and beats the same measuring
+    budget spent on random candidates (the method without the model)."""
     CM.load_records(CM.RECORDS)
     n, loss = CM.train(epochs=3000, lr=3e-3, seed=1, holdout=CM.HOLDOUT)
     assert n > 500 and loss < 0.5
@@ -125,3 +126,20 @@ def test_model_gain_on_held_out_site():
         assert ev['model_vs_exhaustive'] <= 1.05, ev
         assert ev['model_vs_exhaustive'] < ev['random_k_vs_exhaustive'], ev
         assert ev['model_speedup_vs_default'] > 1.0, ev
+
+
+@pytest.mark.gpu
+@needs_records
+def test_trial_set_reaches_rare_variant():
+    """The ADI sweeps along the innermost axis at 256^3 have a kernel variant
+    (tiled line sweep) that few records show, so the model ranks it low; the
+    trial set still times its best-ranked candidate, and the site's choice is
+    within 10% of its exhaustive optimum (in-sample: these sites are trained on)."""
+    CM.load_records(CM.RECORDS)
+    CM.train(epochs=3000, lr=3e-3, seed=1, holdout=CM.HOLDOUT)
+    sites = [s for s in CM.sites() if s.startswith(('adi3d.cdv:54|', 'adi3d.cdv:59|')) and s.endswith('e256x256x255')]
+    assert sites
+    for site in sites:
+        ev = CM.evaluate(site, k=4)
+        print(ev)
+        assert ev['model_vs_exhaustive'] <= 1.10, ev
