@@ -7,10 +7,15 @@
 // (a few thousand values) in a single 1024-thread workgroup and only the final
 // record goes back to the host.  Order is fixed (thread t folds t, t+1024, ...
 // sequentially, then fixed wave64 shuffle trees), so the result is
-// deterministic; for MAX/MIN with loc the lowest partial index wins ties.
+// deterministic.  MAXLOC/MINLOC ties keep the lower loc when the loc record is
+// a scalar integer (1, 2, 4 or 8 bytes, read as signed), as inside the loop
+// kernels, so the winner is the sequential program's whenever the loc grows
+// with the iteration; other records keep the lower partial index.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <type_traits>
 
@@ -30,37 +35,71 @@ __device__ inline T identityOf(int f) {
     return (T)0;
 }
 
-// acc <- acc (op) v.  Returns true when v replaced acc (MAX/MIN family).
+// Tie-break key of partial e for MAXLOC/MINLOC: its loc record read as a
+// signed integer of `bytes` bytes (1, 2, 4 or 8), or 0 when bytes is 0 or e
+// is no position (-1, the winner of an empty range).
+struct LocKey {
+    const char *locs;
+    long size;
+    int bytes;
+    __device__ inline long operator()(long e) const {
+        if (e < 0) return 0;
+        const char *p = locs + e * size;
+        switch (bytes) {
+            case 8: { long long v; __builtin_memcpy(&v, p, 8); return (long)v; }
+            case 4: { int v; __builtin_memcpy(&v, p, 4); return v; }
+            case 2: { short v; __builtin_memcpy(&v, p, 2); return v; }
+            case 1: return (long)(signed char)p[0];
+            default: return 0;
+        }
+    }
+};
+
+__host__ __device__ inline int locKeyBytes(int f, const void *locs, long locSize) {
+    if ((f != rf_MAXLOC && f != rf_MINLOC) || !locs) return 0;
+    return (locSize == 1 || locSize == 2 || locSize == 4 || locSize == 8) ? (int)locSize : 0;
+}
+
+// Partial being folded: value, tie-break key, position in the partial buffer.
 template <typename T>
-__device__ inline bool combine(int f, T &acc, long &accIdx, T v, long vIdx) {
+struct Acc {
+    T v;
+    long key, pos;
+};
+
+// a <- a (op) b for the arithmetic and MAX/MIN families.  Equal (unordered)
+// values keep the lower key, then the lower position.
+template <typename T>
+__device__ inline void combine(int f, Acc<T> &a, const Acc<T> &b) {
+    const bool lower = b.key < a.key || (b.key == a.key && b.pos < a.pos);
     switch (f) {
-        case rf_SUM: acc = acc + v; return false;
-        case rf_PROD: acc = acc * v; return false;
+        case rf_SUM: a.v = a.v + b.v; return;
+        case rf_PROD: a.v = a.v * b.v; return;
         case rf_MAX: case rf_MAXLOC:
-            if (acc < v || (!(v < acc) && vIdx < accIdx)) { acc = v; accIdx = vIdx; return true; }
-            return false;
+            if (a.v < b.v || (!(b.v < a.v) && lower)) a = b;
+            return;
         case rf_MIN: case rf_MINLOC:
-            if (acc > v || (!(v > acc) && vIdx < accIdx)) { acc = v; accIdx = vIdx; return true; }
-            return false;
-        default: return false;
+            if (a.v > b.v || (!(b.v > a.v) && lower)) a = b;
+            return;
+        default: return;
     }
 }
 
 template <typename T>
-__device__ inline bool combineBits(int f, T &acc, long &accIdx, T v, long vIdx) {
+__device__ inline void combineBits(int f, Acc<T> &a, const Acc<T> &b) {
     switch (f) {
-        case rf_AND: acc = acc & v; return false;
-        case rf_OR: acc = acc | v; return false;
-        case rf_XOR: case rf_NE: acc = acc ^ v; return false;
-        case rf_EQU: case rf_EQ: acc = (T)(acc == v); return false;
-        default: return combine<T>(f, acc, accIdx, v, vIdx);
+        case rf_AND: a.v = a.v & b.v; return;
+        case rf_OR: a.v = a.v | b.v; return;
+        case rf_XOR: case rf_NE: a.v = a.v ^ b.v; return;
+        case rf_EQU: case rf_EQ: a.v = (T)(a.v == b.v); return;
+        default: combine<T>(f, a, b); return;
     }
 }
 
 template <typename T, bool BITS>
-__device__ inline void comb(int f, T &acc, long &accIdx, T v, long vIdx) {
-    if constexpr (BITS) combineBits<T>(f, acc, accIdx, v, vIdx);
-    else combine<T>(f, acc, accIdx, v, vIdx);
+__device__ inline void comb(int f, Acc<T> &a, const Acc<T> &b) {
+    if constexpr (BITS) combineBits<T>(f, a, b);
+    else combine<T>(f, a, b);
 }
 
 // Bit-exact lane shuffle of any element type used here (1, 4 or 8 bytes).
@@ -84,75 +123,91 @@ __device__ inline T shflDownAny(T v, int o) {
 }
 
 // Tree over the 64 lanes of a wavefront; lane 0 ends with the fold of all.
-template <typename T, bool BITS>
-__device__ inline void waveFold(int f, T &acc, long &accIdx, int &has) {
+// KEYS: the partials carry loc keys (MAXLOC/MINLOC), shuffled with them.
+template <typename T, bool BITS, bool KEYS>
+__device__ inline void waveFold(int f, Acc<T> &acc, int &has) {
     const int lane = threadIdx.x & 63;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
-        const T v = shflDownAny(acc, o);
-        const long vi = shflDownAny(accIdx, o);
+        Acc<T> b;
+        b.v = shflDownAny(acc.v, o);
+        b.pos = shflDownAny(acc.pos, o);
+        b.key = KEYS ? shflDownAny(acc.key, o) : 0;
         const int vh = __shfl_down(has, o, 64);
         if (vh && lane + o < 64) {
-            if (!has) { acc = v; accIdx = vi; has = 1; }
-            else comb<T, BITS>(f, acc, accIdx, v, vi);
+            if (!has) { acc = b; has = 1; }
+            else comb<T, BITS>(f, acc, b);
         }
     }
 }
 
 // One workgroup (FIN_THREADS threads) folds vals[begin, end).  Element e of
-// the range carries the tie-break index idx ? idx[e] : e.  The result and the
-// winner's index go to outVal / outIdx (index -1 for an empty range).  Thread
-// t folds t, t + FIN_THREADS, ... in order (8 loads in flight), then a wave64
-// shuffle tree per wavefront and one more over the wavefront results: a fixed
-// order, so the result is deterministic.
-template <typename T, bool BITS>
-__device__ void foldRange(int f, long begin, long end, const T *__restrict__ vals, const long *__restrict__ idx, T *outVal,
-        long *outIdx) {
+// the range sits at position idx ? idx[e] : e of the partial buffer, and its
+// tie-break key is lk(position).  The result and the winner's position go to
+// outVal / outIdx (position -1 for an empty range).  Thread t folds t,
+// t + FIN_THREADS, ... in order (8 loads in flight), then a wave64 shuffle
+// tree per wavefront and one more over the wavefront results: a fixed order,
+// so the result is deterministic.
+template <typename T, bool BITS, bool KEYS>
+__device__ void foldRangeK(int f, long begin, long end, const T *__restrict__ vals, const long *__restrict__ idx, LocKey lk,
+        T *outVal, long *outIdx) {
     constexpr int NW = FIN_THREADS / 64;
     __shared__ T sVal[NW];
-    __shared__ long sIdx[NW];
+    __shared__ long sPos[NW];
+    __shared__ long sKey[NW];
     __shared__ int sHas[NW];
     const int t = threadIdx.x;
-    T acc = identityOf<T>(f);
-    long accIdx = -1;
+    Acc<T> acc{identityOf<T>(f), 0, -1};
     int has = 0;
     long i = begin + t;
     for (; i + 7 * FIN_THREADS < end; i += 8 * FIN_THREADS) {
-        T v[8];
-        long vi[8];
+        Acc<T> v[8];
 #pragma unroll
         for (int q = 0; q < 8; q++) {
-            v[q] = vals[i + q * FIN_THREADS];
-            vi[q] = idx ? idx[i + q * FIN_THREADS] : i + q * FIN_THREADS;
+            v[q].v = vals[i + q * FIN_THREADS];
+            v[q].pos = idx ? idx[i + q * FIN_THREADS] : i + q * FIN_THREADS;
         }
-        if (!has) { acc = v[0]; accIdx = vi[0]; has = 1; } else comb<T, BITS>(f, acc, accIdx, v[0], vi[0]);
 #pragma unroll
-        for (int q = 1; q < 8; q++) comb<T, BITS>(f, acc, accIdx, v[q], vi[q]);
+        for (int q = 0; q < 8; q++) v[q].key = KEYS ? lk(v[q].pos) : 0;
+        if (!has) { acc = v[0]; has = 1; } else comb<T, BITS>(f, acc, v[0]);
+#pragma unroll
+        for (int q = 1; q < 8; q++) comb<T, BITS>(f, acc, v[q]);
     }
     for (; i < end; i += FIN_THREADS) {
-        T v = vals[i];
-        long vi = idx ? idx[i] : i;
-        if (!has) { acc = v; accIdx = vi; has = 1; }
-        else comb<T, BITS>(f, acc, accIdx, v, vi);
+        Acc<T> v;
+        v.v = vals[i];
+        v.pos = idx ? idx[i] : i;
+        v.key = KEYS ? lk(v.pos) : 0;
+        if (!has) { acc = v; has = 1; }
+        else comb<T, BITS>(f, acc, v);
     }
-    waveFold<T, BITS>(f, acc, accIdx, has);
+    waveFold<T, BITS, KEYS>(f, acc, has);
     const int w = t >> 6, lane = t & 63;
     if (lane == 0) {
-        sVal[w] = acc;
-        sIdx[w] = has ? accIdx : -1;
+        sVal[w] = acc.v;
+        sPos[w] = has ? acc.pos : -1;
+        sKey[w] = acc.key;
         sHas[w] = has;
     }
     __syncthreads();
     if (w == 0) {
-        acc = lane < NW ? sVal[lane] : identityOf<T>(f);
-        accIdx = lane < NW ? sIdx[lane] : -1;
+        acc.v = lane < NW ? sVal[lane] : identityOf<T>(f);
+        acc.pos = lane < NW ? sPos[lane] : -1;
+        acc.key = lane < NW ? sKey[lane] : 0;
         has = lane < NW ? sHas[lane] : 0;
-        waveFold<T, BITS>(f, acc, accIdx, has);
+        waveFold<T, BITS, KEYS>(f, acc, has);
         if (lane == 0) {
-            *outVal = has ? acc : identityOf<T>(f);
-            if (outIdx) *outIdx = has ? accIdx : -1;
+            *outVal = has ? acc.v : identityOf<T>(f);
+            if (outIdx) *outIdx = has ? acc.pos : -1;
         }
     }
+}
+
+template <typename T, bool BITS>
+__device__ void foldRange(int f, long begin, long end, const T *__restrict__ vals, const long *__restrict__ idx, LocKey lk,
+        T *outVal, long *outIdx) {
+    if (lk.bytes) foldRangeK<T, BITS, true>(f, begin, end, vals, idx, lk, outVal, outIdx);
+    else foldRangeK<T, BITS, false>(f, begin, end, vals, idx, lk, outVal, outIdx);
 }
 
 // Copies the winner's loc record; *winShared was written by thread 0.
@@ -166,10 +221,12 @@ __device__ void copyLoc(const char *locs, long locSize, const long *winShared, c
 
 // pass 1: block b folds [b * chunk, (b + 1) * chunk)
 template <typename T, bool BITS>
-__global__ __launch_bounds__(FIN_THREADS) void chunkKernel(int f, long n, long chunk, const T *vals, T *outVals, long *outIdx) {
+__global__ __launch_bounds__(FIN_THREADS) void chunkKernel(int f, long n, long chunk, const T *vals, const char *locs,
+        long locSize, T *outVals, long *outIdx) {
     long b = blockIdx.x;
     long e = (b + 1) * chunk < n ? (b + 1) * chunk : n;
-    foldRange<T, BITS>(f, b * chunk, e, vals, nullptr, outVals + b, outIdx + b);
+    const LocKey lk{locs, locSize, locKeyBytes(f, locs, locSize)};
+    foldRange<T, BITS>(f, b * chunk, e, vals, nullptr, lk, outVals + b, outIdx + b);
 }
 
 // final pass over n values (with their original indices if idx != 0)
@@ -177,7 +234,8 @@ template <typename T, bool BITS>
 __global__ __launch_bounds__(FIN_THREADS) void finishKernel(int f, long n, const T *vals, const long *idx, const char *locs,
         long locSize, T *outVal, char *outLoc) {
     __shared__ long win;
-    foldRange<T, BITS>(f, 0, n, vals, idx, outVal, &win);
+    const LocKey lk{locs, locSize, locKeyBytes(f, locs, locSize)};
+    foldRange<T, BITS>(f, 0, n, vals, idx, lk, outVal, &win);
     copyLoc<T>(locs, locSize, &win, outLoc);
 }
 
@@ -186,7 +244,8 @@ __global__ __launch_bounds__(FIN_THREADS) void segmentsKernel(int f, const long 
         long locSize, T *outVals, char *outLocs) {
     __shared__ long win;
     const long g = blockIdx.x;
-    foldRange<T, BITS>(f, offsets[g], offsets[g + 1], vals, nullptr, outVals + g, &win);
+    const LocKey lk{locs, locSize, locKeyBytes(f, locs, locSize)};
+    foldRange<T, BITS>(f, offsets[g], offsets[g + 1], vals, nullptr, lk, outVals + g, &win);
     copyLoc<T>(locs, locSize, &win, outLocs ? outLocs + g * locSize : nullptr);
 }
 
@@ -215,7 +274,8 @@ void launchFinish(int f, long n, const void *vals, const void *locs, long locSiz
         long g = (n + FIN_CHUNK - 1) / FIN_CHUNK;
         T *pv = (T *)scratch;
         long *pi = (long *)((char *)scratch + ((g * sizeof(T) + 15) & ~15L));
-        hipLaunchKernelGGL((chunkKernel<T, BITS>), dim3((unsigned)g), dim3(FIN_THREADS), 0, s, f, n, FIN_CHUNK, (const T *)vals, pv, pi);
+        hipLaunchKernelGGL((chunkKernel<T, BITS>), dim3((unsigned)g), dim3(FIN_THREADS), 0, s, f, n, FIN_CHUNK, (const T *)vals,
+                (const char *)locs, locSize, pv, pi);
         hipLaunchKernelGGL((finishKernel<T, BITS>), dim3(1), dim3(FIN_THREADS), 0, s, f, g, (const T *)pv, (const long *)pi,
                 (const char *)locs, locSize, (T *)outVal, (char *)outLoc);
         return;
@@ -230,7 +290,7 @@ void launchFinish(int f, long n, const void *vals, const void *locs, long locSiz
 // all and writes the result.  A larger one gets a block per MULTI_CHUNK
 // partials: block c = b - first[r] folds [c * MULTI_CHUNK, (c + 1) * MULTI_CHUNK)
 // into slot c, and the block that arrives last at r's counter folds r's slots
-// in slot order (ties: lower partial index), writes the result and the
+// in slot order (ties as in foldRange), writes the result and the
 // winner's loc record, and resets the counter for the next launch.
 constexpr long MULTI_CHUNK = 8 * FIN_THREADS;
 constexpr long MULTI_SINGLE = 16 * FIN_THREADS;
@@ -280,8 +340,9 @@ __global__ __launch_bounds__(FIN_THREADS) void multiFinishKernel(MultiArgs a) {
     while (r + 1 < a.count && b >= a.first[r + 1]) r++;
     const long c = b - a.first[r];
     const long nb = a.first[r + 1] - a.first[r];
+    const LocKey lk{a.locs[r], a.locSize[r], locKeyBytes(a.func[r], a.locs[r], a.locSize[r])};
     if (nb == 1) {
-        foldRange<T, BITS>(a.func[r], 0, a.n[r], (const T *)a.vals[r], nullptr, (T *)a.outVal[r], &win);
+        foldRange<T, BITS>(a.func[r], 0, a.n[r], (const T *)a.vals[r], nullptr, lk, (T *)a.outVal[r], &win);
         copyLoc<T>(a.locs[r], a.locSize[r], &win, a.outLoc[r]);
         multiResultWritten(a);
         return;
@@ -290,7 +351,7 @@ __global__ __launch_bounds__(FIN_THREADS) void multiFinishKernel(MultiArgs a) {
     long *slotIdx = (long *)((char *)a.slots[r] + ((nb * (long)sizeof(T) + 15) & ~15L));
     const long lo = c * MULTI_CHUNK;
     const long hi = lo + MULTI_CHUNK < a.n[r] ? lo + MULTI_CHUNK : a.n[r];
-    foldRange<T, BITS>(a.func[r], lo, hi, (const T *)a.vals[r], nullptr, slotVal + c, slotIdx + c);
+    foldRange<T, BITS>(a.func[r], lo, hi, (const T *)a.vals[r], nullptr, lk, slotVal + c, slotIdx + c);
     if (threadIdx.x == 0) {
         __threadfence();   // the slot is visible before the arrival is counted
         const unsigned before = atomicAdd(&a.counters[r], 1u);
@@ -299,7 +360,7 @@ __global__ __launch_bounds__(FIN_THREADS) void multiFinishKernel(MultiArgs a) {
     __syncthreads();
     if (!isLast) return;
     __threadfence();
-    foldRange<T, BITS>(a.func[r], 0, nb, slotVal, slotIdx, (T *)a.outVal[r], &win);
+    foldRange<T, BITS>(a.func[r], 0, nb, slotVal, slotIdx, lk, (T *)a.outVal[r], &win);
     copyLoc<T>(a.locs[r], a.locSize[r], &win, a.outLoc[r]);
     if (threadIdx.x == 0) a.counters[r] = 0u;
     multiResultWritten(a);
@@ -342,12 +403,17 @@ void launchSegments(int f, long groups, const long *offsets, const void *vals, c
 
 #define DISPATCH_TYPES(LAUNCH, ...)                                                        \
     switch (rtType) {                                                                      \
-        case rt_CHAR: case rt_UCHAR: LAUNCH<int8_t, true>(__VA_ARGS__); break;             \
-        case rt_INT: case rt_UINT: case rt_LOGICAL: LAUNCH<int32_t, true>(__VA_ARGS__); break; \
-        case rt_LONG: case rt_ULONG: case rt_LLONG: case rt_ULLONG: LAUNCH<int64_t, true>(__VA_ARGS__); break; \
+        case rt_CHAR: LAUNCH<int8_t, true>(__VA_ARGS__); break;                            \
+        case rt_UCHAR: LAUNCH<uint8_t, true>(__VA_ARGS__); break;                          \
+        case rt_INT: case rt_LOGICAL: LAUNCH<int32_t, true>(__VA_ARGS__); break;           \
+        case rt_UINT: LAUNCH<uint32_t, true>(__VA_ARGS__); break;                          \
+        case rt_LONG: case rt_LLONG: LAUNCH<int64_t, true>(__VA_ARGS__); break;            \
+        case rt_ULONG: case rt_ULLONG: LAUNCH<uint64_t, true>(__VA_ARGS__); break;         \
         case rt_FLOAT: LAUNCH<float, false>(__VA_ARGS__); break;                           \
         case rt_DOUBLE: LAUNCH<double, false>(__VA_ARGS__); break;                         \
-        default: break;                                                                    \
+        default:                                                                           \
+            fprintf(stderr, "DVMH internal error: reduction element type %d has no GPU fold\n", rtType); \
+            abort();                                                                       \
     }
 
 extern "C" long dvmh_hip_reduce_scratch_bytes(long n) {

@@ -428,6 +428,8 @@ void *Data::hostNaturalBase() const {
 // Element operations of DvmhReduction (rts/dvmh/loop.cpp:37-254): SUM/PROD and
 // the bit/logic family start from an identity, MAX/MIN from the variable's value
 // at registration; MAX/MIN replace on strict compare and then copy the loc.
+// Unsigned element types compare as unsigned (the reference folds only the
+// signed ones, rts/dvmh/loop.cpp:125-254).
 template <typename T>
 static void initTyped(int func, void *p) {
     T v = (func == rf_PROD) ? (T)1 : (T)0;
@@ -473,12 +475,12 @@ static int opTyped(int func, void *d, const void *s) {
     switch (func) {
         case rf_SUM: a = a + b; break;
         case rf_PROD: a = a * b; break;
-        case rf_MAX: case rf_MAXLOC: if (a < b) { a = b; replaced = 1; } break;
-        case rf_MIN: case rf_MINLOC: if (a > b) { a = b; replaced = 1; } break;
+        case rf_MAX: case rf_MAXLOC: if (a < b) { a = b; replaced = 1; } else if (a == b) replaced = 2; break;
+        case rf_MIN: case rf_MINLOC: if (a > b) { a = b; replaced = 1; } else if (a == b) replaced = 2; break;
         default: return -1;
     }
     std::memcpy(d, &a, sizeof(T));
-    return replaced;
+    return replaced;   // 2: equal values (MAX/MIN family)
 }
 
 template <typename T>
@@ -513,16 +515,20 @@ static int opComplex(int func, void *d, const void *s) {
     return 0;
 }
 
-void Reduction::performOperation(void *dArr, void *dLoc, const void *sArr, const void *sLoc) const {
+void Reduction::performOperation(void *dArr, void *dLoc, const void *sArr, const void *sLoc, bool lowerLocOnTie) const {
     for (long i = 0; i < elemCount; i++) {
         void *d = (char *)dArr + elemSize * i;
         const void *s = (const char *)sArr + elemSize * i;
         int r = -1;
         switch (rtType) {
-            case rt_CHAR: case rt_UCHAR: r = opBits<char>(func, d, s); break;
-            case rt_INT: case rt_UINT: case rt_LOGICAL: r = opBits<int>(func, d, s); break;
-            case rt_LONG: case rt_ULONG: r = opBits<long>(func, d, s); break;
-            case rt_LLONG: case rt_ULLONG: r = opBits<long long>(func, d, s); break;
+            case rt_CHAR: r = opBits<signed char>(func, d, s); break;
+            case rt_UCHAR: r = opBits<unsigned char>(func, d, s); break;
+            case rt_INT: case rt_LOGICAL: r = opBits<int>(func, d, s); break;
+            case rt_UINT: r = opBits<unsigned int>(func, d, s); break;
+            case rt_LONG: r = opBits<long>(func, d, s); break;
+            case rt_ULONG: r = opBits<unsigned long>(func, d, s); break;
+            case rt_LLONG: r = opBits<long long>(func, d, s); break;
+            case rt_ULLONG: r = opBits<unsigned long long>(func, d, s); break;
             case rt_FLOAT: r = opTyped<float>(func, d, s); break;
             case rt_DOUBLE: r = opTyped<double>(func, d, s); break;
             case rt_FLOAT_COMPLEX: r = opComplex<float>(func, d, s); break;
@@ -530,14 +536,28 @@ void Reduction::performOperation(void *dArr, void *dLoc, const void *sArr, const
             default: break;
         }
         DVMH_INTERNAL(r != -1, "No suitable reduction found for function %d and type %d", func, rtType);
-        if (isLoc() && dLoc && r == 1)
+        if (isLoc() && dLoc && sLoc &&
+                (r == 1 || (r == 2 && lowerLocOnTie && locKeyLess((const char *)sLoc + locSize * i, (const char *)dLoc + locSize * i))))
             std::memcpy((char *)dLoc + locSize * i, (const char *)sLoc + locSize * i, (size_t)locSize);
     }
 }
 
-void Reduction::postValues(const void *arr, const void *loc) {
+// Tie rule of MAXLOC/MINLOC merges: a loc record of 1, 2, 4 or 8 bytes is a
+// scalar integer (read as signed) and the lower one wins, as in the GPU folds;
+// other records keep the destination's.
+bool Reduction::locKeyLess(const void *a, const void *b) const {
+    switch (locSize) {
+        case 1: return *(const signed char *)a < *(const signed char *)b;
+        case 2: { short x, y; std::memcpy(&x, a, 2); std::memcpy(&y, b, 2); return x < y; }
+        case 4: { int x, y; std::memcpy(&x, a, 4); std::memcpy(&y, b, 4); return x < y; }
+        case 8: { long x, y; std::memcpy(&x, a, 8); std::memcpy(&y, b, 8); return x < y; }
+        default: return false;
+    }
+}
+
+void Reduction::postValues(const void *arr, const void *loc, bool lowerLocOnTie) {
     std::lock_guard<std::mutex> g(lock);
-    performOperation(arrayAddr, locAddr, arr, loc);
+    performOperation(arrayAddr, locAddr, arr, loc, lowerLocOnTie);
 }
 
 }  // namespace dvmh

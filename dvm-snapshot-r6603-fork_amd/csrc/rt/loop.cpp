@@ -160,7 +160,7 @@ void flushDeferredPosts() {
     pendingDone = 0;
     for (auto &p : pending) {
         p.red->postValues(p.val, p.loc);
-        if (p.addInitial) p.red->postValues(p.red->initArr.data(), p.red->isLoc() ? p.red->initLoc.data() : nullptr);
+        if (p.addInitial) p.red->postValues(p.red->initArr.data(), p.red->isLoc() ? p.red->initLoc.data() : nullptr, false);
     }
     pending.clear();
     pinnedPool.reset();
@@ -292,7 +292,7 @@ static void finishLoopReductions(Loop *loop) {
         bool inFlight = false;
         for (auto it = pending.rbegin(); it != pending.rend(); ++it)
             if (it->red == r) { it->addInitial = true; inFlight = true; break; }
-        if (!inFlight) r->postValues(r->initArr.data(), r->isLoc() ? r->initLoc.data() : nullptr);
+        if (!inFlight) r->postValues(r->initArr.data(), r->isLoc() ? r->initLoc.data() : nullptr, false);
     }
 }
 

@@ -192,8 +192,11 @@ struct Reduction {
     long items = 0;
     std::vector<char> localArr, localLoc;
     void initValues(void *arr, void *loc) const;                               // :37-91
-    void performOperation(void *dArr, void *dLoc, const void *sArr, const void *sLoc) const;  // :125-254
-    void postValues(const void *arr, const void *loc);                           // :97-102
+    // lowerLocOnTie: equal values take the lower scalar-integer loc (the merge
+    // of partial results); false keeps the destination's (initial values)
+    void performOperation(void *dArr, void *dLoc, const void *sArr, const void *sLoc, bool lowerLocOnTie) const;  // :125-254
+    void postValues(const void *arr, const void *loc, bool lowerLocOnTie = true);   // :97-102
+    bool locKeyLess(const void *a, const void *b) const;
     bool isLoc() const { return locAddr != nullptr; }
 };
 

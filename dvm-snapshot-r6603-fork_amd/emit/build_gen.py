@@ -15,9 +15,10 @@ import cdvmh  # noqa: E402
 import hipgen  # noqa: E402
 
 # programs/*.cdv that are complete C-DVMH programs: the configurations, and
-# relax2d for constructs they do not use (batched.cdv sketches the batched
-# workload's structure and is not compilable C)
-CDV_PROGRAMS = ['jac2d', 'jac3d', 'sor3d', 'adi3d', 'relax2d']
+# relax2d and redall for constructs and reduction kinds they do not use
+# (batched.cdv sketches the batched workload's structure and is not
+# compilable C)
+CDV_PROGRAMS = ['jac2d', 'jac3d', 'sor3d', 'adi3d', 'relax2d', 'redall']
 
 for spec in sorted(glob.glob(os.path.join(PKG, 'programs', '*.loops'))):
     hipgen.emit(spec, os.path.join(PKG, 'gen'))

@@ -60,8 +60,8 @@ QUALIFIERS = {'const', 'static', 'volatile', 'register', 'extern'}
 RT_TYPE = {'char': 'rt_CHAR', 'unsigned char': 'rt_UCHAR', 'int': 'rt_INT', 'unsigned int': 'rt_UINT',
            'unsigned': 'rt_UINT', 'long': 'rt_LONG', 'unsigned long': 'rt_ULONG', 'long long': 'rt_LLONG',
            'unsigned long long': 'rt_ULLONG', 'float': 'rt_FLOAT', 'double': 'rt_DOUBLE'}
-RED_FUNC = {'sum': 'rf_SUM', 'product': 'rf_PROD', 'max': 'rf_MAX', 'min': 'rf_MIN', 'and': 'rf_AND',
-            'or': 'rf_OR', 'xor': 'rf_XOR', 'maxloc': 'rf_MAXLOC', 'minloc': 'rf_MINLOC'}
+RED_FUNC = {'sum': 'rf_SUM', 'product': 'rf_PROD', 'prod': 'rf_PROD', 'max': 'rf_MAX', 'min': 'rf_MIN',
+            'and': 'rf_AND', 'or': 'rf_OR', 'xor': 'rf_XOR', 'maxloc': 'rf_MAXLOC', 'minloc': 'rf_MINLOC'}
 
 
 class ConvertError(Exception):
@@ -652,7 +652,7 @@ class Converter:
             if name == 'reduction':
                 for r in split_top(args):
                     rm = re.match(r'(\w+)\s*\((.*)\)$', r.strip(), re.S)
-                    if not rm or rm.group(1) not in RED_FUNC:
+                    if not rm or rm.group(1) not in RED_FUNC or rm.group(1) == 'prod':
                         self.err(d.line, f'reduction {r!r} is not supported')
                     vs = [x.strip() for x in split_top(rm.group(2))]
                     for x in vs:
@@ -665,7 +665,9 @@ class Converter:
                     else:
                         if len(vs) != 1:
                             self.err(d.line, f'{rm.group(1)} takes one variable')
-                        reds.append((rm.group(1), vs[0], self.scope[vs[0]].ctype))
+                        # the handler emitter's name for PRODUCT is 'prod'
+                        fn = 'prod' if rm.group(1) == 'product' else rm.group(1)
+                        reds.append((fn, vs[0], self.scope[vs[0]].ctype))
             elif name == 'private':
                 for x in split_top(args):
                     x = x.strip()

@@ -249,8 +249,24 @@ __device__ __forceinline__ DiagView<T, I, R, 0> makeDiagView(T *base, const I *s
 }
 
 // ------------------------------------------------------------------ reductions
+// Lane shuffle of any reduction element type (1, 2, 4 or 8 bytes, signed or
+// unsigned, integer or floating), bit for bit.
 template <typename T>
-__device__ __forceinline__ T shflDown(T v, int o) { return __shfl_down(v, o, 64); }
+__device__ __forceinline__ T shflDown(T v, int o) {
+    static_assert(sizeof(T) == 1 || sizeof(T) == 2 || sizeof(T) == 4 || sizeof(T) == 8, "reduction element size");
+    if constexpr (sizeof(T) == 8) {
+        long long x;
+        __builtin_memcpy(&x, &v, 8);
+        x = __shfl_down(x, o, 64);
+        __builtin_memcpy(&v, &x, 8);
+    } else {
+        int x = 0;
+        __builtin_memcpy(&x, &v, sizeof(T));
+        x = __shfl_down(x, o, 64);
+        __builtin_memcpy(&v, &x, sizeof(T));
+    }
+    return v;
+}
 
 struct OpSum {
     template <typename T> __device__ __forceinline__ T operator()(T a, T b) const { return a + b; }

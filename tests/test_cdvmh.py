@@ -10,9 +10,13 @@
 - relax2d.cdv (ACROSS on two axes, private, MAX + MIN, cuda_block, get_actual
   of a sub-array, host element reads) against its sequential semantics, the
   lexicographic order an ACROSS loop preserves.
+- redall.cdv (SUM, PRODUCT, MAX, MIN, AND, OR, XOR, MAXLOC, MINLOC over
+  double, long, int, unsigned int and float, initial values folded in, ties)
+  against numpy folds of the same data (first occurrence for the LOC forms).
 - Constructs outside the subset stop the conversion with the line.
 """
 import ast
+import math
 import ctypes
 import os
 import re
@@ -227,4 +231,110 @@ def test_relax2d_gpu(n, iters):
     anti-diagonal, MAX and MIN in one loop, and the sub-array fetch."""
     before = _gpu_loops()
     _check_relax2d(n, iters)
+    assert _gpu_loops() > before
+
+
+# ---------------------------------------------------------------- redall
+_GOLD = np.uint64(0x9E3779B97F4A7C15)
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+
+
+def _mix64(z):
+    """SplitMix64 finaliser (programs/dvmh_prog_common.h prog_mix64) on uint64 arrays."""
+    z = (z ^ (z >> np.uint64(30))) * _M1
+    z = (z ^ (z >> np.uint64(27))) * _M2
+    return z ^ (z >> np.uint64(31))
+
+
+def _u01(seed, idx):
+    """prog_u01 of programs/dvmh_prog_common.h on uint64 index arrays."""
+    z = np.uint64(seed) + (idx + np.uint64(1)) * _GOLD
+    return (_mix64(z) >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def test_generator_port_matches_oracle_grid():
+    """The numpy port of the programs' generator equals the oracle's (interior
+    points of orc_init_grid2d are prog_u01(seed, i * n + j))."""
+    n, seed = 9, D.SEED_BASE + 8
+    g = np.zeros(n * n)
+    H.oracle().orc_init_grid2d(H.vp(g), n, seed, 1.0)
+    u = _u01(seed, np.arange(n * n, dtype=np.uint64)).reshape(n, n)
+    assert H.bits_equal(g.reshape(n, n)[1:-1, 1:-1], u[1:-1, 1:-1])
+
+
+def redall_reference(n, m, seed):
+    """Results of redall.cdv from numpy on the same data: library folds, first
+    occurrence for MAXLOC/MINLOC (numpy.argmax/argmin), each reduction's
+    initial value folded in as the program's comment states."""
+    idx = np.arange(n * m, dtype=np.uint64)
+    z = _mix64(np.uint64(seed) + idx)
+    A = _u01(seed, idx)
+    K = (z % np.uint64(1000)).astype(np.int64) - 500
+    I = ((z >> np.uint64(20)) % np.uint64(300)).astype(np.int64)
+    U = z >> np.uint64(32)
+    F = _u01(seed + 1, idx).astype(np.float32)
+    w = (_mix64(np.uint64(seed + 7) + idx) | np.uint64(0x0101000000000000)).view(np.int64)
+    return {
+        'dsum': math.fsum(list(A) + [1.5]),
+        'dprod': 2.0 * math.exp(math.fsum(np.log1p((A - 0.5) * 0.001))),
+        'dmax': max(float(A.max()), 0.25), 'dmn': float(A.min()), 'dloc': int(A.argmin()),
+        'lsum': int(K.sum()) + 1000000, 'lmax': max(int(K.max()), 10000), 'lmin': min(int(K.min()), -450),
+        'land': int(np.bitwise_and.reduce(w)) & ~0xFF, 'lor': int(np.bitwise_or.reduce(w & 0xFF00)) | 0x10,
+        'lxor': int(np.bitwise_xor.reduce(w)) ^ 0x5A5A, 'kmx': int(K.max()), 'kloc': int(K.argmax()),
+        'isum': int(I.sum()) + 7, 'imn': int(I.min()), 'iloc': int(I.argmin()),
+        'umax': max(int(U.max()), 5), 'umin': min(int(U.min()), 0xFFFFFFFF),
+        'fsum': math.fsum(F.astype(np.float64)) + 0.5, 'fmax': float(F.max()),
+        'ties': (int((K == K.max()).sum()), int((I == I.min()).sum())),
+    }
+
+
+def run_redall(n, m, reps, seed):
+    f = _lib('redall').redall
+    f.argtypes = [ctypes.c_long, ctypes.c_long, ctypes.c_int, ctypes.c_ulonglong, _dp, _lp,
+                  ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_uint), ctypes.POINTER(ctypes.c_float)]
+    dres, lres = np.zeros(4), np.zeros(9, dtype=np.int64)
+    ires, ures, fres = np.zeros(3, dtype=np.int32), np.zeros(2, dtype=np.uint32), np.zeros(2, dtype=np.float32)
+    f(n, m, reps, seed, H.dp(dres), lres.ctypes.data_as(_lp), ires.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+      ures.ctypes.data_as(ctypes.POINTER(ctypes.c_uint)), fres.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+    names = ['dsum', 'dprod', 'dmax', 'dmn'], ['dloc', 'lsum', 'lmax', 'lmin', 'land', 'lor', 'lxor', 'kmx', 'kloc'], \
+        ['isum', 'imn', 'iloc'], ['umax', 'umin'], ['fsum', 'fmax']
+    out = {}
+    for ns, arr in zip(names, (dres, lres, ires, ures, fres)):
+        for k, v in zip(ns, arr):
+            out[k] = v.item()
+    return out
+
+
+def _check_redall(n, m, reps=2, seed=D.SEED_BASE + 8):
+    got, ref = run_redall(n, m, reps, seed), redall_reference(n, m, seed)
+    exact = [k for k in got if k not in ('dsum', 'dprod', 'fsum')]
+    assert {k: got[k] for k in exact} == {k: ref[k] for k in exact}
+    # SUM and PRODUCT in FP64 within 1e-12 relative of the exact value; SUM in
+    # FP32 within 1e-4 relative (n * m float additions in the loop's order)
+    assert abs(got['dsum'] - ref['dsum']) <= 1e-12 * abs(ref['dsum'])
+    assert abs(got['dprod'] - ref['dprod']) <= 1e-12 * abs(ref['dprod'])
+    assert abs(got['fsum'] - ref['fsum']) <= 1e-4 * abs(ref['fsum'])
+    return ref
+
+
+@pytest.mark.parametrize('n,m', [(1, 1), (3, 5), (37, 61), (256, 300)])
+def test_redall_host_path(n, m):
+    if D.stats()['gpu_present']:
+        pytest.skip('the host path runs where no GPU is present; the GPU tests cover this case')
+    ref = _check_redall(n, m)
+    if n * m > 1000:
+        assert min(ref['ties']) > 1   # MAXLOC / MINLOC ties are exercised
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('n,m', [(1, 1), (3, 5), (37, 61), (256, 300), (1000, 1037), (2048, 2500)])
+def test_redall_gpu(n, m):
+    """Every reduction function over double, long, int, unsigned and float on
+    the GPU path: one-block and multi-block finishes, element types grouped
+    per finish launch, MAXLOC/MINLOC ties across workgroups."""
+    before = _gpu_loops()
+    ref = _check_redall(n, m)
+    if n * m > 1000:
+        assert min(ref['ties']) > 1
     assert _gpu_loops() > before
