@@ -45,7 +45,15 @@ std::vector<Candidate> makeCandidates(int kind, int loopRank, int depCount, cons
     std::vector<Candidate> v;
     int mapped = loopRank - (kind == DVMH_AMD_KIND_PARALLEL ? 0 : depCount);   // axes spread over threads
     const int tfMax = settings.tuneTransforms ? 1 : 0;   // also try device layout transforms
-    if (kind == DVMH_AMD_KIND_PARALLEL || kind == DVMH_AMD_KIND_STENCIL) {
+    if (kind == DVMH_AMD_KIND_PARALLEL || kind == DVMH_AMD_KIND_STENCIL || kind == DVMH_AMD_KIND_BATCH) {
+        if (kind == DVMH_AMD_KIND_BATCH) {
+            // batch kernels walk many small instances' tiles: wider rows and four
+            // rows per thread (best of the 2-D sweep over 512 grids of sides
+            // 64..1024; 1.3x over the per-loop default below)
+            if (loopRank >= 3) addCand(v, 64, 4, 1, {{DVMH_AMD_T_DEPTH, 4}});
+            else if (loopRank == 2) addCand(v, 128, 4, 1, {{DVMH_AMD_T_DEPTH, 4}});
+            else addCand(v, 256, 1, 1, {});
+        }
         if (loopRank >= 3) addCand(v, 32, 4, 2, {});
         else if (loopRank == 2) addCand(v, 32, 8, 1, {});
         else addCand(v, 256, 1, 1, {});

@@ -966,7 +966,7 @@ def batch_handler(L):
          'long nl = (long)dvmh_amd_batch_count_C(batchRef);',
          'DvmType lr0 = dvmh_amd_batch_loop_C(batchRef, 0);',
          'DvmType tune[DVMH_AMD_TUNE_N];',
-         'dvmh_amd_loop_get_tuning_C(lr0, DVMH_AMD_KIND_PARALLEL, tune);',
+         'dvmh_amd_loop_get_tuning_C(lr0, DVMH_AMD_KIND_BATCH, tune);',
          'dim3 threads(0, 0, 0);',
          'hipStream_t stream;',
          'dvmh_loop_cuda_get_config_C(lr0, 0, 0, &threads, &stream, 0);',

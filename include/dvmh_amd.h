@@ -27,6 +27,7 @@ extern "C" {
 #define DVMH_AMD_KIND_STENCIL 3    /* PARALLEL whose handler also has the LDS march variant
                                       (VARIANT 1: TILE_A = points per thread on the innermost
                                       axis, CHUNK = planes per workgroup on the outermost) */
+#define DVMH_AMD_KIND_BATCH 4      /* PARALLEL loops of many instances run by one batch kernel */
 /* tune[] slots */
 #define DVMH_AMD_T_DEPTH 0     /* iterations of the slowest mapped axis per thread */
 #define DVMH_AMD_T_XCD 1       /* 1: XCD-aware block-to-tile order */

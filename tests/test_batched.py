@@ -62,6 +62,33 @@ def test_gpu_ragged_bit_exact(batched):
         assert d['batch_loops'] == 2 * iters * nonempty
 
 
+KIND_BATCH = 4
+# (block, depth, xcd): the per-loop default shape, a one-row shape with eight
+# rows per thread, a narrow shape with two rows per thread
+BATCH_FORCED = [((32, 8, 1), 1, 0), ((256, 1, 1), 8, 1), ((64, 2, 1), 2, 0)]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('cfg', BATCH_FORCED)
+def test_gpu_batch_configs_bit_exact(cfg):
+    """Batch kernels under forced configurations other than the batch
+    default: the same ragged grids, bit-exact against the oracle."""
+    import ctypes
+    (block, depth, xcd) = cfg
+    r = D.runtime()
+    r.dvmh_amd_tuner_force_C.argtypes = [ctypes.c_long, ctypes.c_void_p, ctypes.c_void_p]
+    b = (ctypes.c_long * 3)(*block)
+    t = (ctypes.c_long * 8)(depth, xcd, 0, 0, 0, 0, 0, 0)
+    r.dvmh_amd_tuner_force_C(KIND_BATCH, ctypes.cast(b, ctypes.c_void_p), ctypes.cast(t, ctypes.c_void_p))
+    try:
+        sides = [2, 3, 67, 5, 130, 64, 257, 33, 1024]
+        res, d = _gpu_run(len(sides), 4, 1, sides=sides)
+        _check(res, H.oracle_batched(sides, 4), 4)
+        assert d['batch_groups'] == 8
+    finally:
+        r.dvmh_amd_tuner_force_C(KIND_BATCH, None, None)
+
+
 @pytest.mark.gpu
 def test_gpu_batched_matches_per_grid_seeded():
     """64 seeded grids, 6 iterations: batched and per-grid forms agree bit for
