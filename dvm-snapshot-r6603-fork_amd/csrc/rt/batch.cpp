@@ -9,9 +9,7 @@
 // group.  Each group runs once: a lone loop through its ordinary handler, a
 // larger group through the batch handler, which sees every loop's bounds and
 // headers and launches one persistent kernel over all of their tiles.
-#include <algorithm>
 #include <cstring>
-#include <set>
 
 #include "rt.h"
 #include "../kernels/launch.h"
@@ -25,7 +23,7 @@ namespace dvmh {
 long graphReplays = 0, batchGroups = 0, batchLoops = 0;
 static bool batching = false;
 static std::vector<Loop *> queued;
-static std::vector<Region *> endedRegions;   // regions that ended while loops were queued
+static long batchEpoch = 0;   // number of the batch being grouped (Data::batchEpoch)
 
 constexpr uint32_t BATCH_MAGIC = 0x42415443u;   // "BATC"
 
@@ -55,7 +53,7 @@ bool loopIsQueued(Loop *loop) { return loop->queued; }
 // Called by dvmh_region_end_C: a region with queued loops lives until they ran.
 bool batchKeepsRegion(Region *r) {
     if (r->pendingBatch == 0) return false;
-    endedRegions.push_back(r);
+    r->endedInBatch = true;
     return true;
 }
 
@@ -68,17 +66,21 @@ int hostSlotCount() {
 #endif
 }
 
-static std::set<const Data *> arraysOf(const Loop *l) {
-    std::set<const Data *> s;
+// Last group of this batch holding a loop that shares an array with l (-1 if none).
+static int lastGroupOf(const Loop *l) {
+    int g = -1;
     if (l->region)
-        for (auto &ra : l->region->arrays) s.insert(ra.data);
-    return s;
+        for (auto &ra : l->region->arrays)
+            if (ra.data->batchEpoch == batchEpoch && ra.data->batchGroup > g) g = ra.data->batchGroup;
+    return g;
 }
 
-static bool disjoint(const std::set<const Data *> &a, const std::set<const Data *> &b) {
-    for (const Data *d : a)
-        if (b.count(d)) return false;
-    return true;
+static void markGroup(const Loop *l, int g) {
+    if (l->region)
+        for (auto &ra : l->region->arrays) {
+            ra.data->batchEpoch = batchEpoch;
+            ra.data->batchGroup = g;
+        }
 }
 
 void finishLoopReductionsExt(Loop *loop);
@@ -113,38 +115,29 @@ static void runBatchGroup(std::vector<Loop *> &group) {
 static void releaseLoop(Loop *l) {
     Region *r = l->region;
     delete l;
-    if (r && --r->pendingBatch == 0) {
-        auto it = std::find(endedRegions.begin(), endedRegions.end(), r);
-        if (it != endedRegions.end()) {
-            endedRegions.erase(it);
-            delete r;
-        }
-    }
+    if (r && --r->pendingBatch == 0 && r->endedInBatch) delete r;
 }
 
 static void runQueue() {
     struct Group {
         DvmHandlerFunc func;
         std::vector<Loop *> loops;
-        std::set<const Data *> arrays;
     };
     std::vector<Group> groups;
+    batchEpoch++;
     for (Loop *l : queued) {
-        std::set<const Data *> mine = arraysOf(l);
         int target = -1;
         for (int g = (int)groups.size() - 1; g >= 0; g--)
             if (groups[g].func == l->batchH.func) { target = g; break; }
-        if (target >= 0) {
-            bool ok = disjoint(mine, groups[target].arrays);
-            for (size_t h = target + 1; h < groups.size() && ok; h++) ok = disjoint(mine, groups[h].arrays);
-            if (!ok) target = -1;
-        }
+        // joining group `target` moves l before the groups after it: allowed
+        // when none of target's or the later groups' loops shares an array
+        if (target >= 0 && lastGroupOf(l) >= target) target = -1;
         if (target < 0) {
-            groups.push_back(Group{l->batchH.func, {}, {}});
+            groups.push_back(Group{l->batchH.func, {}});
             target = (int)groups.size() - 1;
         }
         groups[target].loops.push_back(l);
-        groups[target].arrays.insert(mine.begin(), mine.end());
+        markGroup(l, target);
     }
     queued.clear();
     for (auto &g : groups) {

@@ -140,7 +140,10 @@ struct Data : Object {
     DevBuffer dev;
     bool hasDev = false;
     bool hostActual = true, devActual = false;
-    // pending reduction results, see Loop::deferredPosts
+    // grouping of a closing batch: the last group (of batch number batchEpoch)
+    // holding a loop whose region registers this array
+    long batchEpoch = 0;
+    int batchGroup = -1;
     Data(): Object(Kind::Data) {}
     ~Data() override;
     long elemCount() const;
@@ -175,6 +178,7 @@ struct Region : Object {
     std::vector<RegionArray> scalars;
     std::vector<struct Loop *> deferred;   // GPU loops queued by a lazy (graph) region
     int pendingBatch = 0;                  // loops of this region queued in an open batch
+    bool endedInBatch = false;             // ended while pendingBatch > 0: deleted with its last loop
     Region(): Object(Kind::Region) {}
 };
 

@@ -878,41 +878,39 @@ def batch_desc_layout(L):
 
 
 def batch_kernel(L):
+    """Persistent kernel over all instances' tiles.  The tile body runs in a
+    separate inline function whose array bases are __restrict__ parameters
+    (distinct DVM arrays never overlap), so once inlined the loads of its
+    DEPTH rows may be issued ahead of the stores, as in the per-loop kernels
+    whose bases are kernel parameters."""
     r = L.rank
     lay, dlen = batch_desc_layout(L)
-    fp = ['const long *__restrict__ desc', 'const long *__restrict__ tileStart', 'long nloops', 'long totalTiles']
-    for red in L.reds:
-        fp.append(f'{red[2]} *{red[1]}_grid')
-        if len(red) > 3:
-            fp.append(f'{red[4]} *{red[3]}_grid')
-    k = ['template <typename IndexT, int DEPTH>',
-         f'__global__ __launch_bounds__(1024) void {L.name}_batch_kernel({", ".join(fp)}) {{']
-    b = ['for (long tile = blockIdx.x; tile < totalTiles; tile += gridDim.x) {']
-    t = []
-    t.append('long lo_ = 0, hi_ = nloops - 1;')
-    t.append('while (lo_ < hi_) { long mid = (lo_ + hi_ + 1) >> 1; if (tileStart[mid] <= tile) lo_ = mid; else hi_ = mid - 1; }')
-    t.append(f'const long *d = desc + lo_ * {dlen};')
+    # ---- the tile function
+    tp = []
     for p in L.arrays:
         _, nm, ct, ar = p
-        so = lay[('str', nm)]
-        t.append(f'const IndexT {nm}_str[{ar}] = {{{", ".join(f"(IndexT)d[{so + q}]" for q in range(ar))}}};')
-        t.append(f'const auto {nm} = dvmh_dev::makeView<{ct}, IndexT, {ar}, true>(({ct} *)d[{lay[("base", nm)]}], {nm}_str);')
-    for p in L.scalars:
-        t.append(f'{p[2]} &{p[1]} = *({p[2]} *)d[{lay[("sc", p[1])]}];')
-    bo = lay['bounds']
+        tp.append(f'{ct} *__restrict__ {nm}_base')
+        tp += [f'IndexT {nm}_s{q}' for q in range(ar)]
     for i in range(r):
-        t.append(f'const IndexT lo{i} = (IndexT)d[{bo + 3 * i}], cnt{i} = (IndexT)d[{bo + 3 * i + 1}], st{i} = (IndexT)d[{bo + 3 * i + 2}];')
-    nbo = lay['nb']
-    t.append(f'const long nbx = d[{nbo}], nby = d[{nbo + 1}], nbz = d[{nbo + 2}];')
+        tp += [f'IndexT lo{i}', f'IndexT cnt{i}', f'IndexT st{i}']
+    tp += ['long nbx', 'long nby', 'long nbz', 'long tl']
+    for p in L.scalars:
+        tp.append(f'{p[2]} &{p[1]}')
+    for red in L.reds:
+        tp.append(f'{red[2]} &{red[1]}')
+        if len(red) > 3:
+            tp.append(f'{red[4]} &{red[3]}')
+    f = ['template <typename IndexT, int DEPTH>',
+         f'__device__ static inline void {L.name}_batch_tile({", ".join(tp)}) {{']
+    t = []
+    for p in L.arrays:
+        _, nm, ct, ar = p
+        t.append(f'const IndexT {nm}_str[{ar}] = {{{", ".join(f"{nm}_s{q}" for q in range(ar))}}};')
+        t.append(f'const auto {nm} = dvmh_dev::makeView<{ct}, IndexT, {ar}, true>({nm}_base, {nm}_str);')
     for v, ct in L.vars:
         t.append(f'{ct} {v};')
     for pr in L.privates:
         t.append(pr + ';')
-    for red in L.reds:
-        t.append(f'{red[2]} {red[1]} = *(const {red[2]} *)&d[{lay[("ri", red[1])]}];')
-        if len(red) > 3:
-            t.append(f'{red[4]} {red[3]} = *(const {red[4]} *)&d[{lay[("ri", red[3])]}];')
-    t.append('long tl = tile - tileStart[lo_];')
     t.append('const IndexT bx = (IndexT)(tl % nbx); tl /= nbx;')
     t.append('const IndexT by = (IndexT)(tl % nby); tl /= nby;')
     t.append('const IndexT bz = (IndexT)(tl % nbz); tl /= nbz;')
@@ -934,28 +932,79 @@ def batch_kernel(L):
         else:
             t.append(f'const IndexT t{a} = t{a}_0;')
             conds.append(f't{a} < cnt{a}')
-    inner = []
+    def rows(guarded):
+        inner = []
+        if r >= 2:
+            inner.append(f'const IndexT t{dep_ax} = t{dep_ax}_b + dd;')
+        cs = conds + ([f't{dep_ax} < cnt{dep_ax}'] if r >= 2 else [])
+        if guarded:
+            inner.append(f'if ({" && ".join(cs)}) {{')
+        else:
+            inner.append('{')
+        for i in range(r):
+            inner.append(IND + var_assign(L, i, f't{i}', f'lo{i}', f'st{i}'))
+        inner.append(indent('{\n' + indent(L.body + '\n', 1) + '}', 1))
+        inner.append('}')
+        out = ['#pragma unroll', 'for (int dd = 0; dd < DEPTH; dd++) {']
+        out += [IND + ln for ln in '\n'.join(inner).split('\n')]
+        out.append('}')
+        return out
     if r >= 2:
-        inner.append(f'const IndexT t{dep_ax} = t{dep_ax}_b + dd;')
-        conds = conds + [f't{dep_ax} < cnt{dep_ax}']
-    inner.append(f'if ({" && ".join(conds)}) {{')
-    for i in range(r):
-        inner.append(IND + var_assign(L, i, f't{i}', f'lo{i}', f'st{i}'))
-    inner.append(indent('{\n' + indent(L.body + '\n', 1) + '}', 1))
-    inner.append('}')
-    if r >= 2:
-        t.append('#pragma unroll')
-        t.append('for (int dd = 0; dd < DEPTH; dd++) {')
-        t += [IND + ln for ln in '\n'.join(inner).split('\n')]
+        # a strip wholly inside the bounds runs unguarded: one basic block, so
+        # the loads of all DEPTH rows can be issued before the first store
+        full = conds + [f't{dep_ax}_b + DEPTH <= cnt{dep_ax}']
+        t.append(f'if ({" && ".join(full)}) {{')
+        t += [IND + ln for ln in rows(False)]
+        t.append('} else {')
+        t += [IND + ln for ln in rows(True)]
         t.append('}')
     else:
-        t += inner
+        t.append(f'if ({" && ".join(conds)}) {{')
+        for i in range(r):
+            t.append(IND + var_assign(L, i, f't{i}', f'lo{i}', f'st{i}'))
+        t.append(indent('{\n' + indent(L.body + '\n', 1) + '}', 1))
+        t.append('}')
+    f.append(indent('\n'.join(t) + '\n', 1).rstrip('\n'))
+    f.append('}')
+    # ---- the kernel
+    fp = ['const long *__restrict__ desc', 'const long *__restrict__ tileStart', 'long nloops', 'long totalTiles']
+    for red in L.reds:
+        fp.append(f'{red[2]} *{red[1]}_grid')
+        if len(red) > 3:
+            fp.append(f'{red[4]} *{red[3]}_grid')
+    k = ['template <typename IndexT, int DEPTH>',
+         f'__global__ __launch_bounds__(1024) void {L.name}_batch_kernel({", ".join(fp)}) {{']
+    b = ['for (long tile = blockIdx.x; tile < totalTiles; tile += gridDim.x) {']
+    t = []
+    t.append('long lo_ = 0, hi_ = nloops - 1;')
+    t.append('while (lo_ < hi_) { long mid = (lo_ + hi_ + 1) >> 1; if (tileStart[mid] <= tile) lo_ = mid; else hi_ = mid - 1; }')
+    t.append(f'const long *d = desc + lo_ * {dlen};')
+    args = []
+    for p in L.arrays:
+        _, nm, ct, ar = p
+        so = lay[('str', nm)]
+        args.append(f'dvmh_dev::globalPtr<{ct}>(d[{lay[("base", nm)]}])')
+        args += [f'(IndexT)d[{so + q}]' for q in range(ar)]
+    bo = lay['bounds']
+    for i in range(r):
+        args += [f'(IndexT)d[{bo + 3 * i}]', f'(IndexT)d[{bo + 3 * i + 1}]', f'(IndexT)d[{bo + 3 * i + 2}]']
+    nbo = lay['nb']
+    args += [f'd[{nbo}]', f'd[{nbo + 1}]', f'd[{nbo + 2}]', 'tile - tileStart[lo_]']
+    for p in L.scalars:
+        args.append(f'*dvmh_dev::globalPtr<{p[2]}>(d[{lay[("sc", p[1])]}])')
+    for red in L.reds:
+        t.append(f'{red[2]} {red[1]} = *(const {red[2]} *)&d[{lay[("ri", red[1])]}];')
+        args.append(red[1])
+        if len(red) > 3:
+            t.append(f'{red[4]} {red[3]} = *(const {red[4]} *)&d[{lay[("ri", red[3])]}];')
+            args.append(red[3])
+    t.append(f'{L.name}_batch_tile<IndexT, DEPTH>({", ".join(args)});')
     t += red_kernel_tail(L, 'tile', accumulate=False)
     b += [IND + ln for ln in '\n'.join(t).split('\n')]
     b.append('}')
     k.append(indent('\n'.join(b) + '\n', 1).rstrip('\n'))
     k.append('}')
-    return '\n'.join(k) + '\n'
+    return '\n'.join(f) + '\n\n' + '\n'.join(k) + '\n'
 
 
 def batch_handler(L):

@@ -54,6 +54,15 @@ struct View<T, I, 1, UNIT> {
     __device__ __forceinline__ T &operator[](I i) const { return UNIT ? base[i] : base[i * s[0]]; }
 };
 
+// A device address held as an integer (descriptor tables of batch kernels) as
+// a pointer the compiler knows to be global memory: the cast through address
+// space 1 lets its loads and stores compile to global_* rather than flat_*
+// instructions (no LDS/scratch aperture check, no LGKM counter wait).
+template <typename T>
+__device__ __forceinline__ T *globalPtr(long a) {
+    return (T *)(__attribute__((address_space(1))) T *)a;
+}
+
 template <typename T, typename I, int R, bool UNIT>
 __device__ __forceinline__ View<T, I, R, UNIT> makeView(T *base, const I *strides) {
     View<T, I, R, UNIT> v;
