@@ -113,6 +113,18 @@ std::vector<Candidate> makeCandidates(int kind, int loopRank, int depCount, cons
                     for (int tf = 0; tf <= tfMax; tf++)
                         addCand(v, bx, by, 1, {{DVMH_AMD_T_XCD, xcd}, {DVMH_AMD_T_TRANSFORM, tf}});
             }
+        if (depMask > 1 && __builtin_popcountl((unsigned long)depMask) == 1 && (loopRank == 2 || loopRank == 3)) {
+            // register-staged line sweep (dependent axis not innermost): threads
+            // over the independent axes, TILE_A steps per chunk held in registers
+            const int rbs[][2] = {{64, 1}, {128, 1}, {256, 1}, {64, 2}, {32, 4}};
+            const int rcks[] = {8, 16};
+            for (auto &bs : rbs) {
+                if (mapped < 2 && bs[1] > 1) continue;
+                for (int ck : rcks)
+                    for (int xcd = 0; xcd <= 1; xcd++)
+                        addCand(v, bs[0], bs[1], 1, {{DVMH_AMD_T_VARIANT, 2}, {DVMH_AMD_T_TILE_A, ck}, {DVMH_AMD_T_XCD, xcd}});
+            }
+        }
         if (depMask == 1 && (loopRank == 2 || loopRank == 3)) {
             // tiled line sweep (dependent axis innermost): blockDim.x lines per
             // workgroup, TILE_A iterations per staged chunk; LDS of two chunks

@@ -148,11 +148,15 @@ class Loop:
         return True
 
     def line_tile_eligible(self):
+        """A line along the innermost axis can be staged in LDS and walked by
+        one thread (the tiled line sweep)."""
+        return self.line_stage_eligible(self.rank - 1)
+
+    def line_stage_eligible(self, axis):
         """Rank 2 or 3, and every access of the ACROSS array in the body is
-        X[i][j][k + d] with the loop's own variables, offset 0 on all axes but
-        the innermost, |d| <= 1 there, and d = 0 where it is written: a line
-        along the innermost axis can then be staged in LDS and walked by one
-        thread (the tiled line sweep)."""
+        X[i][j][k] with the loop's own variables, offset 0 on all axes but
+        `axis`, |d| <= 1 there, and d = 0 where it is written: a thread's line
+        along `axis` can then be staged (LDS or registers) chunk by chunk."""
         if not self.across or self.rank not in (2, 3):
             return False
         name = self.across['array']
@@ -172,7 +176,7 @@ class Loop:
                 if not mm:
                     return False
                 moved = mm.group(2) == '1'
-                if moved and (a != self.rank - 1 or m.group(2)):
+                if moved and (a != axis or m.group(2)):
                     return False
         return True
 
@@ -1157,6 +1161,115 @@ def line_kernel(L, dep):
     return '\n'.join(k) + '\n'
 
 
+LINE_REG_CK = [8, 16]
+
+
+def line_reg_kernel(L, dep):
+    """ACROSS with one dependent axis `dep` other than the innermost (threads
+    over the independent axes, coalesced along the innermost one): each thread
+    walks its line as in line_kernel, but in chunks of CK steps.  The chunk's
+    elements of the ACROSS array (and the one ahead) are loaded together into
+    a local buffer -- CK + 1 loads in flight instead of one dependent load per
+    step -- the body runs against the buffer (RegLineView), and the chunk is
+    stored back.  The last element of a full chunk is carried into the next.
+    Steps are unrolled, so slot indices are constants and the buffer lives in
+    registers.  Loads are clamped into the indices the body reads."""
+    r = L.rank
+    ind = [a for a in range(r) if a != dep]
+    X = L.across['array']
+    xct = [p for p in L.arrays if p[1] == X][0][2]
+    dmin, dmax = L.stencil_range(X)[dep]
+    fp, setup = kernel_array_params(L, unit=False)
+    for i in range(r):
+        fp += [f'IndexT lo{i}', f'IndexT cnt{i}', f'IndexT st{i}']
+    fp += ['IndexT nbx', 'IndexT nby', 'long nblocks', 'int xcd']
+    fp += red_kernel_params(L)
+    k = ['template <typename IndexT, int CK, int DIR>',
+         f'__global__ __launch_bounds__(256) void {L.name}_lreg{dep}_kernel({", ".join(fp)}) {{']
+    b = list(setup)
+    for v, ct in L.vars:
+        b.append(f'{ct} {v};')
+    for pr in L.privates:
+        b.append(pr + ';')
+    b += red_kernel_locals(L)
+    b.append('long tile = dvmh_dev::xcdTile((long)blockIdx.x, nblocks, xcd);')
+    b.append('const IndexT bx = (IndexT)(tile % nbx); tile /= nbx;')
+    b.append('const IndexT by = (IndexT)(tile % nby); tile /= nby;')
+    ax_x = ind[-1]
+    ax_y = ind[-2] if len(ind) >= 2 else None
+    conds = []
+    b.append(f'const IndexT t{ax_x} = bx * (IndexT)blockDim.x + (IndexT)threadIdx.x;')
+    conds.append(f't{ax_x} < cnt{ax_x}')
+    if ax_y is not None:
+        b.append(f'const IndexT t{ax_y} = by * (IndexT)blockDim.y + (IndexT)threadIdx.y;')
+        conds.append(f't{ax_y} < cnt{ax_y}')
+    conds.append(f'cnt{dep} > 0')   # an empty line reads nothing (the clamp range would be empty)
+    b.append(f'if ({" && ".join(conds)}) {{')
+    inner = []
+    for a in ind:
+        inner.append(var_assign(L, a, f't{a}', f'lo{a}', f'st{a}'))
+    inner.append(f'{xct} *const lr_row = {X}_base' + ''.join(f' + (IndexT){L.vars[a][0]} * {X}_s{a}' for a in ind) +
+                 ';   // this thread\'s line')
+    inner.append(f'const IndexT lr_first = lo{dep}, lr_last = lo{dep} + (cnt{dep} - 1) * (IndexT)DIR;')
+    inner.append(f'// indices along the line the body reads (offsets {dmin} .. {dmax}): loads are clamped into them')
+    inner.append(f'const IndexT lr_rlo = (DIR > 0 ? lr_first : lr_last) + ({dmin}), lr_rhi = (DIR > 0 ? lr_last : lr_first) + ({dmax});')
+    inner.append(f'auto lr_ld = [&](IndexT x) -> {xct} {{ x = x < lr_rlo ? lr_rlo : (x > lr_rhi ? lr_rhi : x); return lr_row[x * {X}_s{dep}]; }};')
+    inner.append(f'{xct} lr_buf[CK + 2];   // slot 1 + s: step s of the chunk; 0: the element behind; CK + 1: ahead')
+    inner.append('lr_buf[0] = lr_ld(lr_first - (IndexT)DIR);')
+    inner.append(f'for (IndexT lr_c0 = 0; lr_c0 < cnt{dep}; lr_c0 += CK) {{')
+    ch = []
+    ch.append(f'const IndexT lr_n = cnt{dep} - lr_c0 < CK ? cnt{dep} - lr_c0 : CK;')
+    ch.append(f'const IndexT lr_x0 = lo{dep} + lr_c0 * (IndexT)DIR;   // array index of slot 1')
+    ch.append('#pragma unroll')
+    ch.append('for (int m = 1; m <= CK + 1; m++) lr_buf[m] = lr_ld(lr_x0 + (IndexT)(m - 1) * (IndexT)DIR);')
+    ch.append(f'const dvmh_dev::RegLineView<{xct}, IndexT, {r}, {dep}, DIR> {X}{{lr_buf, lr_x0, 0}};')
+    ch.append('#pragma unroll')
+    ch.append('for (int lr_s = 0; lr_s < CK; lr_s++) {')
+    ch.append(IND + 'if (lr_s < lr_n) {')
+    ch.append(IND * 2 + f'const IndexT t{dep} = lr_c0 + lr_s;')
+    ch.append(IND * 2 + var_assign(L, dep, f't{dep}', f'lo{dep}', '(IndexT)DIR'))
+    ch.append(indent('{\n' + indent(L.body + '\n', 1) + '}', 2))
+    ch.append(IND + '}')
+    ch.append('}')
+    ch.append('#pragma unroll')
+    ch.append(f'for (int m = 1; m <= CK; m++) if (m <= lr_n) lr_row[(lr_x0 + (IndexT)(m - 1) * (IndexT)DIR) * {X}_s{dep}] = lr_buf[m];')
+    ch.append('lr_buf[0] = lr_buf[CK];   // carried (only a full chunk has a successor)')
+    inner += [IND + ln for ln in '\n'.join(ch).split('\n')]
+    inner.append('}')
+    b += [IND + ln for ln in '\n'.join(inner).split('\n')]
+    b.append('}')
+    b += red_kernel_tail(L, 'blockIdx.x', accumulate=False)
+    k.append(indent('\n'.join(b) + '\n', 1).rstrip('\n'))
+    k.append('}')
+    return '\n'.join(k) + '\n'
+
+
+def line_reg_launch(L, dep):
+    r = L.rank
+    ind = [a for a in range(r) if a != dep]
+    ax_x = ind[-1]
+    ax_y = ind[-2] if len(ind) >= 2 else None
+    b = [f'DvmType nbx = (cnt{ax_x} + threads.x - 1) / threads.x;',
+         f'DvmType nby = {"(cnt%d + threads.y - 1) / threads.y" % ax_y if ax_y is not None else "1"};',
+         'DvmType nblocks = nbx * nby;']
+    b += handler_reds_prepare(L, 'nblocks', 0)
+    b.append('if (nblocks > 0) {')
+    b.append(IND + 'dvmh_amd_mark_kernel_C(loop_ref, 0);')
+    args = array_fact_args(L, 'int')
+    for i in range(r):
+        args += [f'(int)boundsLow[{i}]', f'(int)cnt{i}', f'(int)loopSteps[{i}]']
+    args += ['(int)nbx', '(int)nby', 'nblocks', '(int)tune[DVMH_AMD_T_XCD]']
+    args += red_fact_args(L)
+    for ck in LINE_REG_CK:
+        for d in (1, -1):
+            b.append(IND + f'if (CK == {ck} && loopSteps[{dep}] == {d}) hipLaunchKernelGGL(({L.name}_lreg{dep}_kernel<int, {ck}, {d}>), '
+                     f'dim3((unsigned)nblocks), threads, 0, stream, {", ".join(args)});')
+    b.append(IND + 'dvmh_amd_mark_kernel_C(loop_ref, 1);')
+    b.append('}')
+    b += handler_reds_finish(L)
+    return b
+
+
 def line_launch(L, dep):
     r = L.rank
     ind = [a for a in range(r) if a != dep]
@@ -1604,6 +1717,18 @@ def across_handler(L):
             b.append(IND + '} else {')
             b += [IND * 2 + s for s in line_launch(L, dep)]
             b.append(IND + '}')
+        elif dep < r - 1 and L.line_stage_eligible(dep):
+            X = L.across['array']
+            cks = ' || '.join(f'CK == {ck}' for ck in LINE_REG_CK)
+            b.append(IND + '// register-staged line sweep: unit steps, natural layout, 32-bit indices')
+            b.append(IND + 'DvmType CK = tune[DVMH_AMD_T_TILE_A];')
+            b.append(IND + f'bool lreg = tune[DVMH_AMD_T_VARIANT] == 2 && (loopSteps[{dep}] == 1 || loopSteps[{dep}] == -1) && '
+                     f'{X}_tfm == 0 && kernelIndexT == rt_INT && threads.x * threads.y * threads.z <= 256 && ({cks});')
+            b.append(IND + 'if (lreg) {')
+            b += [IND * 2 + s for s in line_reg_launch(L, dep)]
+            b.append(IND + '} else {')
+            b += [IND * 2 + s for s in line_launch(L, dep)]
+            b.append(IND + '}')
         else:
             b += [IND + s for s in line_launch(L, dep)]
     # two or more dependent axes: hyperplane kernels for every mask with >= 2 bits
@@ -1648,6 +1773,9 @@ def gpu_loop(L):
             text.append(tile_kernel(L))
         if L.line_tile_eligible():
             text.append(line_tile_kernel(L))
+        for dep in range(r - 1):
+            if L.line_stage_eligible(dep):
+                text.append(line_reg_kernel(L, dep))
         text.append(across_handler(L))
     else:
         text.append(parallel_kernel(L))

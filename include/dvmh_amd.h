@@ -22,7 +22,10 @@ extern "C" {
  * of iterations to threads. */
 #define DVMH_AMD_TUNE_N 8
 #define DVMH_AMD_KIND_PARALLEL 0   /* independent iterations, one kernel */
-#define DVMH_AMD_KIND_LINE 1       /* ACROSS with one dependent axis: line sweeps */
+#define DVMH_AMD_KIND_LINE 1       /* ACROSS with one dependent axis: line sweeps
+                                      (VARIANT 1, dependent axis innermost: lines staged in
+                                      LDS chunks of TILE_A steps; VARIANT 2, other axes: each
+                                      thread's line in register chunks of TILE_A steps) */
 #define DVMH_AMD_KIND_WAVEFRONT 2  /* ACROSS with >= 2 dependent axes: hyperplane wavefront */
 #define DVMH_AMD_KIND_STENCIL 3    /* PARALLEL whose handler also has the LDS march variant
                                       (VARIANT 1: TILE_A = points per thread on the innermost
@@ -33,7 +36,7 @@ extern "C" {
 #define DVMH_AMD_T_XCD 1       /* 1: XCD-aware block-to-tile order */
 #define DVMH_AMD_T_VARIANT 2   /* kernel variant (0 = direct global-memory body) */
 #define DVMH_AMD_T_CHUNK 3     /* LINE: steps per LDS window; WAVEFRONT: tile depth */
-#define DVMH_AMD_T_TILE_A 4    /* WAVEFRONT: tile extent on the first dependent axis */
+#define DVMH_AMD_T_TILE_A 4    /* WAVEFRONT: tile extent on the first dependent axis; LINE: chunk steps */
 #define DVMH_AMD_T_TILE_B 5    /* WAVEFRONT: tile extent on the second dependent axis */
 #define DVMH_AMD_T_TRANSFORM 6 /* 1: arrays may be re-laid out for this loop */
 #define DVMH_AMD_T_RESERVED 7

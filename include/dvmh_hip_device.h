@@ -165,6 +165,28 @@ struct LineView<T, I, 1> {
     __device__ __forceinline__ T &operator[](I k) const { return p[(k - kb) * dir]; }
 };
 
+// A thread's chunk of its own line of an ACROSS array held in a local buffer
+// (the line sweep with register staging, lines along a non-innermost axis):
+// for X[i][j][k] with the loop's own indices, the subscript on axis AX selects
+// slot (x - x0) * DIR + 1 of the buffer, the others select nothing (they are
+// the thread's line).  With the chunk's steps unrolled every slot index is a
+// compile-time constant, so the buffer is promoted to registers.
+template <typename T, typename I, int D, int AX, int DIR>
+struct RegLineView {
+    T *buf;
+    I x0, slot;
+    __device__ __forceinline__ RegLineView<T, I, D - 1, AX - 1, DIR> operator[](I x) const {
+        return RegLineView<T, I, D - 1, AX - 1, DIR>{buf, x0, AX == 0 ? (x - x0) * DIR + 1 : slot};
+    }
+};
+
+template <typename T, typename I, int AX, int DIR>
+struct RegLineView<T, I, 1, AX, DIR> {
+    T *buf;
+    I x0, slot;
+    __device__ __forceinline__ T &operator[](I x) const { return buf[AX == 0 ? (x - x0) * DIR + 1 : slot]; }
+};
+
 // Diagonal position inside an Rx x Ry plane (x fastest).  Also compiled for
 // the host, so the layout-transform kernels, the handlers and host-side checks
 // share one definition.

@@ -167,6 +167,49 @@ def test_adi3d_tiled_line_sweep_bit_exact(cfg, n, steps):
     assert np.array_equal(res['max'], ref['max']) and np.array_equal(res['loc'], ref['loc'])
 
 
+def _lreg(bx, by, ck, xcd=0):
+    tune = [0] * 8
+    tune[T_VARIANT] = 2
+    tune[T_TILE_A] = ck
+    tune[T_XCD] = xcd
+    return (bx, by, 1), tune
+
+
+LREGS = [(64, 1, 8, 0), (128, 1, 16, 1), (256, 1, 8, 0), (32, 4, 16, 0), (64, 2, 8, 1)]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('cfg', LREGS)
+@pytest.mark.parametrize('n,steps', [(3, 2), (6, 2), (19, 2), (64, 3)])
+def test_adi3d_register_line_sweep_bit_exact(cfg, n, steps):
+    """The four sweeps along i and j (forward and backward) use the
+    register-staged variant (the two along k fall back to the direct line
+    kernel with the same block); n = 3 and 6 give one partial chunk, 19 and 64
+    several with a partial last one for CK = 8 and 16."""
+    block, tune = _lreg(*cfg)
+    _force(KIND_LINE, block, tune)
+    try:
+        res = H.run_adi3d(n, steps)
+    finally:
+        _force(KIND_LINE, None, None)
+    ref = H.oracle_adi3d(n, steps)
+    assert H.bits_equal(res['A'], ref['A'])
+    assert np.array_equal(res['max'], ref['max']) and np.array_equal(res['loc'], ref['loc'])
+
+
+@pytest.mark.gpu
+def test_adi3d_register_line_sweep_full_size():
+    """256^3 (the ADI workload's size), two steps, register-staged i and j sweeps."""
+    block, tune = _lreg(64, 1, 16)
+    _force(KIND_LINE, block, tune)
+    try:
+        res = H.run_adi3d(256, 2)
+    finally:
+        _force(KIND_LINE, None, None)
+    ref = H.oracle_adi3d(256, 2)
+    assert H.bits_equal(res['A'], ref['A']) and np.array_equal(res['loc'], ref['loc'])
+
+
 @pytest.mark.gpu
 def test_adi3d_tiled_line_sweep_full_size():
     """256^3 (the ADI workload's size), two steps."""
