@@ -134,7 +134,8 @@ std::vector<Candidate> makeCandidates(int kind, int loopRank, int depCount, cons
             for (int tl : tls)
                 for (int ck : cks) {
                     if ((long)tl * (2 * ((ck + 2) | 1) * 8 + 8) + 16 > 64512) continue;
-                    addCand(v, tl, 1, 1, {{DVMH_AMD_T_VARIANT, 1}, {DVMH_AMD_T_TILE_A, ck}});
+                    for (int reg = 0; reg <= 1; reg++)   // TILE_B 1: the walk runs on a register copy of the line's chunk
+                        addCand(v, tl, 1, 1, {{DVMH_AMD_T_VARIANT, 1}, {DVMH_AMD_T_TILE_A, ck}, {DVMH_AMD_T_TILE_B, reg}});
                 }
         }
     } else {

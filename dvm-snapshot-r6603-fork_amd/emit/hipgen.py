@@ -1528,7 +1528,7 @@ def line_tile_kernel(L):
     kx = r - 1
     # at most 256 threads (the handler checks): lines are few, so registers
     # for deep prefetch matter more than occupancy
-    k = ['template <typename IndexT, int CK>',
+    k = ['template <typename IndexT, int CK, int DIR, int REG>',
          f'__global__ __launch_bounds__(256) void {L.name}_ltile_kernel({", ".join(fp)}) {{']
     b = ['extern __shared__ __align__(16) unsigned char dvmh_lds[];']
     b += setup
@@ -1587,7 +1587,26 @@ def line_tile_kernel(L):
           IND * 2 + 'const IndexT row0 = e / dl_w1, sl = 1 + (e - row0 * dl_w1);',
           IND * 2 + f'dl_pre[m] = dl_load(dl_kb + (CK + sl) * st{kx}, row0 < dl_nl ? row0 : dl_nl - 1);',
           IND + '}',
-          IND + 'if (dl_mine) {',
+          IND + 'if (dl_mine && REG) {',
+          IND * 2 + '// the line\'s slots in registers: the recurrence runs at register latency',
+          IND * 2 + f'{xct} *dl_line = dl_cur + dl_tid * dl_RS;',
+          IND * 2 + f'{xct} dl_reg[CK + 2];',
+          IND * 2 + '#pragma unroll',
+          IND * 2 + 'for (int m = 0; m < CK + 2; m++) dl_reg[m] = dl_line[m];',
+          IND * 2 + f'const IndexT dl_x0 = lo{kx} + dl_c0 * (IndexT)DIR;   // array index of slot 1',
+          IND * 2 + f'const dvmh_dev::RegLineView<{xct}, IndexT, {r}, {kx}, DIR> {X}{{dl_reg, dl_x0, 0}};',
+          IND * 2 + '#pragma unroll',
+          IND * 2 + 'for (int dl_s = 0; dl_s < CK; dl_s++) {',
+          IND * 3 + 'if (dl_s < dl_n) {',
+          IND * 4 + f'const IndexT t{kx} = dl_c0 + dl_s;']
+    for i in range(r):
+        b.append(IND * 4 + var_assign(L, i, f't{i}', f'lo{i}', '(IndexT)DIR' if i == kx else f'st{i}'))
+    b.append(indent('{\n' + indent(L.body + '\n', 1) + '}', 4))
+    b += [IND * 3 + '}',
+          IND * 2 + '}',
+          IND * 2 + '#pragma unroll',
+          IND * 2 + 'for (int m = 1; m <= CK; m++) dl_line[m] = dl_reg[m];',
+          IND + '} else if (dl_mine) {',
           IND * 2 + f'const dvmh_dev::LineView<{xct}, IndexT, {r}> {X}{{dl_cur + dl_tid * dl_RS, dl_kb, st{kx}}};',
           IND * 2 + 'for (IndexT dl_i = 1; dl_i <= dl_n; dl_i++) {',
           IND * 3 + f'const IndexT t{kx} = dl_c0 + dl_i - 1;']
@@ -1595,8 +1614,8 @@ def line_tile_kernel(L):
         b.append(IND * 3 + var_assign(L, i, f't{i}', f'lo{i}', f'st{i}'))
     b.append(indent('{\n' + indent(L.body + '\n', 1) + '}', 3))
     b += [IND * 2 + '}',
-          IND * 2 + 'dl_nxt[dl_tid * dl_RS] = dl_cur[dl_tid * dl_RS + dl_n];   // carried into the next chunk',
           IND + '}',
+          IND + 'if (dl_mine) dl_nxt[dl_tid * dl_RS] = dl_cur[dl_tid * dl_RS + dl_n];   // carried into the next chunk',
           IND + '__syncthreads();',
           IND + '// write back this chunk (slots 1 .. n), commit the next one',
           IND + 'for (IndexT e = dl_tid; e < dl_nl * dl_n; e += dl_TL) {',
@@ -1633,9 +1652,12 @@ def line_tile_launch(L):
         args += [f'({it})nlines']
         args += red_fact_args(L)
         for ck in LINE_TILE_CK:
-            cond = f'kernelIndexT == {"rt_INT" if it == "int" else "rt_LLONG"} && CK == {ck}'
-            b.append(IND + f'if ({cond}) hipLaunchKernelGGL(({L.name}_ltile_kernel<{it}, {ck}>), dim3((unsigned)nblocks), threads, '
-                     f'ldsBytes, stream, {", ".join(args)});')
+            for d in (1, -1):
+                for reg in ((0, 1) if it == 'int' else (0,)):
+                    cond = (f'kernelIndexT == {"rt_INT" if it == "int" else "rt_LLONG"} && CK == {ck} && '
+                            f'loopSteps[{r - 1}] == {d} && REG == {reg}')
+                    b.append(IND + f'if ({cond}) hipLaunchKernelGGL(({L.name}_ltile_kernel<{it}, {ck}, {d}, {reg}>), '
+                             f'dim3((unsigned)nblocks), threads, ldsBytes, stream, {", ".join(args)});')
     b.append(IND + 'dvmh_amd_mark_kernel_C(loop_ref, 1);')
     b.append('}')
     b += handler_reds_finish(L)
@@ -1707,7 +1729,8 @@ def across_handler(L):
             X = L.across['array']
             xct = [p for p in L.arrays if p[1] == X][0][2]
             b.append(IND + '// tiled line sweep: unit steps along the line, natural layout, chunk fits 64 KB of LDS')
-            b.append(IND + 'DvmType CK = tune[DVMH_AMD_T_TILE_A], RS = (CK + 2) | 1;')
+            b.append(IND + '// REG (TILE_B = 1): the walk on a register copy, 32-bit indices only (the slot arithmetic must fold)')
+            b.append(IND + 'DvmType CK = tune[DVMH_AMD_T_TILE_A], RS = (CK + 2) | 1, REG = tune[DVMH_AMD_T_TILE_B] && kernelIndexT == rt_INT ? 1 : 0;')
             b.append(IND + f'size_t ldsBytes = (((size_t)2 * threads.x * RS * sizeof({xct}) + 15) & ~(size_t)15) + (size_t)threads.x * sizeof(DvmType);')
             cks = ' || '.join(f'CK == {ck}' for ck in LINE_TILE_CK)
             b.append(IND + f'bool ltile = tune[DVMH_AMD_T_VARIANT] == 1 && (loopSteps[{r - 1}] == 1 || loopSteps[{r - 1}] == -1) && '

@@ -139,14 +139,15 @@ def test_sor3d_tiled_full_size():
 KIND_LINE = 1
 
 
-def _ltile(tl, ck):
+def _ltile(tl, ck, reg=0):
     tune = [0] * 8
     tune[T_VARIANT] = 1
     tune[T_TILE_A] = ck
+    tune[5] = reg   # TILE_B: the walk on a register copy of the chunk
     return (tl, 1, 1), tune
 
 
-LTILES = [(64, 8), (128, 16), (256, 8), (64, 32), (128, 8)]
+LTILES = [(64, 8, 0), (128, 16, 0), (256, 8, 0), (64, 32, 0), (128, 8, 0), (64, 8, 1), (128, 16, 1), (256, 32, 1)]
 
 
 @pytest.mark.gpu
@@ -212,8 +213,8 @@ def test_adi3d_register_line_sweep_full_size():
 
 @pytest.mark.gpu
 def test_adi3d_tiled_line_sweep_full_size():
-    """256^3 (the ADI workload's size), two steps."""
-    block, tune = _ltile(128, 16)
+    """256^3 (the ADI workload's size), two steps, the walk on registers."""
+    block, tune = _ltile(128, 16, 1)
     _force(KIND_LINE, block, tune)
     try:
         res = H.run_adi3d(256, 2)
