@@ -317,6 +317,8 @@ def main():
     ap.add_argument('--skip-others', action='store_true', help='omit the batched-grid and other-config legs')
     ap.add_argument('--chosen-only', action='store_true',
                     help='run only the chosen form (for profiling runs); prints the JSON line without the other legs')
+    ap.add_argument('--others-only', action='store_true',
+                    help='run only the batched-grid and other-config legs (for profiling runs); prints their JSON')
     ap.add_argument('--calibrate-copy', action='store_true',
                     help='also run one 1 GiB device-to-device copy (scale reference for HBM counters)')
     args = ap.parse_args()
@@ -333,6 +335,13 @@ def main():
     log('device', D.device_name(), 'CUs', st['cu_count'])
     if args.calibrate_copy:
         calibrate_copy()
+    if args.others_only:
+        batched = batched_leg(rt, 20)
+        log('batched', batched)
+        others = other_configs(rt)
+        log('others', others)
+        print(json.dumps({'batched_grids': batched, 'other_configs': others}), flush=True)
+        return
     bench = Jac3d(rt)
     plain = exhaustive = None
     if not args.chosen_only:
