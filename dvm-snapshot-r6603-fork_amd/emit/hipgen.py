@@ -1561,8 +1561,8 @@ def line_tile_kernel(L):
           IND + f'return {X}_base[dl_rowOff[row] + kk * {X}_s{kx}];',
           '};',
           f'const IndexT dl_nchunks = (cnt{kx} + CK - 1) / CK;',
-          '// chunk 0: slots 0 .. n + 1 straight into buffer 0',
-          '{',
+          '// chunk 0: slots 0 .. n + 1 straight into buffer 0 (none for an empty line: nothing is read)',
+          'if (dl_nchunks > 0) {',
           IND + f'const IndexT n = cnt{kx} < CK ? cnt{kx} : CK, w = n + 2;',
           IND + f'const IndexT kb = lo{kx} - st{kx};',
           IND + 'for (IndexT e = dl_tid; e < dl_nl * w; e += dl_TL) {',
