@@ -50,8 +50,11 @@ std::vector<Candidate> makeCandidates(int kind, int loopRank, int depCount, cons
             // batch kernels walk many small instances' tiles: wider rows and four
             // rows per thread (best of the 2-D sweep over 512 grids of sides
             // 64..1024; 1.3x over the per-loop default below)
+            // (rank 2: a batch of stencil loops runs its handler's own march
+            // shape instead, DVMH_AMD_T_OWN: 256 threads, 4 points each, 8 rows
+            // per tile, 1.4x the direct tiles on the same sweep)
             if (loopRank >= 3) addCand(v, 64, 4, 1, {{DVMH_AMD_T_DEPTH, 4}});
-            else if (loopRank == 2) addCand(v, 128, 4, 1, {{DVMH_AMD_T_DEPTH, 4}});
+            else if (loopRank == 2) addCand(v, 128, 4, 1, {{DVMH_AMD_T_DEPTH, 4}, {DVMH_AMD_T_OWN, 1}});
             else addCand(v, 256, 1, 1, {});
         }
         if (loopRank >= 3) addCand(v, 32, 4, 2, {});
@@ -74,7 +77,9 @@ std::vector<Candidate> makeCandidates(int kind, int loopRank, int depCount, cons
                             addCand(v, bx, by, bz, {{DVMH_AMD_T_DEPTH, dp}, {DVMH_AMD_T_XCD, xcd}});
                     }
                 }
-        if (kind == DVMH_AMD_KIND_STENCIL && (loopRank == 2 || loopRank == 3)) {
+        if ((kind == DVMH_AMD_KIND_STENCIL || kind == DVMH_AMD_KIND_BATCH) && (loopRank == 2 || loopRank == 3)) {
+            // (a batch of stencil loops runs the same march per tile; loops without
+            // a stencil array ignore the variant and run the direct tiles)
             // LDS march variant: block (bx, by), TILE_A points per thread on the
             // innermost axis, CHUNK planes per workgroup; three planes of
             // (by + 2) x (bx * TILE_A + 2) FP64 elements must fit 64 KB of LDS

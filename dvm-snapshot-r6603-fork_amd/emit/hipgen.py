@@ -605,18 +605,22 @@ def march_kernel(L):
         setup.append(f'{p[2]} &{p[1]} = *{p[1]}_ptr;')
     for i in range(r):
         fp += [f'IndexT lo{i}', f'IndexT cnt{i}', f'IndexT st{i}']
-    fp += ['IndexT nbx', 'IndexT nby', 'IndexT nbz', 'long nblocks', 'int xcd', 'IndexT march']
-    fp += red_kernel_params(L)
+    fp += ['IndexT nbx', 'IndexT nby', 'IndexT nbz', 'IndexT march']
+    # the march over one tile is a device function: the per-loop kernel calls it
+    # for its workgroup's tile, the batch kernel for every tile it walks
+    tp = list(fp) + ['long tile']
+    for red in L.reds:
+        tp.append(f'{red[2]} &{red[1]}')
+        if len(red) > 3:
+            tp.append(f'{red[4]} &{red[3]}')
     k = ['template <typename IndexT, int COLS, int NPF>',
-         f'__global__ __launch_bounds__(1024) void {L.name}_march_kernel({", ".join(fp)}) {{']
+         f'__device__ static inline void {L.name}_march_tile({", ".join(tp)}) {{']
     b = ['extern __shared__ __align__(16) unsigned char dvmh_lds[];']
     b += setup
     for v, ct in L.vars:
         b.append(f'{ct} {v};')
     for pr in L.privates:
         b.append(pr + ';')
-    b += red_kernel_locals(L)
-    b.append('long tile = dvmh_dev::xcdTile((long)blockIdx.x, nblocks, xcd);')
     b.append('const IndexT bx = (IndexT)(tile % nbx); tile /= nbx;')
     b.append('const IndexT by = (IndexT)(tile % nby); tile /= nby;')
     b.append('const IndexT bz = (IndexT)tile;')
@@ -746,10 +750,32 @@ def march_kernel(L):
     inner.append('__syncthreads();')
     b += [IND + ln for ln in '\n'.join(inner).split('\n')]
     b.append('}')
-    b += red_kernel_tail(L, 'blockIdx.x', accumulate=False)
     k.append(indent('\n'.join(b) + '\n', 1).rstrip('\n'))
     k.append('}')
-    return '\n'.join(k) + '\n'
+    # the per-loop kernel: one tile per workgroup
+    names = [p_[1] for p_ in L.params]
+    kp = list(fp[:-4]) + ['IndexT nbx', 'IndexT nby', 'IndexT nbz', 'long nblocks', 'int xcd', 'IndexT march'] + red_kernel_params(L)
+    call = []
+    for p_ in L.arrays:
+        _, nm, ct, ar = p_
+        call += [f'{nm}_base'] + [f'{nm}_s{q}' for q in range(ar)]
+    for p_ in L.scalars:
+        call.append(f'{p_[1]}_ptr')
+    for i in range(r):
+        call += [f'lo{i}', f'cnt{i}', f'st{i}']
+    call += ['nbx', 'nby', 'nbz', 'march', 'dvmh_dev::xcdTile((long)blockIdx.x, nblocks, xcd)']
+    for red in L.reds:
+        call.append(red[1])
+        if len(red) > 3:
+            call.append(red[3])
+    g = ['template <typename IndexT, int COLS, int NPF>',
+         f'__global__ __launch_bounds__(1024) void {L.name}_march_kernel({", ".join(kp)}) {{']
+    gb = red_kernel_locals(L)
+    gb.append(f'{L.name}_march_tile<IndexT, COLS, NPF>({", ".join(call)});')
+    gb += red_kernel_tail(L, 'blockIdx.x', accumulate=False)
+    g.append(indent('\n'.join(gb) + '\n', 1).rstrip('\n'))
+    g.append('}')
+    return '\n'.join(k) + '\n\n' + '\n'.join(g) + '\n'
 
 
 def parallel_handler(L):
@@ -1011,6 +1037,51 @@ def batch_kernel(L):
     return '\n'.join(f) + '\n\n' + '\n'.join(k) + '\n'
 
 
+def batch_march_kernel(L):
+    """Persistent kernel over all instances' march tiles of a stencil loop:
+    each tile (a strip of the inner axes times `march` planes of the outermost)
+    runs the LDS march of the per-loop march kernel (march_tile)."""
+    r = L.rank
+    lay, dlen = batch_desc_layout(L)
+    fp = ['const long *__restrict__ desc', 'const long *__restrict__ tileStart', 'long nloops', 'long totalTiles', 'IndexT march']
+    for red in L.reds:
+        fp.append(f'{red[2]} *{red[1]}_grid')
+        if len(red) > 3:
+            fp.append(f'{red[4]} *{red[3]}_grid')
+    k = ['template <typename IndexT, int COLS, int NPF>',
+         f'__global__ __launch_bounds__(1024) void {L.name}_batch_march_kernel({", ".join(fp)}) {{']
+    b = ['for (long tile = blockIdx.x; tile < totalTiles; tile += gridDim.x) {']
+    t = ['long lo_ = 0, hi_ = nloops - 1;',
+         'while (lo_ < hi_) { long mid = (lo_ + hi_ + 1) >> 1; if (tileStart[mid] <= tile) lo_ = mid; else hi_ = mid - 1; }',
+         f'const long *d = desc + lo_ * {dlen};']
+    args = []
+    for p in L.arrays:
+        _, nm, ct, ar = p
+        so = lay[('str', nm)]
+        args.append(f'dvmh_dev::globalPtr<{ct}>(d[{lay[("base", nm)]}])')
+        args += [f'(IndexT)d[{so + q}]' for q in range(ar)]
+    for p in L.scalars:
+        args.append(f'dvmh_dev::globalPtr<{p[2]}>(d[{lay[("sc", p[1])]}])')
+    bo = lay['bounds']
+    for i in range(r):
+        args += [f'(IndexT)d[{bo + 3 * i}]', f'(IndexT)d[{bo + 3 * i + 1}]', f'(IndexT)d[{bo + 3 * i + 2}]']
+    nbo = lay['nb']
+    args += [f'(IndexT)d[{nbo}]', f'(IndexT)d[{nbo + 1}]', f'(IndexT)d[{nbo + 2}]', 'march', 'tile - tileStart[lo_]']
+    for red in L.reds:
+        t.append(f'{red[2]} {red[1]} = *(const {red[2]} *)&d[{lay[("ri", red[1])]}];')
+        args.append(red[1])
+        if len(red) > 3:
+            t.append(f'{red[4]} {red[3]} = *(const {red[4]} *)&d[{lay[("ri", red[3])]}];')
+            args.append(red[3])
+    t.append(f'{L.name}_march_tile<IndexT, COLS, NPF>({", ".join(args)});')
+    t += red_kernel_tail(L, 'tile', accumulate=False)
+    b += [IND + ln for ln in '\n'.join(t).split('\n')]
+    b.append('}')
+    k.append(indent('\n'.join(b) + '\n', 1).rstrip('\n'))
+    k.append('}')
+    return '\n'.join(k) + '\n'
+
+
 def batch_handler(L):
     r = L.rank
     lay, dlen = batch_desc_layout(L)
@@ -1025,7 +1096,42 @@ def batch_handler(L):
          'dvmh_loop_cuda_get_config_C(lr0, 0, 0, &threads, &stream, 0);',
          'int depth = (int)tune[DVMH_AMD_T_DEPTH];',
          'if (depth != 1 && depth != 2 && depth != 4 && depth != 8) depth = 1;',
-         f'long *desc = (long *)calloc((size_t)nl * {dlen}, sizeof(long));',
+         'int bvariant = 0, mcols = 1, npf = 0;',
+         'DvmType march = 1;',
+         'size_t ldsBytes = 0;']
+    bm = L.stencil and r in (2, 3)
+    if bm:
+        # LDS march tiles (VARIANT 1): same shape rules as the per-loop handler
+        shape_ok = 'threads.z == 1' + (' && threads.y == 1' if r == 2 else '')
+        rows = '((size_t)threads.y + 2)' if r == 3 else '(size_t)1'
+        b += ['bvariant = (int)tune[DVMH_AMD_T_VARIANT];',
+              'mcols = (int)tune[DVMH_AMD_T_TILE_A];',
+              'march = tune[DVMH_AMD_T_CHUNK];']
+        if r == 2:
+            # the default batch configuration leaves a stencil batch its own shape:
+            # march tiles of 256 threads x 4 points, 8 rows each
+            b += ['if (tune[DVMH_AMD_T_OWN] == 1) {',
+                  IND + 'bvariant = 1;',
+                  IND + 'mcols = 4;',
+                  IND + 'march = 8;',
+                  IND + 'threads = dim3(256, 1, 1);',
+                  '}']
+        b += [
+              f'if (bvariant == 1 && {shape_ok} && mcols >= 1 && mcols <= 4 && march >= 1) {{',
+              IND + f'size_t mplane = {rows} * ((size_t)threads.x * mcols + 2);']
+        for nm in L.stencil:
+            ct = [p for p in L.arrays if p[1] == nm][0][2]
+            b.append(IND + f'ldsBytes += (3 * mplane * sizeof({ct}) + 15) & ~(size_t)15;')
+        b += [IND + 'size_t mthreads = (size_t)threads.x * threads.y;',
+              IND + 'size_t need = (mplane + mthreads - 1) / mthreads;']
+        for cols, sets in MARCH_NPF.items():
+            b.append(IND + f'if (mcols == {cols}) {{ ' + ' '.join(f'if (npf == 0 && need <= {v}) npf = {v};' for v in sets) + ' }')
+        b += [IND + 'if (npf == 0 || ldsBytes > 64512) bvariant = 0;',
+              '} else {',
+              IND + 'bvariant = 0;',
+              '}']
+    b += ['int nonunit = 0;',
+          f'long *desc = (long *)calloc((size_t)nl * {dlen}, sizeof(long));',
          'DvmType *tstart = (DvmType *)calloc((size_t)nl + 1, sizeof(DvmType));',
          'long tiles = 0;',
          'int wide = 0;',
@@ -1063,6 +1169,13 @@ def batch_handler(L):
         lb.append(f'long nbz = (cnt{ax_z} + threads.z * depth - 1) / (threads.z * depth);')
     else:
         lb.append('long nbz = 1;')
+    if bm:
+        lb.append('if (bvariant == 1) {')
+        lb.append(IND + 'if (' + ' || '.join(f'bst[{i}] != 1' for i in range(r)) + ') nonunit = 1;')
+        lb.append(IND + f'nbx = (cnt{ax_x} + (long)threads.x * mcols - 1) / ((long)threads.x * mcols);')
+        lb.append(IND + (f'nby = (cnt{ax_y} + threads.y - 1) / threads.y;' if r == 3 else 'nby = 1;'))
+        lb.append(IND + 'nbz = (cnt0 + march - 1) / march;')
+        lb.append('}')
     lb.append(f'd[{lay["nb"]}] = nbx; d[{lay["nb"] + 1}] = nby; d[{lay["nb"] + 2}] = nbz;')
     for i in range(r):
         lb.append(f'if ((labs(blo[{i}]) + cnt{i} * labs(bst[{i}]) + 64) * maxStride > 2147483647L) wide = 1;')
@@ -1079,6 +1192,31 @@ def batch_handler(L):
     b += [IND + s for s in lb]
     b.append('}')
     b.append('tstart[nl] = tiles;')
+    if bm:
+        # a loop with non-unit steps, or 64-bit indices: every loop of the batch runs the direct tiles
+        b.append('if (bvariant == 1 && (nonunit || wide)) {')
+        b.append(IND + 'bvariant = 0;')
+        b.append(IND + 'tiles = 0;')
+        b.append(IND + 'for (long li = 0; li < nl; li++) {')
+        b.append(IND * 2 + f'long *d = desc + li * {dlen};')
+        for i in range(r):
+            b.append(IND * 2 + f'long cnt{i} = d[{lay["bounds"] + 3 * i + 1}];')
+        b.append(IND * 2 + f'long nbx = (cnt{ax_x} + threads.x - 1) / threads.x;')
+        if ax_y is not None:
+            dy = ' * depth' if dep_ax == ax_y else ''
+            b.append(IND * 2 + f'long nby = (cnt{ax_y} + threads.y{dy} - 1) / (threads.y{dy});')
+        else:
+            b.append(IND * 2 + 'long nby = 1;')
+        if ax_z is not None:
+            b.append(IND * 2 + f'long nbz = (cnt{ax_z} + threads.z * depth - 1) / (threads.z * depth);')
+        else:
+            b.append(IND * 2 + 'long nbz = 1;')
+        b.append(IND * 2 + f'd[{lay["nb"]}] = nbx; d[{lay["nb"] + 1}] = nby; d[{lay["nb"] + 2}] = nbz;')
+        b.append(IND * 2 + 'tstart[li] = tiles;')
+        b.append(IND * 2 + 'tiles += nbx * nby * nbz' + ''.join(f' * cnt{a}' for a in range(0, r - 3)) + ';')
+        b.append(IND + '}')
+        b.append(IND + 'tstart[nl] = tiles;')
+        b.append('}')
     b.append(f'long *dDesc = (long *)dvmh_amd_batch_scratch_C(batchRef, nl * {dlen} * (long)sizeof(long));')
     b.append(f'dvmh_amd_batch_upload_C(batchRef, dDesc, desc, nl * {dlen} * (long)sizeof(long));')
     b.append('long *dStart = (long *)dvmh_amd_batch_scratch_C(batchRef, (nl + 1) * (long)sizeof(long));')
@@ -1095,11 +1233,22 @@ def batch_handler(L):
         ra.append(f'{red[1]}_grid')
         if len(red) > 3:
             ra.append(f'{red[3]}_grid')
+    if bm:
+        b.append(IND + 'if (bvariant == 1) {')
+        margs = ['dDesc', 'dStart', 'nl', 'tiles', '(int)march'] + ra
+        for cols, sets in MARCH_NPF.items():
+            for v in sets:
+                b.append(IND * 2 + f'if (mcols == {cols} && npf == {v}) hipLaunchKernelGGL(({L.name}_batch_march_kernel<int, {cols}, {v}>), '
+                         f'dim3((unsigned)wgs), threads, ldsBytes, stream, {", ".join(margs)});')
+        b.append(IND + '} else {')
+    pre = IND * 2 if bm else IND
     for it in ['int', 'long long']:
         for dp in [1, 2, 4, 8]:
             cond = f'{"!wide" if it == "int" else "wide"} && depth == {dp}'
             args = ['dDesc', 'dStart', 'nl', 'tiles'] + ra
-            b.append(IND + f'if ({cond}) hipLaunchKernelGGL(({L.name}_batch_kernel<{it}, {dp}>), dim3((unsigned)wgs), threads, 0, stream, {", ".join(args)});')
+            b.append(pre + f'if ({cond}) hipLaunchKernelGGL(({L.name}_batch_kernel<{it}, {dp}>), dim3((unsigned)wgs), threads, 0, stream, {", ".join(args)});')
+    if bm:
+        b.append(IND + '}')
     b.append(IND + 'dvmh_amd_mark_kernel_C(lr0, 1);')
     b.append('}')
     for ri, red in enumerate(L.reds):
@@ -1807,6 +1956,8 @@ def gpu_loop(L):
         text.append(parallel_handler(L))
         if L.batch:
             text.append(batch_kernel(L))
+            if L.stencil and L.rank in (2, 3):
+                text.append(batch_march_kernel(L))
             text.append(batch_handler(L))
     return '\n'.join(text)
 

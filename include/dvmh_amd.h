@@ -39,7 +39,7 @@ extern "C" {
 #define DVMH_AMD_T_TILE_A 4    /* WAVEFRONT: tile extent on the first dependent axis; LINE: chunk steps */
 #define DVMH_AMD_T_TILE_B 5    /* WAVEFRONT: tile extent on the second dependent axis */
 #define DVMH_AMD_T_TRANSFORM 6 /* 1: arrays may be re-laid out for this loop */
-#define DVMH_AMD_T_RESERVED 7
+#define DVMH_AMD_T_OWN 7       /* BATCH: 1 = a batch of stencil loops runs its handler's own LDS march shape */
 void dvmh_amd_loop_get_tuning_C(DvmType curLoop, DvmType kind, DvmType tune[]);
 
 /* ---- kernel timing -----------------------------------------------------------

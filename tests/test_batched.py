@@ -63,9 +63,14 @@ def test_gpu_ragged_bit_exact(batched):
 
 
 KIND_BATCH = 4
-# (block, depth, xcd): the per-loop default shape, a one-row shape with eight
-# rows per thread, a narrow shape with two rows per thread
-BATCH_FORCED = [((32, 8, 1), 1, 0), ((256, 1, 1), 8, 1), ((64, 2, 1), 2, 0)]
+# (block, tune): the per-loop default shape, a one-row shape with eight rows per
+# thread, a narrow shape with two rows per thread (direct tiles), then LDS march
+# tiles of the stencil loop (VARIANT 1, TILE_A points per thread, CHUNK rows per
+# tile; the copy + MAX loop ignores the variant)
+BATCH_FORCED = [((32, 8, 1), (1, 0, 0, 0, 0, 0, 0, 0)), ((256, 1, 1), (8, 1, 0, 0, 0, 0, 0, 0)),
+                ((64, 2, 1), (2, 0, 0, 0, 0, 0, 0, 0)), ((128, 1, 1), (1, 0, 1, 16, 2, 0, 0, 0)),
+                ((256, 1, 1), (1, 0, 1, 8, 1, 0, 0, 0)), ((64, 1, 1), (1, 0, 1, 32, 4, 0, 0, 0)),
+                ((512, 1, 1), (1, 0, 1, 64, 1, 0, 0, 0))]
 
 
 @pytest.mark.gpu
@@ -74,11 +79,11 @@ def test_gpu_batch_configs_bit_exact(cfg):
     """Batch kernels under forced configurations other than the batch
     default: the same ragged grids, bit-exact against the oracle."""
     import ctypes
-    (block, depth, xcd) = cfg
+    (block, tune) = cfg
     r = D.runtime()
     r.dvmh_amd_tuner_force_C.argtypes = [ctypes.c_long, ctypes.c_void_p, ctypes.c_void_p]
     b = (ctypes.c_long * 3)(*block)
-    t = (ctypes.c_long * 8)(depth, xcd, 0, 0, 0, 0, 0, 0)
+    t = (ctypes.c_long * 8)(*tune)
     r.dvmh_amd_tuner_force_C(KIND_BATCH, ctypes.cast(b, ctypes.c_void_p), ctypes.cast(t, ctypes.c_void_p))
     try:
         sides = [2, 3, 67, 5, 130, 64, 257, 33, 1024]
