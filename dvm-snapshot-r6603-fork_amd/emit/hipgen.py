@@ -1595,29 +1595,67 @@ def tile_kernel(L):
           f'{xct} *dt_tile = ({xct} *)dvmh_lds;',
           'const IndexT dt_oi = lo0 + dt_a0 - 1, dt_oj = lo1 + dt_b0 - 1, dt_ok = lo2 + dt_c0 - 1;   // array index of tile slot (0, 0, 0)',
           'const IndexT dt_tid = (IndexT)threadIdx.x, dt_nthr = (IndexT)blockDim.x;',
-          '// stage the tile and its halo: only indices the body reads, per axis',
-          'for (IndexT dt_e = dt_tid; dt_e < dt_elems; dt_e += dt_nthr) {',
-          IND + 'const IndexT dt_p = dt_e / dt_R12, dt_rem = dt_e - (dt_e / dt_R12) * dt_R12;',
-          IND + 'const IndexT dt_q = dt_rem / dt_R2, dt_r = dt_rem - (dt_rem / dt_R2) * dt_R2;',
-          IND + 'const IndexT dt_ii = dt_oi + dt_p, dt_jj = dt_oj + dt_q, dt_kk = dt_ok + dt_r;',
-          IND + 'if (' + ' && '.join(f'{v} >= lo{a} + ({rng[a][0]}) && {v} <= lo{a} + cnt{a} - 1 + ({rng[a][1]})'
-                                    for a, v in enumerate(('dt_ii', 'dt_jj', 'dt_kk'))) + ')',
-          IND * 2 + f'dt_tile[dt_e] = {X}_g[dt_ii][dt_jj][dt_kk];',
+          '// stage the tile and its halo, four loads in flight per thread; an index',
+          '// outside those the body reads is clamped into them (its slot is never read)']
+    for a, v in enumerate(('dt_ii', 'dt_jj', 'dt_kk')):
+        b.append(f'const IndexT {v}_lo = lo{a} + ({rng[a][0]}), {v}_hi = lo{a} + cnt{a} - 1 + ({rng[a][1]});')
+    b += ['for (IndexT dt_e0 = dt_tid; dt_e0 < dt_elems; dt_e0 += 4 * dt_nthr) {',
+          IND + f'{xct} dt_v[4];',
+          IND + '#pragma unroll',
+          IND + 'for (int u = 0; u < 4; u++) {',
+          IND * 2 + 'IndexT dt_e = dt_e0 + (IndexT)u * dt_nthr;',
+          IND * 2 + 'dt_e = dt_e < dt_elems ? dt_e : dt_elems - 1;',
+          IND * 2 + 'const IndexT dt_p = dt_e / dt_R12, dt_rem = dt_e - (dt_e / dt_R12) * dt_R12;',
+          IND * 2 + 'const IndexT dt_q = dt_rem / dt_R2, dt_r = dt_rem - (dt_rem / dt_R2) * dt_R2;',
+          IND * 2 + 'IndexT dt_ii = dt_oi + dt_p, dt_jj = dt_oj + dt_q, dt_kk = dt_ok + dt_r;']
+    for v in ('dt_ii', 'dt_jj', 'dt_kk'):
+        b.append(IND * 2 + f'{v} = {v} < {v}_lo ? {v}_lo : ({v} > {v}_hi ? {v}_hi : {v});')
+    b += [IND * 2 + f'dt_v[u] = {X}_g[dt_ii][dt_jj][dt_kk];',
+          IND + '}',
+          IND + '#pragma unroll',
+          IND + 'for (int u = 0; u < 4; u++)',
+          IND * 2 + 'if (dt_e0 + (IndexT)u * dt_nthr < dt_elems) dt_tile[dt_e0 + (IndexT)u * dt_nthr] = dt_v[u];',
           '}',
           '__syncthreads();',
           f'const dvmh_dev::TileView3<{xct}, IndexT> {X}{{dt_tile, dt_R1, dt_R2, dt_oi, dt_oj, dt_ok}};',
-          '// local hyperplanes p + q + r = s of the tile, in order',
+          '// local hyperplanes p + q + r = s of the tile, in order; a thread\'s (p, q)',
+          '// columns do not change from one hyperplane to the next: up to four are',
+          '// worked out once (more than four per thread take the generic loop)',
           'const IndexT dt_nab = dt_na * dt_nb;',
+          'IndexT dt_myp[4], dt_myq[4];',
+          'bool dt_has[4];',
+          '#pragma unroll',
+          'for (int u = 0; u < 4; u++) {',
+          IND + 'const IndexT dt_idx = dt_tid + (IndexT)u * dt_nthr;',
+          IND + 'dt_has[u] = dt_idx < dt_nab;',
+          IND + 'dt_myp[u] = dt_idx / dt_nb;',
+          IND + 'dt_myq[u] = dt_idx - dt_myp[u] * dt_nb;',
+          '}',
+          'const bool dt_few = dt_nab <= 4 * dt_nthr;',
           'for (IndexT dt_s = 0; dt_s < dt_na + dt_nb + dt_nc - 2; dt_s++) {',
-          IND + 'for (IndexT dt_idx = dt_tid; dt_idx < dt_nab; dt_idx += dt_nthr) {',
-          IND * 2 + 'const IndexT dt_p = dt_idx / dt_nb, dt_q = dt_idx - (dt_idx / dt_nb) * dt_nb;',
-          IND * 2 + 'const IndexT dt_r = dt_s - dt_p - dt_q;',
-          IND * 2 + 'if (dt_r >= 0 && dt_r < dt_nc) {',
-          IND * 3 + 'const IndexT t0 = dt_a0 + dt_p, t1 = dt_b0 + dt_q, t2 = dt_c0 + dt_r;']
+          IND + 'if (dt_few) {',
+          IND * 2 + '#pragma unroll',
+          IND * 2 + 'for (int u = 0; u < 4; u++) {',
+          IND * 3 + 'const IndexT dt_p = dt_myp[u], dt_q = dt_myq[u];',
+          IND * 3 + 'const IndexT dt_r = dt_s - dt_p - dt_q;',
+          IND * 3 + 'if (dt_has[u] && dt_r >= 0 && dt_r < dt_nc) {',
+          IND * 4 + 'const IndexT t0 = dt_a0 + dt_p, t1 = dt_b0 + dt_q, t2 = dt_c0 + dt_r;']
     for i in range(3):
-        b.append(IND * 3 + var_assign(L, i, f't{i}', f'lo{i}', f'st{i}'))
-    b.append(indent('{\n' + indent(L.body + '\n', 1) + '}', 3))
-    b += [IND * 2 + '}',
+        b.append(IND * 4 + var_assign(L, i, f't{i}', f'lo{i}', f'st{i}'))
+    b.append(indent('{\n' + indent(L.body + '\n', 1) + '}', 4))
+    b += [IND * 3 + '}',
+          IND * 2 + '}',
+          IND + '} else {',
+          IND * 2 + 'for (IndexT dt_idx = dt_tid; dt_idx < dt_nab; dt_idx += dt_nthr) {',
+          IND * 3 + 'const IndexT dt_p = dt_idx / dt_nb, dt_q = dt_idx - (dt_idx / dt_nb) * dt_nb;',
+          IND * 3 + 'const IndexT dt_r = dt_s - dt_p - dt_q;',
+          IND * 3 + 'if (dt_r >= 0 && dt_r < dt_nc) {',
+          IND * 4 + 'const IndexT t0 = dt_a0 + dt_p, t1 = dt_b0 + dt_q, t2 = dt_c0 + dt_r;']
+    for i in range(3):
+        b.append(IND * 4 + var_assign(L, i, f't{i}', f'lo{i}', f'st{i}'))
+    b.append(indent('{\n' + indent(L.body + '\n', 1) + '}', 4))
+    b += [IND * 3 + '}',
+          IND * 2 + '}',
           IND + '}',
           IND + '__syncthreads();',
           '}',
