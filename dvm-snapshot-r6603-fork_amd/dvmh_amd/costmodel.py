@@ -33,7 +33,7 @@ MODEL = os.path.join(DATA_DIR, 'costmodel.bin')
 DEFAULT_SPECS = [
     ('jac3d', [64, 96, 128, 160, 192, 256, 320, 384, 448]),
     ('jac2d', [256, 512, 1024, 1536, 2048, 3072, 4096, 6144]),
-    ('adi3d', [64, 96, 128, 192]),
+    ('adi3d', [64, 96, 128, 192, 256]),
     ('sor3d', [48, 64, 96, 128]),
 ]
 HOLDOUT = '|e382x382x382'
