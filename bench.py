@@ -307,6 +307,42 @@ def calibrate_copy(nbytes=1 << 30):
     log('calibration copy of', nbytes, 'bytes done')
 
 
+def copy_rate(nbytes=PTS * 8, reps=10):
+    """Device-to-device copy rate of this part (GB/s counting the bytes read
+    and written), timed with HIP events over reps copies of nbytes: the
+    practical ceiling of a kernel that reads and writes each element once."""
+    hip = ctypes.CDLL('libamdhip64.so')
+    hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    hip.hipMemset.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t]
+    hip.hipMemcpyAsync.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_void_p]
+    hip.hipFree.argtypes = [ctypes.c_void_p]
+    hip.hipEventCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+    hip.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    hip.hipEventSynchronize.argtypes = [ctypes.c_void_p]
+    hip.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p, ctypes.c_void_p]
+    hip.hipEventDestroy.argtypes = [ctypes.c_void_p]
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(a), nbytes) or hip.hipMalloc(ctypes.byref(b), nbytes):
+        raise RuntimeError('hipMalloc failed')
+    e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
+    hip.hipEventCreate(ctypes.byref(e0))
+    hip.hipEventCreate(ctypes.byref(e1))
+    hip.hipMemset(a, 1, nbytes)
+    hip.hipMemcpyAsync(b, a, nbytes, 3, None)   # warm-up, hipMemcpyDeviceToDevice
+    hip.hipEventRecord(e0, None)
+    for _ in range(reps):
+        hip.hipMemcpyAsync(b, a, nbytes, 3, None)
+    hip.hipEventRecord(e1, None)
+    hip.hipEventSynchronize(e1)
+    ms = ctypes.c_float()
+    hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1)
+    hip.hipEventDestroy(e0)
+    hip.hipEventDestroy(e1)
+    hip.hipFree(a)
+    hip.hipFree(b)
+    return 2.0 * nbytes * reps / (ms.value * 1e-3) / 1e9
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -365,6 +401,8 @@ def main():
     log('cpu', cpu)
     kernel_s = chosen['kernel_us'] * 1e-6
     achieved = PTS * B_ALG / kernel_s
+    copy_gbps = copy_rate()
+    log('device copy', copy_gbps, 'GB/s')
     out = {
         'metric': METRIC,
         'value': chosen['value'],
@@ -383,7 +421,8 @@ def main():
         'roofline': {'bound': 'hbm', 'achieved': achieved / 1e9, 'peak': HBM_PEAK / 1e9, 'unit': 'GB/s',
                      'frac': achieved / HBM_PEAK, 'traffic': measured_traffic(),
                      'kernel': 'jac3d stencil loop kernels (loop_jac3d_41/54, the configuration the tuner kept)', 'kernel_us': chosen['kernel_us'],
-                     'bytes_per_point_update': B_ALG, 'point_updates_per_launch': PTS},
+                     'bytes_per_point_update': B_ALG, 'point_updates_per_launch': PTS,
+                     'device_copy_GBps': copy_gbps, 'frac_of_device_copy': achieved / 1e9 / copy_gbps},
         'plain': plain,
         'exhaustive': exhaustive,
         'speedup_vs_plain': chosen['value'] / plain['value'] if plain else None,
