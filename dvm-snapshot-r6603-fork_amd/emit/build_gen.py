@@ -15,10 +15,10 @@ import cdvmh  # noqa: E402
 import hipgen  # noqa: E402
 
 # programs/*.cdv that are complete C-DVMH programs: the configurations, and
-# relax2d and redall for constructs and reduction kinds they do not use
+# relax2d, redall and jacg for constructs and reduction kinds they do not use
 # (batched.cdv sketches the batched workload's structure and is not
 # compilable C)
-CDV_PROGRAMS = ['jac2d', 'jac3d', 'sor3d', 'adi3d', 'relax2d', 'redall']
+CDV_PROGRAMS = ['jac2d', 'jac3d', 'sor3d', 'adi3d', 'relax2d', 'redall', 'jacg']
 
 for spec in sorted(glob.glob(os.path.join(PKG, 'programs', '*.loops'))):
     hipgen.emit(spec, os.path.join(PKG, 'gen'))

@@ -200,6 +200,8 @@ class DvmArray:
         self.sizes = sizes              # extents as text (first may be None for incomplete)
         self.incomplete = incomplete
         self.rule = rule                # ('distribute', [...]) or ('align', dummy vars, target, target subscripts)
+        self.shadow = [('1', '1')] * rank   # shadow widths (low, high) per axis
+        self.line = 0
 
 
 def parse_declaration(text):
@@ -243,6 +245,9 @@ class Converter:
         self.handlers = []         # (name, param kinds) for prototypes
         self.region_no = 0
         self.includes = []
+        self.defines = []          # file-scope #define lines, repeated in the handler files
+        self.globals = {}          # file-scope variables and distributed arrays
+        self.global_arrays = []
 
     # -- helpers
     def text(self, a, b):
@@ -262,16 +267,47 @@ class Converter:
     def convert(self):
         out, i, last = [], 0, 0
         toks = self.toks
+        stmt = 0            # first token of the current file-scope statement
+        pending = None      # array directive waiting for its file-scope declaration
         while i < len(toks):
             t = toks[i]
             if t.kind == 'pp':
                 if re.match(r'#\s*include', t.text):
                     self.includes.append(t.text.strip())
+                if re.match(r'#\s*define\b', t.text):
+                    self.defines.append(t.text.strip())   # loop bodies may use the program's macros
                 if re.match(r'#\s*pragma\s+dvm\b', t.text):
-                    self.err(t, 'directives outside functions are not supported')
+                    d = Directive(t)
+                    if d.kind != 'array':
+                        self.err(t, f'directive {d.kind!r} outside functions is not supported')
+                    if pending:
+                        self.err(t, 'one directive per declaration')
+                    pending = d
+                    out.append(self.src[last:t.start])
+                    last = t.end
                 i += 1
+                stmt = i
+                continue
+            if t.text == ';' and t.kind == 'op':
+                # a file-scope declaration: globals are visible to every function
+                text = self.text(stmt, i)
+                if pending:
+                    v, a = self.array_spec(pending, text)
+                    self.globals[v.name] = v
+                    self.global_arrays.append(a)
+                    out.append(self.src[last:toks[stmt].start])
+                    out.append(f'DvmType {v.name}[64];')
+                    last = t.end
+                    pending = None
+                else:
+                    for nm, ct, dims, pr, _ in (parse_declaration(re.sub(r'^\s*(static|extern)\s+', '', text)) or []):
+                        self.globals[nm] = Var(nm, ct, dims, pr)
+                i += 1
+                stmt = i
                 continue
             if t.text == '{' and t.kind == 'op':
+                if pending:
+                    self.err(pending.line, 'an array directive at file scope is followed by a declaration')
                 # a function definition: `... name ( params ) {`
                 j = i - 1
                 if j < 0 or toks[j].text != ')':
@@ -292,13 +328,23 @@ class Converter:
                 out.append(self.function(name_tok.text, k, j, i, close))
                 last = toks[close].end
                 i = close + 1
+                stmt = i
                 continue
             i += 1
+        if pending:
+            self.err(pending.line, 'array directive without a declaration')
         out.append(self.src[last:])
+        if self.global_arrays:
+            # the file-scope arrays' DVM objects, created once by the first
+            # converted function that runs, in declaration order
+            self.scope = dict(self.globals)
+            body = ''.join('    ' + self.array_create(a, '    ') + '\n' for a in self.global_arrays)
+            out.append(f'\nstatic void {self.prog}_globals_init(void)\n{{\n    static int done = 0;\n'
+                       f'    if (done) return;\n    done = 1;\n{body}}}\n')
         return ''.join(out)
 
     def function(self, name, lparen, rparen, lbrace, rbrace):
-        self.scope = {}
+        self.scope = dict(self.globals)
         for p in split_top(self.text(lparen + 1, rparen)):
             if p in ('', 'void'):
                 continue
@@ -314,6 +360,8 @@ class Converter:
         body = self.block(lbrace + 1, rbrace, ctx={'region': None})
         ind = '    '
         head = '{\n' + ind + 'dvmh_init_lib_C(0);'
+        if self.global_arrays:
+            head += f'\n{ind}{self.prog}_globals_init();'
         tail = ''
         for a in self.arrays_here:
             tail += ind + ('dvmh_forget_header_' if a.incomplete else 'dvmh_delete_object_') + f'({a.var.name});\n'
@@ -468,13 +516,21 @@ class Converter:
                 last = toks[e - 1].end
                 k = e
                 continue
+            if v and v.dvm and (k == i or toks[k - 1].text not in ('.', '->')):
+                # the whole array as a value (fwrite(B, ...), a call argument):
+                # its host copy in natural layout (made actual by get_actual)
+                out.append(self.src[last:t.start])
+                out.append(f'((void *)dvmh_get_natural_base_C(0, {t.text}))')
+                last = t.end
             k += 1
         out.append(self.src[last:toks[j - 1].end])
         return ''.join(out)
 
     # -- directives
-    def array_decl(self, d, i, j):
-        decl = parse_declaration(self.text(i, j - 1))
+    def array_spec(self, d, text):
+        """The distributed array declared by `text` under array directive d:
+        distribute[block|]... or align(...), optionally shadow[lo:hi]... ."""
+        decl = parse_declaration(text)
         if not decl or len(decl) != 1:
             self.err(d.line, 'a distributed array declaration declares one array')
         nm, ct, dims, pr, init = decl[0]
@@ -485,32 +541,61 @@ class Converter:
             self.err(d.line, f'element type {ct!r} is not supported')
         v = Var(nm, ct, dims, pr)
         sizes = ([None] + dims) if pr else dims
-        rule = None
-        m = re.match(r'distribute\s*((?:\[[^\]]*\]\s*)+)$', d.rest)
-        if m:
-            fm = brackets(m.group(1))
-            if len(fm) != rank or any(f not in ('block', '') for f in fm):
-                self.err(d.line, 'distribute accepts [block] or [] per axis')
-            rule = ('distribute', fm)
-        m2 = re.match(r'align\s*\(\s*((?:\[[^\]]*\]\s*)+)\s*with\s+([A-Za-z_]\w*)\s*((?:\[[^\]]*\]\s*)+)\)\s*$', d.rest)
-        if m2:
-            rule = ('align', brackets(m2.group(1)), m2.group(2), brackets(m2.group(3)))
+        rule, shadow = None, None
+        for cl in split_top(d.rest):
+            cl = cl.strip()
+            m = re.match(r'distribute\s*((?:\[[^\]]*\]\s*)+)$', cl)
+            if m:
+                fm = brackets(m.group(1))
+                if len(fm) != rank or any(f not in ('block', '') for f in fm):
+                    self.err(d.line, 'distribute accepts [block] or [] per axis')
+                rule = ('distribute', fm)
+                continue
+            m2 = re.match(r'align\s*\(\s*((?:\[[^\]]*\]\s*)+)\s*with\s+([A-Za-z_]\w*)\s*((?:\[[^\]]*\]\s*)+)\)\s*$', cl)
+            if m2:
+                rule = ('align', brackets(m2.group(1)), m2.group(2), brackets(m2.group(3)))
+                continue
+            m3 = re.match(r'shadow\s*((?:\[[^\]]*\]\s*)+)$', cl)
+            if m3:
+                ws = brackets(m3.group(1))
+                if len(ws) != rank:
+                    self.err(d.line, 'shadow gives a width per axis')
+                shadow = []
+                for w in ws:
+                    lo, sep, hi = w.partition(':')
+                    shadow.append((lo.strip(), (hi if sep else lo).strip()))
+                continue
+            self.err(d.line, f'array directive clause {cl!r} is not supported')
         if rule is None:
             self.err(d.line, f'array directive {d.rest!r} is not supported')
         a = DvmArray(v, rank, sizes, bool(pr), rule)
+        if shadow:
+            a.shadow = shadow
+        a.line = d.line
         v.dvm = a
-        self.scope[nm] = v
-        self.arrays_here.append(a)
-        ind = self.indent_of(i)
-        typ = f'-{RT_TYPE[ct]}'
-        out = f'DvmType {nm}[64];\n' + ind + self.line_call(d.line) + '\n' + ind
+        return v, a
+
+    def array_create(self, a, ind):
+        """Calls that create (complete) or declare (pointer) the array's DVM object."""
+        nm, rank = a.var.name, a.rank
+        typ = f'-{RT_TYPE[a.var.ctype]}'
+        out = self.line_call(a.line) + '\n' + ind
+        def w(x):
+            return f'(DvmType){x}' if x.isdigit() else f'(DvmType)({x})'
         if a.incomplete:
-            axes = ''.join(f', (DvmType)({s if s is not None else 0}), (DvmType)1, (DvmType)1' for s in sizes)
+            axes = ''.join(f', (DvmType)({s if s is not None else 0}), {w(lo)}, {w(hi)}' for s, (lo, hi) in zip(a.sizes, a.shadow))
             out += f'dvmh_array_declare_C({nm}, {rank}, {typ}{axes});'
         else:
-            axes = ''.join(f', (DvmType)({s}), (DvmType)1, (DvmType)1' for s in sizes)
+            axes = ''.join(f', (DvmType)({s}), {w(lo)}, {w(hi)}' for s, (lo, hi) in zip(a.sizes, a.shadow))
             out += f'dvmh_array_create_C({nm}, {rank}, {typ}{axes});\n' + ind + self.distribution_call(a)
         return out
+
+    def array_decl(self, d, i, j):
+        v, a = self.array_spec(d, self.text(i, j - 1))
+        self.scope[v.name] = v
+        self.arrays_here.append(a)
+        ind = self.indent_of(i)
+        return f'DvmType {v.name}[64];\n' + ind + self.array_create(a, ind)
 
     def distribution_call(self, a):
         nm = a.var.name
@@ -829,12 +914,16 @@ class Converter:
         head = (f'/* {self.file} converted by emit/cdvmh.py: directives replaced by calls of the DVMH C\n'
                 f' * interface; loop handlers are emitted from {self.prog}.loops. */\n'
                 '#include "dvmhlib2.h"\n' + '\n'.join(protos) + '\n')
+        if self.global_arrays:
+            head += f'static void {self.prog}_globals_init(void);\n'
         return head + converted
 
     def loops_text(self):
         inc = [re.match(r'#\s*include\s*"([^"]+)"', x).group(1) for x in self.includes
                if re.match(r'#\s*include\s*"', x)]
         spec = {'program': self.prog, 'include': inc[0] if len(inc) == 1 else inc, 'loops': self.loops}
+        if self.defines:
+            spec['defines'] = self.defines
         import pprint
         return f'# Loop descriptions of {self.file}, written by emit/cdvmh.py.\n' + pprint.pformat(spec, width=110, sort_dicts=False) + '\n'
 

@@ -2007,19 +2007,19 @@ def emit(spec_path, outdir):
     prog = spec['program']
     loops = [Loop(d, prog) for d in spec['loops']]
     inc = spec.get('include')
+    incs = [inc] if isinstance(inc, str) else list(inc or [])
+    defs = list(spec.get('defines', []))   # the program's own macros, which loop bodies may use
     host = ['/* Host handlers generated from ' + os.path.basename(spec_path) + ' by emit/hipgen.py. */',
             '#include <math.h>', '#include <stdio.h>', '#include <stdlib.h>',
             '#ifdef _OPENMP', '#include <omp.h>', '#endif', '#include "dvmhlib2.h"']
-    if inc:
-        host.append(f'#include "{inc}"')
+    host += [f'#include "{x}"' for x in incs] + defs
     host.append('')
     for L in loops:
         host.append(host_handler(L))
     hip = ['// GPU handlers generated from ' + os.path.basename(spec_path) + ' by emit/hipgen.py (gfx950).',
            '#include <hip/hip_runtime.h>', '#include <math.h>', '#include <stdio.h>', '#include <stdlib.h>', '#include <string.h>',
            '#include "dvmhlib2.h"', '#include "dvmh_amd.h"', '#include "dvmh_hip_device.h"']
-    if inc:
-        hip.append(f'#include "{inc}"')
+    hip += [f'#include "{x}"' for x in incs] + defs
     hip.append('')
     for L in loops:
         hip.append(gpu_loop(L))

@@ -338,3 +338,44 @@ def test_redall_gpu(n, m):
     if n * m > 1000:
         assert min(ref['ties']) > 1
     assert _gpu_loops() > before
+
+
+# ---------------------------------------------------------------- jacg
+def _check_jacg(iters, seed=D.SEED_BASE + 1):
+    """jacg.cdv (file-scope arrays, loop variables and macros, shadow widths,
+    whole arrays handed to host code) against the oracle's 2-D Jacobi."""
+    n = 131
+    f = _lib('jacg').jacg
+    f.argtypes = [ctypes.c_int, ctypes.c_ulonglong, _dp, _dp, _dp]
+    eps, A, B = np.zeros(iters), np.zeros(n * n), np.zeros(n * n)
+    f(iters, seed, H.dp(eps), H.dp(A), H.dp(B))
+    ref = H.oracle_jac2d(n, iters, seed)
+    assert np.array_equal(eps, ref['eps'])
+    assert H.bits_equal(A, ref['A']) and H.bits_equal(B, ref['B'])
+
+
+def test_jacg_host_path():
+    if D.stats()['gpu_present']:
+        pytest.skip('the host path runs where no GPU is present; the GPU tests cover this case')
+    _check_jacg(4)
+
+
+@pytest.mark.gpu
+def test_jacg_gpu():
+    before = _gpu_loops()
+    _check_jacg(6)
+    assert _gpu_loops() > before
+
+
+def test_file_scope_conversion(tmp_path):
+    """File-scope arrays become headers created once by the first converted
+    function that runs; shadow widths reach the creation call; a whole array
+    used as a value is its host copy."""
+    cdvmh.convert_file(os.path.join(PKG, 'programs', 'jacg.cdv'), str(tmp_path), prog='jacg_cdv')
+    c = open(tmp_path / 'jacg_cdv.c').read()
+    for call in ['DvmType A[64];', 'DvmType B[64];', 'static void jacg_cdv_globals_init(void);',
+                 'dvmh_array_create_C(A, 2, -rt_DOUBLE, (DvmType)(N), (DvmType)1, (DvmType)1, (DvmType)(N), (DvmType)1, (DvmType)1);',
+                 'copy_out(a_out, ((void *)dvmh_get_natural_base_C(0, A)));']:
+        assert call in c, call
+    spec = ast.literal_eval(open(tmp_path / 'jacg_cdv.loops').read().split('\n', 1)[1])
+    assert '#define MAXOF(a, b) ((a) > (b) ? (a) : (b))' in spec['defines']

@@ -1,0 +1,140 @@
+"""The reference's own C-DVMH example programs (examples/jac2d.cdv,
+examples/jacross.cdv, examples/jac3d.cdv), read from the reference tree,
+converted by the front end (emit/cdvmh.py: file-scope distributed arrays and
+variables, the programs' macros, get_actual and fwrite of whole arrays),
+their handlers emitted (emit/hipgen.py), compiled, linked against
+libdvmh_amd and run.  What each prints (the per-iteration MAX reduction) and
+the array it writes to jacobi.dat must equal a numpy restatement of the
+program, bit for bit.
+
+These run where the reference tree is present (on the host path when no GPU
+is visible); the programs and their conversions are never stored in the
+repository.  Without the reference tree the module is skipped.
+"""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+REF = '/root/reference/dvm_sys/examples'
+PKG = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), 'dvm-snapshot-r6603-fork_amd')
+REPO = os.path.dirname(PKG)
+sys.path.insert(0, os.path.join(PKG, 'emit'))
+
+pytestmark = pytest.mark.skipif(not os.path.isdir(REF), reason='the reference tree is not present')
+
+
+def build_and_run(stem, tmp_path):
+    import cdvmh
+    import hipgen
+    out = str(tmp_path)
+    cdvmh.convert_file(os.path.join(REF, stem + '.cdv'), out, prog=stem)
+    hipgen.emit(os.path.join(out, stem + '.loops'), out)
+    inc = ['-I' + os.path.join(REPO, 'include'), '-I' + os.path.join(PKG, 'programs')]
+    cc = ['gcc', '-O2', '-fPIC', '-fopenmp', '-ffp-contract=off'] + inc
+    objs = []
+    for src in (f'{stem}.c', f'{stem}_host.c'):
+        o = os.path.join(out, src + '.o')
+        subprocess.run(cc + ['-c', os.path.join(out, src), '-o', o], check=True, timeout=120)
+        objs.append(o)
+    o = os.path.join(out, stem + '_hip.o')
+    subprocess.run(['/opt/rocm/bin/hipcc', '--offload-arch=gfx950', '-O3', '-fPIC', '-std=c++17', '-ffp-contract=off',
+                    '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops', '-D__HIP_PLATFORM_AMD__',
+                    '-I' + os.path.join(PKG, 'csrc', 'rt'), '-I' + os.path.join(PKG, 'csrc', 'kernels')] + inc +
+                   ['-c', os.path.join(out, stem + '_hip.hip'), '-o', o], check=True, timeout=300,
+                   stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+    objs.append(o)
+    exe = os.path.join(out, stem)
+    lib = os.path.join(PKG, 'lib')
+    subprocess.run(['g++', '-fopenmp', '-o', exe] + objs + ['-L' + lib, '-ldvmh_amd', '-L/opt/rocm/lib', '-lamdhip64',
+                                                           '-Wl,-rpath,' + lib, '-Wl,-rpath,/opt/rocm/lib'],
+                   check=True, timeout=120)
+    r = subprocess.run([exe], cwd=out, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    return r.stdout, np.fromfile(os.path.join(out, 'jacobi.dat'))
+
+
+def eps_lines(hist):
+    return ''.join('it=%4i   eps=%e\n' % (it, e) for it, e in hist)
+
+
+def ref_jac2d(L=8, itmax=20, maxeps=0.5):
+    """examples/jac2d.cdv: A = 0, B = 3 + i + j inside a zero boundary; per
+    iteration eps = max |B - A| with A = B, then the 4-point mean of A into B."""
+    A = np.zeros((L, L))
+    B = np.zeros((L, L))
+    for i in range(1, L - 1):
+        for j in range(1, L - 1):
+            B[i, j] = 3 + i + j
+    hist = []
+    for it in range(1, itmax + 1):
+        eps = 0.0
+        for i in range(1, L - 1):
+            for j in range(1, L - 1):
+                eps = max(abs(B[i, j] - A[i, j]), eps)
+                A[i, j] = B[i, j]
+        for i in range(1, L - 1):
+            for j in range(1, L - 1):
+                B[i, j] = (A[i - 1, j] + A[i, j - 1] + A[i, j + 1] + A[i + 1, j]) / 4.0
+        hist.append((it, eps))
+        if eps < maxeps:
+            break
+    return hist, B.ravel()
+
+
+def ref_jacross(L=8, itmax=20, maxeps=0.5):
+    """examples/jacross.cdv: in-place lexicographic sweeps (ACROSS on both
+    axes) of the 4-point mean, eps = max |new - old|."""
+    A = np.zeros((L, L))
+    for i in range(1, L - 1):
+        for j in range(1, L - 1):
+            A[i, j] = 3.0 + i + j
+    hist = []
+    for it in range(1, itmax + 1):
+        eps = 0.0
+        for i in range(1, L - 1):
+            for j in range(1, L - 1):
+                b = (A[i - 1, j] + A[i + 1, j] + A[i, j - 1] + A[i, j + 1]) / 4.0
+                eps = max(abs(b - A[i, j]), eps)
+                A[i, j] = b
+        hist.append((it, eps))
+        if eps < maxeps:
+            break
+    return hist, A.ravel()
+
+
+def ref_jac3d(L=8, itmax=20, maxeps=0.5):
+    """examples/jac3d.cdv: the 3-D version (B = 4 + i + j + k inside, 6-point mean)."""
+    A = np.zeros((L, L, L))
+    B = np.zeros((L, L, L))
+    for i in range(1, L - 1):
+        for j in range(1, L - 1):
+            for k in range(1, L - 1):
+                B[i, j, k] = 4 + i + j + k
+    hist = []
+    for it in range(1, itmax + 1):
+        eps = 0.0
+        for i in range(1, L - 1):
+            for j in range(1, L - 1):
+                for k in range(1, L - 1):
+                    eps = max(abs(B[i, j, k] - A[i, j, k]), eps)
+                    A[i, j, k] = B[i, j, k]
+        for i in range(1, L - 1):
+            for j in range(1, L - 1):
+                for k in range(1, L - 1):
+                    B[i, j, k] = (A[i - 1, j, k] + A[i, j - 1, k] + A[i, j, k - 1] + A[i, j, k + 1] + A[i, j + 1, k] +
+                                  A[i + 1, j, k]) / 6.0
+        hist.append((it, eps))
+        if eps < maxeps:
+            break
+    return hist, B.ravel()
+
+
+@pytest.mark.parametrize('stem,ref', [('jac2d', ref_jac2d), ('jacross', ref_jacross), ('jac3d', ref_jac3d)])
+def test_reference_example_program(stem, ref, tmp_path):
+    out, data = build_and_run(stem, tmp_path)
+    hist, final = ref()
+    assert out == eps_lines(hist)
+    assert data.shape == final.shape and np.array_equal(data.view(np.uint64), final.view(np.uint64))
