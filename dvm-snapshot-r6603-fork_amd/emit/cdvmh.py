@@ -463,7 +463,14 @@ class Converter:
             return 'do' + self.src[t.end:toks[i + 1].start] + self.plain_statement(i + 1, e, ctx) + \
                 self.src[toks[e - 1].end:toks[e].start] + self.host_expr(e, j, ctx)
         if t.text == 'return' and self.arrays_here:
-            self.err(t, 'return before the end of a function with distributed arrays is not supported')
+            # the function's distributed arrays are released first (as at its
+            # end); the returned value must not read them
+            if self.uses_dvm_array(i + 1, j):
+                self.err(t, 'a return value that reads a distributed array is not supported')
+            ind = self.indent_of(i)
+            rel = ''.join(('dvmh_forget_header_' if a.incomplete else 'dvmh_delete_object_') + f'({a.var.name}); '
+                          for a in self.arrays_here)
+            return '{ ' + rel + self.host_expr(i, j, ctx) + ' }'
         # declarations: remember the types
         d = parse_declaration(self.text(i, j - 1))
         if d:
@@ -546,9 +553,9 @@ class Converter:
             cl = cl.strip()
             m = re.match(r'distribute\s*((?:\[[^\]]*\]\s*)+)$', cl)
             if m:
-                fm = brackets(m.group(1))
+                fm = ['' if f.strip() == '*' else f.strip() for f in brackets(m.group(1))]   # [*]: not distributed
                 if len(fm) != rank or any(f not in ('block', '') for f in fm):
-                    self.err(d.line, 'distribute accepts [block] or [] per axis')
+                    self.err(d.line, 'distribute accepts [block], [*] or [] per axis')
                 rule = ('distribute', fm)
                 continue
             m2 = re.match(r'align\s*\(\s*((?:\[[^\]]*\]\s*)+)\s*with\s+([A-Za-z_]\w*)\s*((?:\[[^\]]*\]\s*)+)\)\s*$', cl)
@@ -770,7 +777,9 @@ class Converter:
                 across = {'array': am.group(1), 'widths': widths}
             elif name == 'shadow_renew':
                 for x in split_top(args):
-                    x = x.strip()
+                    # A(corner): diagonal neighbours too; with one process every
+                    # shadow element is already the array's own
+                    x = re.sub(r'\(\s*corner\s*\)\s*$', '', x.strip()).strip()
                     if x not in self.scope or not self.scope[x].dvm:
                         self.err(d.line, f'shadow_renew of {x!r}, which is not a distributed array')
                     shadows.append(x)
