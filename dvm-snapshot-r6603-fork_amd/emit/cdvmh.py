@@ -248,6 +248,7 @@ class Converter:
         self.defines = []          # file-scope #define lines, repeated in the handler files
         self.globals = {}          # file-scope variables and distributed arrays
         self.global_arrays = []
+        self.inherits = {}         # function -> positions of its inherited (distributed array) parameters
 
     # -- helpers
     def text(self, a, b):
@@ -269,6 +270,7 @@ class Converter:
         toks = self.toks
         stmt = 0            # first token of the current file-scope statement
         pending = None      # array directive waiting for its file-scope declaration
+        inherit = None      # inherit directive waiting for its function declaration
         while i < len(toks):
             t = toks[i]
             if t.kind == 'pp':
@@ -278,11 +280,17 @@ class Converter:
                     self.defines.append(t.text.strip())   # loop bodies may use the program's macros
                 if re.match(r'#\s*pragma\s+dvm\b', t.text):
                     d = Directive(t)
-                    if d.kind != 'array':
+                    if d.kind not in ('array', 'inherit'):
                         self.err(t, f'directive {d.kind!r} outside functions is not supported')
-                    if pending:
+                    if pending or inherit:
                         self.err(t, 'one directive per declaration')
-                    pending = d
+                    if d.kind == 'inherit':
+                        m = re.match(r'\((.*)\)$', d.rest, re.S)
+                        if not m:
+                            self.err(t, 'inherit(names) expected')
+                        inherit = (t.line, [x.strip() for x in split_top(m.group(1))])
+                    else:
+                        pending = d
                     out.append(self.src[last:t.start])
                     last = t.end
                 i += 1
@@ -291,7 +299,13 @@ class Converter:
             if t.text == ';' and t.kind == 'op':
                 # a file-scope declaration: globals are visible to every function
                 text = self.text(stmt, i)
-                if pending:
+                if inherit:
+                    # a prototype of a function with inherited parameters
+                    out.append(self.src[last:toks[stmt].start])
+                    out.append(self.inherit_header(stmt, i, inherit)[1])
+                    last = toks[i - 1].end
+                    inherit = None
+                elif pending:
                     v, a = self.array_spec(pending, text)
                     self.globals[v.name] = v
                     self.global_arrays.append(a)
@@ -324,8 +338,15 @@ class Converter:
                     k -= 1
                 name_tok = toks[k - 1]
                 close = matching(toks, i)
-                out.append(self.src[last:t.start])
-                out.append(self.function(name_tok.text, k, j, i, close))
+                inherited = ()
+                if inherit:
+                    out.append(self.src[last:toks[stmt].start])
+                    inherited, head = self.inherit_header(stmt, i, inherit)
+                    out.append(head + self.src[toks[i - 1].end:t.start])
+                    inherit = None
+                else:
+                    out.append(self.src[last:t.start])
+                out.append(self.function(name_tok.text, k, j, i, close, inherited))
                 last = toks[close].end
                 i = close + 1
                 stmt = i
@@ -333,6 +354,8 @@ class Converter:
             i += 1
         if pending:
             self.err(pending.line, 'array directive without a declaration')
+        if inherit:
+            self.err(inherit[0], 'inherit directive without a function declaration')
         out.append(self.src[last:])
         if self.global_arrays:
             # the file-scope arrays' DVM objects, created once by the first
@@ -343,7 +366,64 @@ class Converter:
                        f'    if (done) return;\n    done = 1;\n{body}}}\n')
         return ''.join(out)
 
-    def function(self, name, lparen, rparen, lbrace, rbrace):
+    def inherit_header(self, a, b, inherit):
+        """Declaration tokens [a, b) of a function under `inherit(names)`:
+        (the inherited parameter names, the declaration with each of them
+        turned into the DvmType header array its caller passes).  The
+        reference converter rewrites such a parameter the same way
+        (dvm_sys/cdvmh/src/converter_main.cpp:856-863); every name must be a
+        parameter (:920-928)."""
+        toks = self.toks
+        line, names = inherit
+        r = b - 1
+        if toks[r].text != ')':
+            self.err(line, 'inherit directive not followed by a function declaration')
+        k = r
+        depth = 0
+        while k >= a:
+            if toks[k].text == ')':
+                depth += 1
+            elif toks[k].text == '(':
+                depth -= 1
+                if depth == 0:
+                    break
+            k -= 1
+        fname = toks[k - 1].text
+        if fname == 'main':
+            self.err(line, 'main cannot inherit distributed arrays')
+        text, base = self.text(a, b), toks[a].start
+        pos, found = [], []
+        for n, (p, q) in reversed(list(enumerate(self.param_ranges(k + 1, r)))):
+            d = parse_declaration(self.text(p, q)) if q > p else None
+            if d and d[0][0] in names:
+                text = text[:toks[p].start - base] + f'DvmType {d[0][0]}[64]' + text[toks[q - 1].end - base:]
+                pos.insert(0, n)
+                found.insert(0, d[0][0])
+        missing = [x for x in names if x not in found]
+        if missing:
+            self.err(line, f'inherited names that are not parameters of {fname}: {", ".join(missing)}')
+        prev = self.inherits.setdefault(fname, pos)
+        if prev != pos:
+            self.err(line, f'{fname}: inherited parameters differ from its earlier declaration')
+        return found, text
+
+    def param_ranges(self, a, b):
+        """Token ranges [p, q) of the comma-separated items within [a, b)."""
+        out, p, depth = [], a, 0
+        for k in range(a, b):
+            x = self.toks[k].text
+            if x in ('(', '[', '{'):
+                depth += 1
+            elif x in (')', ']', '}'):
+                depth -= 1
+            elif x == ',' and depth == 0:
+                out.append((p, k))
+                p = k + 1
+        if b > a:
+            out.append((p, b))
+        return out
+
+    def function(self, name, lparen, rparen, lbrace, rbrace, inherited=()):
         self.scope = dict(self.globals)
         for p in split_top(self.text(lparen + 1, rparen)):
             if p in ('', 'void'):
@@ -355,6 +435,15 @@ class Converter:
             v = Var(nm, ct, dims, pr, is_param=True)
             if dims and not pr:           # `double x[]` parameter: a pointer
                 v.pointer_rank = 0
+            if nm in inherited:
+                # the caller's distributed array, passed as its header: the
+                # extents come from the header, the element type and rank
+                # from the declaration (`double (*a)[n][m]`, `double a[][m]`)
+                rank = len(dims) + (1 if pr else 0)
+                if rank < 1:
+                    self.err(self.toks[lparen], f'inherited parameter {nm} is not an array')
+                v.dvm = DvmArray(v, rank, ([None] if pr else []) + list(dims), True, None)
+                v.dvm.line = self.toks[lparen].line
             self.scope[nm] = v
         self.arrays_here = []
         body = self.block(lbrace + 1, rbrace, ctx={'region': None})
@@ -506,9 +595,22 @@ class Converter:
         distributed arrays replaced by element accesses."""
         toks = self.toks
         out, k, last = [], i, toks[i].start
+        headers = set()     # arguments passed to inherited parameters: the header itself
         while k < j:
             t = toks[k]
+            if t.kind == 'id' and t.text in self.inherits and k + 1 < j and toks[k + 1].text == '(':
+                args = self.param_ranges(k + 2, matching(toks, k + 1))
+                for n in self.inherits[t.text]:
+                    if n < len(args):
+                        p, q = args[n]
+                        w = self.scope.get(toks[p].text) if q == p + 1 and toks[p].kind == 'id' else None
+                        if not (w and w.dvm):
+                            self.err(t, f'{t.text}: argument {n + 1} must be a distributed array')
+                        headers.add(p)
             v = self.scope.get(t.text) if t.kind == 'id' else None
+            if k in headers:
+                k += 1
+                continue
             if v and v.dvm and k + 1 < j and toks[k + 1].text == '[' and (k == i or toks[k - 1].text not in ('.', '->')):
                 subs, e = [], k + 1
                 while e < j and toks[e].text == '[' and len(subs) < v.dvm.rank:

@@ -370,12 +370,18 @@ def test_jacg_gpu():
 def test_file_scope_conversion(tmp_path):
     """File-scope arrays become headers created once by the first converted
     function that runs; shadow widths reach the creation call; a whole array
-    used as a value is its host copy."""
+    used as a value is its host copy; an inherited parameter is the caller's
+    header."""
     cdvmh.convert_file(os.path.join(PKG, 'programs', 'jacg.cdv'), str(tmp_path), prog='jacg_cdv')
     c = open(tmp_path / 'jacg_cdv.c').read()
     for call in ['DvmType A[64];', 'DvmType B[64];', 'static void jacg_cdv_globals_init(void);',
                  'dvmh_array_create_C(A, 2, -rt_DOUBLE, (DvmType)(N), (DvmType)1, (DvmType)1, (DvmType)(N), (DvmType)1, (DvmType)1);',
-                 'copy_out(a_out, ((void *)dvmh_get_natural_base_C(0, A)));']:
+                 'copy_out(a_out, ((void *)dvmh_get_natural_base_C(0, A)));',
+                 # inherited parameters: headers in both declarations, passed as headers
+                 'static double sweep(DvmType P[64], DvmType Q[64]);',
+                 'static double sweep(DvmType P[64], DvmType Q[64])\n{',
+                 'eps_hist[it - 1] = sweep(A, B);',
+                 'dvmh_region_register_array_C(dvmh_region_2, INTENT_INOUT, P, "P");']:
         assert call in c, call
     spec = ast.literal_eval(open(tmp_path / 'jacg_cdv.loops').read().split('\n', 1)[1])
     assert '#define MAXOF(a, b) ((a) > (b) ? (a) : (b))' in spec['defines']

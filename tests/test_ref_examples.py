@@ -1,7 +1,8 @@
 """The reference's own C-DVMH example programs (examples/jac2d.cdv,
 examples/jacross.cdv, examples/jac3d.cdv, examples/sor.cdv, examples/redb_c.cdv
--- FP32, a function-local distributed array released before `return`), read
-from the reference tree,
+-- FP32, a function-local distributed array released before `return`,
+examples/adi.cdv -- a distributed array passed to a function that inherits it),
+read from the reference tree,
 converted by the front end (emit/cdvmh.py: file-scope distributed arrays and
 variables, the programs' macros, get_actual and fwrite of whole arrays),
 their handlers emitted (emit/hipgen.py), compiled, linked against
@@ -14,6 +15,7 @@ is visible); the programs and their conversions are never stored in the
 repository.  Without the reference tree the module is skipped.
 """
 import os
+import re
 import subprocess
 import sys
 
@@ -28,14 +30,26 @@ sys.path.insert(0, os.path.join(PKG, 'emit'))
 pytestmark = pytest.mark.skipif(not os.path.isdir(REF), reason='the reference tree is not present')
 
 
-def build_and_run(stem, tmp_path):
+def build_and_run(stem, tmp_path, sizes=None):
+    """sizes: {macro: value} -- the program's `#define macro N` lines are
+    rewritten to N in a copy under tmp_path before conversion."""
     import cdvmh
     import hipgen
     out = str(tmp_path)
-    cdvmh.convert_file(os.path.join(REF, stem + '.cdv'), out, prog=stem)
+    src = os.path.join(REF, stem + '.cdv')
+    if sizes:
+        with open(src) as f:
+            text = f.read()
+        for name, val in sizes.items():
+            text, n = re.subn(r'(#define\s+%s\s+)\d+' % name, r'\g<1>%d' % val, text)
+            assert n == 1, name
+        src = os.path.join(out, stem + '.cdv')
+        with open(src, 'w') as f:
+            f.write(text)
+    cdvmh.convert_file(src, out, prog=stem)
     hipgen.emit(os.path.join(out, stem + '.loops'), out)
     inc = ['-I' + os.path.join(REPO, 'include'), '-I' + os.path.join(PKG, 'programs')]
-    cc = ['gcc', '-O2', '-fPIC', '-fopenmp', '-ffp-contract=off'] + inc
+    cc = ['gcc', '-O2', '-fPIC', '-fopenmp', '-ffp-contract=off', '-D_DVMH=1'] + inc
     objs = []
     for src in (f'{stem}.c', f'{stem}_host.c'):
         o = os.path.join(out, src + '.o')
@@ -189,6 +203,54 @@ def ref_redb(n=8, itmax=10, maxeps=np.float32(0.5e-5), w=np.float32(0.5)):
         if eps < maxeps:
             break
     return hist, None
+
+
+def ref_adi(n, itmax=100, maxeps=0.01):
+    """examples/adi.cdv (the array reaches its initialisation through an
+    inherited parameter): boundary 10 i/(n-1) + 10 j/(n-1) + 10 k/(n-1), zero
+    inside; per iteration an in-place sweep along i, then along j, of the mean
+    of the two neighbours on that axis, then along k with eps = max |a - mean|."""
+    a = np.zeros((n, n, n))
+    r = np.arange(n, dtype=np.float64)
+    edge = 10.0 * r / (n - 1)
+    full = edge[:, None, None] + edge[None, :, None] + edge[None, None, :]
+    inner = np.zeros((n, n, n), dtype=bool)
+    inner[1:-1, 1:-1, 1:-1] = True
+    a[~inner] = full[~inner]
+    lines = []
+    for it in range(1, itmax + 1):
+        for i in range(1, n - 1):
+            a[i, 1:-1, 1:-1] = (a[i - 1, 1:-1, 1:-1] + a[i + 1, 1:-1, 1:-1]) / 2
+        for j in range(1, n - 1):
+            a[1:-1, j, 1:-1] = (a[1:-1, j - 1, 1:-1] + a[1:-1, j + 1, 1:-1]) / 2
+        eps = 0.0
+        for k in range(1, n - 1):
+            t = (a[1:-1, 1:-1, k - 1] + a[1:-1, 1:-1, k + 1]) / 2
+            eps = max(eps, float(np.abs(a[1:-1, 1:-1, k] - t).max()))
+            a[1:-1, 1:-1, k] = t
+        lines.append(' IT = %4i   EPS = %14.7E\n' % (it, eps))
+        if eps < maxeps:
+            break
+    return ''.join(lines), eps
+
+
+def test_reference_adi(tmp_path):
+    """examples/adi.cdv at 20^3: every iteration's printed eps equals the
+    restatement's; the timing line aside, the closing report follows."""
+    out, _ = build_and_run('adi', tmp_path, sizes={'nx': 20, 'ny': 20, 'nz': 20})
+    lines, eps = ref_adi(20)
+    assert out.startswith(lines)
+    tail = out[len(lines):].splitlines()
+    assert tail[0] == ' ADI Benchmark Completed.' and tail[1] == ' Size            =   20 x   20 x   20'
+    assert tail[-2] == ' Verification    =       %12s' % ('SUCCESSFUL' if abs(eps - 0.07249074) < 1e-6 else 'UNSUCCESSFUL')
+
+
+def test_reference_adi_full_size(tmp_path):
+    """examples/adi.cdv as written (384^3, 100 iterations, about 10 s on the
+    host path): the program's own check |eps - 0.07249074| < 1e-6 passes."""
+    out, _ = build_and_run('adi', tmp_path)
+    assert ' IT =  100   EPS =  7.2490737E-02\n' in out
+    assert ' Verification    =         SUCCESSFUL\n' in out
 
 
 @pytest.mark.parametrize('stem,ref', [('jac2d', ref_jac2d), ('jacross', ref_jacross), ('jac3d', ref_jac3d),
