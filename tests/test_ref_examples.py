@@ -1,14 +1,16 @@
 """The reference's own C-DVMH example programs (examples/jac2d.cdv,
 examples/jacross.cdv, examples/jac3d.cdv, examples/sor.cdv, examples/redb_c.cdv
 -- FP32, a function-local distributed array released before `return`,
-examples/adi.cdv -- a distributed array passed to a function that inherits it),
+examples/adi.cdv -- a distributed array passed to a function that inherits it,
+and the FP32 benchmarks examples/jac3d_perf.cdv and examples/jac2d_perf.cdv),
 read from the reference tree,
 converted by the front end (emit/cdvmh.py: file-scope distributed arrays and
 variables, the programs' macros, get_actual and fwrite of whole arrays),
 their handlers emitted (emit/hipgen.py), compiled, linked against
 libdvmh_amd and run.  What each prints (the per-iteration MAX reduction) and
 the array it writes to jacobi.dat must equal a numpy restatement of the
-program, bit for bit.
+program, bit for bit; the benchmark programs' own verification lines must
+read SUCCESSFUL.
 
 These run where the reference tree is present (on the host path when no GPU
 is visible); the programs and their conversions are never stored in the
@@ -250,6 +252,17 @@ def test_reference_adi_full_size(tmp_path):
     host path): the program's own check |eps - 0.07249074| < 1e-6 passes."""
     out, _ = build_and_run('adi', tmp_path)
     assert ' IT =  100   EPS =  7.2490737E-02\n' in out
+    assert ' Verification    =         SUCCESSFUL\n' in out
+
+
+@pytest.mark.parametrize('stem,last', [('jac3d_perf', ' IT =  100   EPS =  5.0580444E+00\n'),
+                                       ('jac2d_perf', ' IT =  100   EPS =  5.8375977E+01\n')])
+def test_reference_benchmark_checks(stem, last, tmp_path):
+    """examples/jac3d_perf.cdv (FP32 384^3) and examples/jac2d_perf.cdv (FP32
+    8000^2), 100 iterations each as written: the programs' own checks
+    (|eps - 5.058044| < 1e-4, |eps - 58.37598| < 1e-3) pass."""
+    out, _ = build_and_run(stem, tmp_path)
+    assert last in out
     assert ' Verification    =         SUCCESSFUL\n' in out
 
 
