@@ -997,6 +997,17 @@ extern "C" DvmType dvmh_loop_has_element_C(DvmType curLoop, const DvmType dvmDes
     return inside ? 1 : 0;
 }
 
+// A workgroup's waves are dealt over the compute unit's 4 SIMDs; each SIMD has
+// 512 vector registers per lane, given to a wave in granules of 8.
+extern "C" DvmType dvmh_amd_block_fits_registers_C(const DvmType block[3], DvmType regsPerThread) {
+    if (regsPerThread <= 0) return 1;
+    long threads = (long)block[0] * block[1] * block[2];
+    long waves = (threads + 63) / 64;
+    long perSimd = (waves + 3) / 4;
+    long alloc = (regsPerThread + 7) / 8 * 8;
+    return perSimd * alloc <= 512 ? 1 : 0;
+}
+
 // dvmh_loop_cuda_get_config_C (:4281) -> DvmhLoopCuda::pickBlock (rts/dvmh/loop.cpp:2134-2199)
 extern "C" void dvmh_loop_cuda_get_config_C(DvmType curLoop, DvmType sharedPerThread, DvmType regsPerThread, void *inOutThreads,
         void *outStream, DvmType *outSharedPerBlock) {
@@ -1033,7 +1044,16 @@ extern "C" void dvmh_loop_cuda_get_config_C(DvmType curLoop, DvmType sharedPerTh
         block[0] = std::max(1, (int)std::min<long>(256, gpu.props.ldsPerBlock / sharedPerThread) / 64 * 64);
         block[1] = block[2] = 1;
     }
-    (void)regsPerThread;
+    if (regsPerThread > 0) {
+        const DvmType b[3] = {block[0], block[1], block[2]};
+        if (!dvmh_amd_block_fits_registers_C(b, regsPerThread)) {
+            DVMH_CHECK(!l->userBlockFlag, "Register count is insufficient: %ld registers per thread do not fit a block of %d threads",
+                    (long)regsPerThread, block[0] * block[1] * block[2]);
+            long perSimdWaves = 512 / ((regsPerThread + 7) / 8 * 8);
+            block[0] = (int)std::max<long>(64, std::min<long>(256, 4 * 64 * perSimdWaves) / 64 * 64);
+            block[1] = block[2] = 1;
+        }
+    }
     if (thr) { thr[0] = block[0]; thr[1] = block[1]; thr[2] = block[2]; }
     if (outStream) *(hipStream_t *)outStream = sl->stream;
     if (outSharedPerBlock) *outSharedPerBlock = (DvmType)block[0] * block[1] * block[2] * sharedPerThread;

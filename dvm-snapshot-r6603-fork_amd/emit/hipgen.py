@@ -399,7 +399,15 @@ def red_kernel_tail(L, slot, accumulate):
     return s
 
 
-def handler_prologue(L, kind_const, want_ext=False, autotfm=False):
+def regs_query(kernel, loop_ref='loop_ref'):
+    """Handler lines that ask once for the registers per thread of its main
+    kernel and hand them to dvmh_loop_cuda_get_config_C."""
+    return ['static int dvmh_regs = -1;   /* registers per thread of the main kernel, asked once */',
+            f'if (dvmh_regs < 0) dvmh_regs = dvmh_dev::kernelRegs(reinterpret_cast<const void *>(&{kernel}));',
+            f'dvmh_loop_cuda_get_config_C({loop_ref}, 0, dvmh_regs, &threads, &stream, 0);']
+
+
+def handler_prologue(L, kind_const, want_ext=False, autotfm=False, regs_kernel=None):
     h = []
     h.append('DvmType tmpVar;')
     h.append('/* Loop reference and device number */')
@@ -425,7 +433,7 @@ def handler_prologue(L, kind_const, want_ext=False, autotfm=False):
     h.append('/* Choose index type for the kernel */')
     h.append('int kernelIndexT = (int)dvmh_loop_guess_index_type_C(loop_ref);')
     h.append('if (kernelIndexT == rt_LONG) kernelIndexT = rt_LLONG;')
-    h.append('dvmh_loop_cuda_get_config_C(loop_ref, 0, 0, &threads, &stream, 0);')
+    h += regs_query(regs_kernel)
     h.append('dvmh_loop_fill_bounds_C(loop_ref, boundsLow, boundsHigh, loopSteps);')
     return h
 
@@ -781,7 +789,8 @@ def march_kernel(L):
 def parallel_handler(L):
     r = L.rank
     h = [f'extern "C" void {L.name}_cuda(DvmType *pLoopRef{"".join(", DvmType " + p[1] + "_hdr[]" if p[0] == "array" else ", " + p[2] + " *" + p[1] + "_ptr" for p in L.params)}) {{']
-    b = handler_prologue(L, 'DVMH_AMD_KIND_STENCIL' if L.stencil else 'DVMH_AMD_KIND_PARALLEL')
+    b = handler_prologue(L, 'DVMH_AMD_KIND_STENCIL' if L.stencil else 'DVMH_AMD_KIND_PARALLEL',
+                         regs_kernel=f'{L.name}_kernel<int, 1>')
     for p in L.arrays:
         b.append(f'if ({p[1]}_tfm != 0) {{ fprintf(stderr, "{L.name}: array {p[1]} is not in natural layout\\n"); abort(); }}')
     b.append('int depth = (int)tune[DVMH_AMD_T_DEPTH];')
@@ -1093,7 +1102,7 @@ def batch_handler(L):
          'dvmh_amd_loop_get_tuning_C(lr0, DVMH_AMD_KIND_BATCH, tune);',
          'dim3 threads(0, 0, 0);',
          'hipStream_t stream;',
-         'dvmh_loop_cuda_get_config_C(lr0, 0, 0, &threads, &stream, 0);',
+         *regs_query(f'{L.name}_batch_kernel<int, 1>', 'lr0'),
          'int depth = (int)tune[DVMH_AMD_T_DEPTH];',
          'if (depth != 1 && depth != 2 && depth != 4 && depth != 8) depth = 1;',
          'int bvariant = 0, mcols = 1, npf = 0;',
@@ -1887,7 +1896,7 @@ def across_handler(L):
     b = ['DvmType dependency_mask = dvmh_loop_get_dependency_mask_C(*pLoopRef);',
          'int dependency_num = __builtin_popcountl((unsigned long)dependency_mask);',
          f'DvmType kind = dependency_num <= 1 ? DVMH_AMD_KIND_LINE : DVMH_AMD_KIND_WAVEFRONT;']
-    pro = handler_prologue(L, 'kind', want_ext=True, autotfm=True)
+    pro = handler_prologue(L, 'kind', want_ext=True, autotfm=True, regs_kernel=f'{L.name}_line0_kernel<int>')
     b += pro
     for p in L.arrays:
         _, nm, ct, ar = p

@@ -80,6 +80,14 @@ const char *dvmh_amd_device_name_C(void);
  * diagonal device layout (rts/dvmh/include/dvmhlib_device.h:56-98).  The same
  * function the layout kernels and the handlers' DiagView use. */
 long dvmh_amd_convert_xy_C(long x, long y, long Rx, long Ry, long slash);
+/* 1 if a block[0] x block[1] x block[2] workgroup of a kernel using
+ * regsPerThread vector registers fits one compute unit's register file (its
+ * waves spread over the 4 SIMDs, each with 512 registers per lane, allocated
+ * in granules of 8), else 0.  dvmh_loop_cuda_get_config_C applies this rule
+ * to the count a handler passes (rts/dvmh/loop.cpp:2177-2194 with ptxas
+ * counts): a block that does not fit becomes (min(256, fitting threads), 1, 1),
+ * or is an error if the program set it. */
+DvmType dvmh_amd_block_fits_registers_C(const DvmType block[3], DvmType regsPerThread);
 
 /* ---- batching of many small loops -------------------------------------------
  * Between begin and end, dvmh_loop_perform_C of loops that registered a batch

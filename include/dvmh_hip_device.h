@@ -33,6 +33,20 @@
 
 namespace dvmh_dev {
 
+// ------------------------------------------------------------------ resources
+// Registers per thread of a compiled kernel (0 if the runtime cannot say): the
+// figure handlers pass to dvmh_loop_cuda_get_config_C so the block is fitted to
+// the register file, as the reference's handlers pass the counts ptxas reports
+// (driver/src/dvm_comp:400-432 -> rts/dvmh/loop.cpp:2177-2194).
+static inline int kernelRegs(const void *kernel) {
+    hipFuncAttributes a;
+    if (hipFuncGetAttributes(&a, kernel) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return a.numRegs > 0 ? a.numRegs : 0;
+}
+
 // ------------------------------------------------------------------ views
 template <typename T, typename I, int R, bool UNIT>
 struct View {

@@ -1,5 +1,6 @@
 """The drop-in boundary: libdvmh_amd.so loads without a GPU and exports every
 function that include/*.h declares (no compute calls)."""
+import ctypes
 import os
 import re
 import subprocess
@@ -57,3 +58,22 @@ def test_library_loads_and_reports_no_gpu_here_or_a_gpu_there():
 def test_entry_points_count():
     """The reference ABI subset (dvmhlib2.h) is the bulk of the boundary."""
     assert len(declared_functions(os.path.join(INCLUDE, 'dvmhlib2.h'))) >= 90
+
+
+@pytest.mark.parametrize('block,regs,fits', [
+    ((1024, 1, 1), 128, 1), ((1024, 1, 1), 129, 0),   # 16 waves: 4 per SIMD, 512 / 4 = 128 registers each
+    ((32, 4, 8), 128, 1), ((32, 4, 8), 136, 0),        # the same 1024 threads as a 3-D block
+    ((256, 1, 1), 512, 1), ((256, 1, 1), 513, 0),      # one wave per SIMD may take the whole file
+    ((64, 1, 1), 512, 1),
+    ((512, 1, 1), 256, 1), ((512, 1, 1), 257, 0),      # 257 rounds up to 264 in granules of 8
+    ((320, 1, 1), 256, 1), ((320, 1, 1), 257, 0),      # 5 waves: two share a SIMD
+    ((1024, 1, 1), 0, 1)])                             # no count given: no constraint
+def test_block_register_fit_rule(block, regs, fits):
+    """The rule dvmh_loop_cuda_get_config_C applies to the registers per
+    thread a handler reports for its kernel (the workgroup's waves over the
+    compute unit's four SIMDs, 512 registers per lane each)."""
+    r = D.runtime()
+    r.dvmh_amd_block_fits_registers_C.restype = ctypes.c_long
+    r.dvmh_amd_block_fits_registers_C.argtypes = [ctypes.c_void_p, ctypes.c_long]
+    b = (ctypes.c_long * 3)(*block)
+    assert r.dvmh_amd_block_fits_registers_C(b, regs) == fits
