@@ -894,6 +894,7 @@ class Converter:
             {p.split()[-1] for p in privates}
         local |= self.body_locals(body)
         params, written = [], set()
+        inreg = ctx['region'] is not None
         for t in tokenize(body):
             if t.kind != 'id' or t.text in local or t.text not in self.scope:
                 continue
@@ -904,8 +905,13 @@ class Converter:
                 params.append(('array', t.text, v.ctype, v.dvm.rank))
             elif not v.dims and not v.pointer_rank:
                 params.append(('scalar', t.text, v.ctype))
+            elif not inreg and v.dims and not v.pointer_rank and not v.is_param and all(v.dims):
+                # a loop outside regions runs on the host, where the program's
+                # own C arrays are at hand: passed by address
+                params.append(('carray', t.text, v.ctype, list(v.dims)))
             else:
-                self.err(d.line, f'{t.text}: C arrays and pointers inside parallel loops are not supported')
+                self.err(d.line, f'{t.text}: C arrays and pointers inside parallel loops are not supported'
+                         + (' (inside regions)' if inreg and v.dims else ''))
         for t_ in re.finditer(r'\b([A-Za-z_]\w*)\s*(?:\[[^;]*?\]\s*)+(?:=(?!=)|\+=|-=|\*=|/=|\+\+|--)', body):
             written.add(t_.group(1))
         if target not in [p[1] for p in params]:
@@ -922,10 +928,11 @@ class Converter:
             desc['privates'] = privates
         if shadows:
             desc['stencil'] = shadows
+        if not inreg:
+            desc['host_only'] = True
         self.loops.append(desc)
-        self.handlers.append((name, params))
+        self.handlers.append((name, params, not inreg))
         # region bookkeeping
-        inreg = ctx['region'] is not None
         if inreg:
             for p in params:
                 if p[0] == 'array':
@@ -956,7 +963,7 @@ class Converter:
         if cuda_block:
             cb = (cuda_block + ['1', '1'])[:3]
             out.append(f'dvmh_loop_set_cuda_block_C({lp}, {", ".join(cb)});')
-        pargs = ', '.join(p[1] if p[0] == 'array' else '&' + p[1] for p in params)
+        pargs = ', '.join(p[1] if p[0] in ('array', 'carray') else '&' + p[1] for p in params)
         out.append(f'dvmh_loop_register_handler_C({lp}, DEVICE_TYPE_HOST, HANDLER_TYPE_PARALLEL, '
                    f'dvmh_handler_func_C((DvmHandlerFunc){name}_host, {len(params)}, {pargs}));')
         if inreg:
@@ -1018,10 +1025,12 @@ class Converter:
     # -- outputs
     def program_text(self, converted):
         protos = []
-        for name, params in self.handlers:
-            ps = ', '.join(['DvmType *'] + ['DvmType[]' if p[0] == 'array' else f'{p[2]} *' for p in params])
+        for name, params, host_only in self.handlers:
+            ps = ', '.join(['DvmType *'] + ['DvmType[]' if p[0] == 'array' else ('void *' if p[0] == 'carray' else f'{p[2]} *')
+                                            for p in params])
             protos.append(f'void {name}_host({ps});')
-            protos.append(f'void {name}_cuda({ps});')
+            if not host_only:
+                protos.append(f'void {name}_cuda({ps});')
         head = (f'/* {self.file} converted by emit/cdvmh.py: directives replaced by calls of the DVMH C\n'
                 f' * interface; loop handlers are emitted from {self.prog}.loops. */\n'
                 '#include "dvmhlib2.h"\n' + '\n'.join(protos) + '\n')

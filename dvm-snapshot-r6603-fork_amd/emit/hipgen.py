@@ -67,7 +67,9 @@ class Loop:
         self.vars = d['vars']                      # [(name, ctype)] outermost first
         self.rank = len(self.vars)
         self.steps = d.get('steps', ['1'] * self.rank)   # constant steps as text, None = runtime
-        self.params = d['params']                  # [('array', name, ctype, rank) | ('scalar', name, ctype)]
+        self.params = d['params']                  # [('array', name, ctype, rank) | ('scalar', name, ctype)
+                                                   #  | ('carray', name, ctype, [extents])]
+        self.host_only = d.get('host_only', False)  # a loop outside regions: no GPU handler
         self.privates = d.get('privates', [])      # [(decl text)]
         self.reds = d.get('reductions', [])        # [(func, var, ctype)] or [(func, var, ctype, loc, loctype)]
         self.across = d.get('across')              # None | {'array': name, 'widths': [(lo, hi), ...]}
@@ -193,7 +195,7 @@ def host_handler(L):
     out = []
     formal = ['DvmType *pLoopRef']
     for p in L.params:
-        formal.append(f'DvmType {p[1]}_hdr[]' if p[0] == 'array' else f'{p[2]} *{p[1]}_ptr')
+        formal.append(f'DvmType {p[1]}_hdr[]' if p[0] == 'array' else ('void *' if p[0] == 'carray' else f'{p[2]} *') + f'{p[1]}_ptr')
     out.append(f'void {L.name}_host({", ".join(formal)}) {{')
     b = []
     b.append('/* Loop reference and device number */')
@@ -207,6 +209,11 @@ def host_handler(L):
             for j in range(2, ar + 1):
                 dims += f'[{nm}_hdr[{j - 1}]/{nm}_hdr[{j}]]' if j < ar else f'[{nm}_hdr[{j - 1}]]'
             b.append(f'{ct} (*{nm}){dims} = ({ct} (*){dims})dvmh_get_natural_base_C(device_num, {nm}_hdr);')
+        elif p[0] == 'carray':
+            # an ordinary C array of the program (loops outside regions run on the host)
+            _, nm, ct, ext = p
+            dims = ''.join(f'[{e}]' for e in ext[1:])
+            b.append(f'{ct} (*{nm}){dims} = ({ct} (*){dims}){nm}_ptr;')
         else:
             b.append(f'{p[2]} {p[1]} = *{p[1]}_ptr;')
     b.append('/* Supplementary variables for loop handling */')
@@ -2031,7 +2038,8 @@ def emit(spec_path, outdir):
     hip += [f'#include "{x}"' for x in incs] + defs
     hip.append('')
     for L in loops:
-        hip.append(gpu_loop(L))
+        if not L.host_only:
+            hip.append(gpu_loop(L))
     os.makedirs(outdir, exist_ok=True)
     with open(os.path.join(outdir, f'{prog}_host.c'), 'w') as f:
         f.write('\n'.join(host))

@@ -93,12 +93,14 @@ BAD = [
      'for (i = 0; i < n; i++) for (j = 0; j < n; j++) A[i][j] = 0;\n}\n', 'remote_access'),
     ('non-unit subscript', '#pragma dvm region\n{\n#pragma dvm parallel([i][j] on A[2 * i][j])\n'
      'for (i = 0; i < n; i++) for (j = 0; j < n; j++) A[i][j] = 0;\n}\n', 'subscript'),
+    ('C array in a region', '#pragma dvm region\n{\n#pragma dvm parallel([i][j] on A[i][j])\n'
+     'for (i = 0; i < n; i++) for (j = 0; j < n; j++) A[i][j] = C[i][j];\n}\n', 'inside regions'),
 ]
 
 
 @pytest.mark.parametrize('what,body,msg', BAD, ids=[b[0] for b in BAD])
 def test_unsupported_constructs_name_the_line(what, body, msg):
-    src = ('void f(long n)\n{\n    int i, j, x;\n    #pragma dvm array distribute[block][block]\n    double (*A)[n];\n'
+    src = ('void f(long n)\n{\n    int i, j, x;\n    double C[8][8];\n    #pragma dvm array distribute[block][block]\n    double (*A)[n];\n'
            '    A = malloc(n * n * sizeof(double));\n' + body + '}\n')
     with pytest.raises(cdvmh.ConvertError) as e:
         cv = cdvmh.Converter(src, 't.cdv', 't')
@@ -365,6 +367,27 @@ def test_jacg_gpu():
     before = _gpu_loops()
     _check_jacg(6)
     assert _gpu_loops() > before
+
+
+def test_host_loop_with_c_array():
+    """A parallel loop outside regions runs on the host only: an ordinary C
+    array of the program reaches its host handler by address, and no GPU
+    handler is made for it."""
+    src = ('#define M 6\nvoid f(long n)\n{\n    int i, j, s;\n    int C[5][M];\n'
+           '    #pragma dvm array distribute[block][block]\n    int (*A)[M];\n    A = malloc(n * M * sizeof(int));\n'
+           '    s = 0;\n    #pragma dvm parallel([i][j] on A[i][j]) reduction(sum(s))\n'
+           '    for (i = 0; i < 5; i++)\n        for (j = 0; j < M; j++)\n            s = s + A[i][j] * C[i][j];\n}\n')
+    cv = cdvmh.Converter(src, 't.cdv', 't')
+    text = cv.program_text(cv.convert())
+    (L,) = cv.loops
+    assert L['host_only'] and ('carray', 'C', 'int', ['5', 'M']) in L['params']
+    assert 'void loop_t_10_host(DvmType *, DvmType[], void *);' in text
+    assert 'loop_t_10_cuda' not in text
+    assert 'dvmh_handler_func_C((DvmHandlerFunc)loop_t_10_host, 2, A, C)' in text
+    sys.path.insert(0, os.path.join(PKG, 'emit'))
+    import hipgen
+    host = hipgen.host_handler(hipgen.Loop(L, 't'))
+    assert 'int (*C)[M] = (int (*)[M])C_ptr;' in host
 
 
 def test_file_scope_conversion(tmp_path):

@@ -2,7 +2,8 @@
 examples/jacross.cdv, examples/jac3d.cdv, examples/sor.cdv, examples/redb_c.cdv
 -- FP32, a function-local distributed array released before `return`,
 examples/adi.cdv -- a distributed array passed to a function that inherits it,
-and the FP32 benchmarks examples/jac3d_perf.cdv and examples/jac2d_perf.cdv),
+the FP32 benchmarks examples/jac3d_perf.cdv and examples/jac2d_perf.cdv, and
+examples/testcdvm.cdv -- an ordinary C array in a parallel loop outside regions),
 read from the reference tree,
 converted by the front end (emit/cdvmh.py: file-scope distributed arrays and
 variables, the programs' macros, get_actual and fwrite of whole arrays),
@@ -253,6 +254,15 @@ def test_reference_adi_full_size(tmp_path):
     out, _ = build_and_run('adi', tmp_path)
     assert ' IT =  100   EPS =  7.2490737E-02\n' in out
     assert ' Verification    =         SUCCESSFUL\n' in out
+
+
+def test_reference_testcdvm(tmp_path):
+    """examples/testcdvm.cdv: a region fills A and the 8-neighbour sum into B;
+    a parallel loop outside regions (on the host) compares B with the same sum
+    over an ordinary C array and MIN-reduces the first mismatch; the program
+    reports complete when there is none."""
+    out, _ = build_and_run('testcdvm', tmp_path)
+    assert out == 'T_CDVM  -  complete\n'
 
 
 @pytest.mark.parametrize('stem,last', [('jac3d_perf', ' IT =  100   EPS =  5.0580444E+00\n'),
