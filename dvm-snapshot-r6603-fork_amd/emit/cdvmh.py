@@ -319,6 +319,9 @@ class Converter:
                 i += 1
                 stmt = i
                 continue
+            if t.text == '{' and t.kind == 'op' and i > 0 and toks[i - 1].text == '=':
+                i = matching(toks, i) + 1    # the initialiser of a file-scope variable
+                continue
             if t.text == '{' and t.kind == 'op':
                 if pending:
                     self.err(pending.line, 'an array directive at file scope is followed by a declaration')

@@ -372,8 +372,9 @@ def test_jacg_gpu():
 def test_host_loop_with_c_array():
     """A parallel loop outside regions runs on the host only: an ordinary C
     array of the program reaches its host handler by address, and no GPU
-    handler is made for it."""
-    src = ('#define M 6\nvoid f(long n)\n{\n    int i, j, s;\n    int C[5][M];\n'
+    handler is made for it.  (A file-scope initialiser in braces is passed
+    through.)"""
+    src = ('#define M 6\nstatic const int W[3] = {1, 2, 3};\nvoid f(long n)\n{\n    int i, j, s;\n    int C[5][M];\n'
            '    #pragma dvm array distribute[block][block]\n    int (*A)[M];\n    A = malloc(n * M * sizeof(int));\n'
            '    s = 0;\n    #pragma dvm parallel([i][j] on A[i][j]) reduction(sum(s))\n'
            '    for (i = 0; i < 5; i++)\n        for (j = 0; j < M; j++)\n            s = s + A[i][j] * C[i][j];\n}\n')
@@ -381,9 +382,9 @@ def test_host_loop_with_c_array():
     text = cv.program_text(cv.convert())
     (L,) = cv.loops
     assert L['host_only'] and ('carray', 'C', 'int', ['5', 'M']) in L['params']
-    assert 'void loop_t_10_host(DvmType *, DvmType[], void *);' in text
-    assert 'loop_t_10_cuda' not in text
-    assert 'dvmh_handler_func_C((DvmHandlerFunc)loop_t_10_host, 2, A, C)' in text
+    assert 'void loop_t_11_host(DvmType *, DvmType[], void *);' in text
+    assert 'loop_t_11_cuda' not in text
+    assert 'dvmh_handler_func_C((DvmHandlerFunc)loop_t_11_host, 2, A, C)' in text
     sys.path.insert(0, os.path.join(PKG, 'emit'))
     import hipgen
     host = hipgen.host_handler(hipgen.Loop(L, 't'))
