@@ -15,9 +15,10 @@ inputs resident in HBM.  Three forms run side by side:
 `value` is the chosen form's point-updates/s.  The roofline uses the stencil
 kernel's average duration measured with HIP events on the runtime's stream
 during the timed region.  The batched-grid workload (512 uneven 2-D grids) and
-the other configurations are reported in the same JSON line.  The CPU baseline
-is the oracle (the C/OpenMP restatement of the host handlers) timed on a
-bounded sample of the headline workload on this machine's host cores.
+the other configurations are reported in the same JSON line, each with a
+parity check of its timed form at full size.  The CPU baseline is the DVMH
+host path (the product's emitted OpenMP host handlers) timed on a bounded
+sample of the headline workload on this machine's host cores.
 """
 import argparse
 import ctypes
@@ -201,18 +202,61 @@ def batched_leg(rt, iters):
             'speedup': res['batched']['value'] / res['per_grid']['value'], 'parity_bit_exact': bool(ok)}
 
 
-def other_configs(rt):
-    """The remaining configurations, plain against the cost-model form, one
-    timed run each (their parity is covered by the GPU test suite)."""
-    from dvmh_amd import workloads as W
-    specs = [('jac2d', 2048, 100, 16 + 24, 'C1 2-D Jacobi 2048^2 x 100 (two loops per iteration)'),
-             ('sor3d', 320, 30, 16, 'C3 SOR 320^3 x 30 (ACROSS on three axes, REDUCTION(MAX))'),
-             ('adi3d', 256, 20, 6 * 16 + 8, 'C4 ADI 256^3 x 20 (six line sweeps + MAXLOC)')]
+def other_parity(rt, name, n, iters):
+    """The chosen form (cost-model configuration settled by the timed legs,
+    results synchronised only when needed, repeating regions replayed as HIP
+    graphs) at the size it is timed, for `iters` iterations from a fresh
+    setup, against the oracle: grids bit-exact, MAX / MAXLOC values and
+    indices exact."""
+    import helpers as H
+    set_form(rt, 2, 1, 1)
+    flags = D.REGION_AMD_GRAPH
+    replays0 = D.stats()['graph_replays']
+    if name == 'jac2d':
+        res, ref = H.run_jac2d(n, iters, flags=flags), H.oracle_jac2d(n, iters)
+        ok = {'grid_bit_exact': bool(H.bits_equal(res['A'], ref['A']) and H.bits_equal(res['B'], ref['B'])),
+              'max_exact': bool(np.array_equal(res['eps'], ref['eps']))}
+    elif name == 'sor3d':
+        res, ref = H.run_sor3d(n, iters, flags=flags), H.oracle_sor3d(n, iters)
+        ok = {'grid_bit_exact': bool(H.bits_equal(res['A'], ref['A'])),
+              'max_exact': bool(np.array_equal(res['eps'], ref['eps']))}
+    else:
+        res, ref = H.run_adi3d(n, iters, flags=flags), H.oracle_adi3d(n, iters)
+        ok = {'grid_bit_exact': bool(H.bits_equal(res['A'], ref['A'])),
+              'max_exact': bool(np.array_equal(res['max'], ref['max'])),
+              'loc_exact': bool(np.array_equal(res['loc'], ref['loc']))}
+    ok['iterations'] = iters
+    ok['graph_replays'] = D.stats()['graph_replays'] - replays0
+    set_form(rt, 0, 0, 0)
+    return ok
+
+
+def chosen_layouts(rt, w):
+    """Kernel variant and device layout the tuner kept at each loop site of the
+    workload (from the tuner's query; transform 0 = natural layout)."""
     out = {}
-    for name, n, iters, bpp, desc in specs:
+    q = (ctypes.c_long * 13)()
+    for line in w.lines:
+        if rt.dvmh_amd_tuner_query_C(w.cdv, line, q) > 0:
+            out[str(line)] = {'block': [q[2], q[3], q[4]], 'variant': q[5 + 2], 'tile_a': q[5 + 4], 'tile_b': q[5 + 5],
+                              'chunk': q[5 + 3], 'transform': q[5 + 6]}
+    return out
+
+
+def other_configs(rt, parity=True):
+    """The remaining configurations, plain against the chosen (cost-model)
+    form.  Each form: wall time of a run without timing events, then the
+    kernels' time from HIP events on a second run whose launches are issued
+    eagerly (graph replays carry no timing marks), so that `frac` is the
+    kernels' algorithmic bytes over their own time.  The chosen form is then
+    checked against the oracle at the same size."""
+    from dvmh_amd import workloads as W
+    specs = [('jac2d', 2048, 100, 4, 'C1 2-D Jacobi 2048^2 x 100 (two loops per iteration)'),
+             ('sor3d', 320, 30, 3, 'C3 SOR 320^3 x 30 (ACROSS on three axes, REDUCTION(MAX))'),
+             ('adi3d', 256, 20, 3, 'C4 ADI 256^3 x 20 (six line sweeps + MAXLOC; 6 x 255 x 256^2 point-updates per step)')]
+    out = {}
+    for name, n, iters, piters, desc in specs:
         r = {'workload': desc}
-        # chosen: cost-model configuration, results synchronised only when needed,
-        # each repeating time-step region captured once and replayed as a HIP graph
         for form, mode, sync, graph in (('plain', 0, 0, 0), ('chosen', 2, 1, 1)):
             set_form(rt, mode, sync, graph)
             w = W.ALL[name](n, flags=D.REGION_AMD_GRAPH if graph else 0)
@@ -222,48 +266,75 @@ def other_configs(rt):
                 if mode == 0 or sum(rt.dvmh_amd_tuner_site_pending_C(w.cdv, ln) for ln in w.lines) == 0:
                     break
             w.run(3)   # settled: the repeating region is captured here, before timing
-            sec = w.run(iters)   # wall time without events; kernel time on a second run
+            sec = w.run(iters)
+            rt.dvmh_amd_set_graph_C(0)
             rt.dvmh_amd_reset_kernel_stats_C()
             rt.dvmh_amd_set_kernel_timing_C(1)
             w.run(iters)
             rt.dvmh_amd_set_kernel_timing_C(0)
+            rt.dvmh_amd_set_graph_C(graph)
             launches, ns = kernel_stats(rt, w.cdv, w.lines)
+            layouts = chosen_layouts(rt, w) if mode == 2 else None
             w.teardown()
-            pts = w.points_per_iter()
-            # kernel time is only observable for eagerly issued launches (graph
-            # replays carry no timing marks); frac_wall uses the whole run's time
-            r[form] = {'value': pts * iters / sec, 'ms': sec * 1e3,
-                       'kernel_ms': ns * 1e-6 if launches else None,
-                       'frac': pts * iters * bpp / (ns * 1e-9) / HBM_PEAK if ns else None,
-                       'frac_wall': pts * iters * bpp / sec / HBM_PEAK}
+            upd, byt = w.points_per_iter(), w.bytes_per_iter()
+            r[form] = {'value': upd * iters / sec, 'ms': sec * 1e3,
+                       'kernel_ms': ns * 1e-6 if launches else None, 'kernel_launch_marks': launches,
+                       'frac': byt * iters / (ns * 1e-9) / HBM_PEAK if ns else None,
+                       'frac_wall': byt * iters / sec / HBM_PEAK,
+                       'point_updates_per_iteration': upd, 'bytes_per_iteration': byt}
+            if layouts is not None:
+                r[form]['sites'] = layouts
         r['speedup'] = r['chosen']['value'] / r['plain']['value']
+        if parity:
+            r['parity'] = other_parity(rt, name, n, piters)
+            log(name, 'parity', r['parity'])
         out[name] = r
     set_form(rt, 0, 0, 0)
     return out
 
 
 def cpu_baseline(target_seconds=12.0):
-    """The oracle's host-handler restatement on all host cores, bounded sample."""
+    """The DVMH host path on the host cores, on a bounded sample of the
+    headline workload: the jac3d program with its regions run on the host,
+    i.e. through the emitted host handlers (cdvmh's host-handler shape, OpenMP
+    over the first loop axis, the loop body's own arithmetic and per-thread
+    reduction partials merged as red_post does).  The oracle's restatement on
+    the same cores is reported beside it as a second figure."""
     import helpers as H
+    rt = D.runtime()
+    rt.dvmh_amd_host_slots_C.restype = ctypes.c_long
+    threads = int(rt.dvmh_amd_host_slots_C())
+    P = D.Program('jac3d')
+    setup = P.fn('setup', [ctypes.c_long, ctypes.c_ulonglong])
+    iterate = P.fn('iterate', [ctypes.c_int, _dp, _dp], ctypes.c_double)
+    teardown = P.fn('teardown', [])
+    P.set_targets(D.DEVICE_TYPE_HOST, 0)
+    D.reset_stats()
+    setup(N, SEED)
+    one = iterate(2, None, None) / 2
+    its = int(max(2, min(4000, target_seconds / max(one, 1e-4))))
+    sec = iterate(its, None, None)
+    teardown()
+    st = D.stats()
+    P.set_targets(D.DEVICE_TYPE_HOST | D.DEVICE_TYPE_CUDA, 0)
+    assert st['loops_gpu'] == 0 and st['loops_host'] > 0, st
+    # second figure: the oracle (test infrastructure) on the same threads
     o = H.oracle()
-    cores = len(os.sched_getaffinity(0)) if hasattr(os, 'sched_getaffinity') else (os.cpu_count() or 1)
     A = np.zeros(N ** 3)
     o.orc_init_grid3d(H.vp(A), N, SEED, 0.0, 1)
     B = A.copy()
-    eps = np.zeros(1)
-    s = np.zeros(1)
+    oits = max(2, its // 2)
+    eps = np.zeros(oits)
+    sm = np.zeros(oits)
     t0 = time.perf_counter()
-    o.orc_jac3d(H.vp(A), H.vp(B), N, 1, H.vp(eps), H.vp(s), cores)
-    one = time.perf_counter() - t0
-    its = int(max(1, min(4000, target_seconds / max(one, 1e-3))))
-    eps = np.zeros(its)   # the oracle writes one history entry per iteration
-    s = np.zeros(its)
-    t0 = time.perf_counter()
-    o.orc_jac3d(H.vp(A), H.vp(B), N, its, H.vp(eps), H.vp(s), cores)
-    sec = time.perf_counter() - t0
-    return {'value': PTS * its / sec, 'unit': 'point-updates/s', 'cores': cores, 'kind': 'port',
-            'sample': f'{its} iterations of the 384^3 sweep (oracle C/OpenMP host-handler restatement, '
-                      f'{sec:.1f} s on {cores} threads)', 'cpu_model': cpu_model()}
+    o.orc_jac3d(H.vp(A), H.vp(B), N, oits, H.vp(eps), H.vp(sm), threads)
+    osec = time.perf_counter() - t0
+    return {'value': PTS * its / sec, 'unit': 'point-updates/s', 'cores': threads, 'kind': 'port',
+            'sample': f'{its} iterations of the 384^3 Jacobi sweep through the emitted host handlers '
+                      f'(jac3d program, regions on the host; {sec:.1f} s on {threads} OpenMP threads)',
+            'cpu_model': cpu_model(),
+            'oracle_restatement': {'value': PTS * oits / osec, 'iterations': oits, 'seconds': osec,
+                                   'note': 'oracle C/OpenMP restatement (compensated SUM), same threads'}}
 
 
 def cpu_model():

@@ -441,12 +441,14 @@ extern "C" DvmType dvmh_fill_header_C(DvmType deviceNum, const void *baseAddr, c
 
 // ============================================================ actual state
 static void getActual(Data *d) {
+    batchDrain();
     flushDeferredPosts();
     if (d->hasDev && d->devActual && !d->hostActual) d->toHost();
 }
 
 extern "C" void dvmh_get_actual_variable2_(void *addr) {   // :2325
     Guard g;
+    batchDrain();
     flushDeferredPosts();
     Data *d = regularVarOf(addr);
     if (d) getActual(d);
@@ -533,12 +535,14 @@ extern "C" void dvmh_get_actual_subarray_C(const DvmType dvmDesc[], DvmType rank
     va_start(ap, rank);
     subarrayBounds(d, ap, lo, hi);
     va_end(ap);
+    batchDrain();
     flushDeferredPosts();
     if (d->hasDev && d->devActual && !d->hostActual) moveBlock(d, lo, hi, true);
 }
 
 extern "C" void dvmh_get_actual_all2_() {
     Guard g;
+    batchDrain();
     flushDeferredPosts();
     for (Data *d : rt.allData)
         if (d->hasDev && d->devActual && !d->hostActual) d->toHost();
@@ -559,6 +563,7 @@ extern "C" void dvmh_actual_subvariable_C(const void *addr, DvmType rank, ...) {
 extern "C" void dvmh_actual_array2_(const DvmType dvmDesc[]) {
     Guard g;
     Data *d = asData(dvmDesc, "dvmh_actual_array2_");
+    batchDrain();
     if (d->hasDev && d->devActual && !d->hostActual) d->toHost();   // keep elements the host did not write
     d->hostActual = true;
     d->devActual = false;
@@ -568,6 +573,7 @@ extern "C" void dvmh_actual_subarray_C(const DvmType dvmDesc[], DvmType rank, ..
     Guard g;
     Data *d = asData(dvmDesc, "dvmh_actual_subarray_C");
     DVMH_CHECK(rank == d->rank, "dvmh_actual_subarray_C: rank mismatch");
+    batchDrain();
     long lo[MAX_RANK], hi[MAX_RANK];
     va_list ap;
     va_start(ap, rank);
@@ -585,6 +591,7 @@ extern "C" void dvmh_actual_subarray_C(const DvmType dvmDesc[], DvmType rank, ..
 
 extern "C" void dvmh_actual_all2_() {
     Guard g;
+    batchDrain();
     for (Data *d : rt.allData) {
         if (d->hasDev && d->devActual && !d->hostActual) d->toHost();
         d->hostActual = true;
@@ -920,6 +927,15 @@ extern "C" void dvmh_amd_loop_register_batch_handler_C(DvmType curLoop, DvmType 
     Loop *l = asLoop(curLoop, "dvmh_amd_loop_register_batch_handler_C");
     l->batchH = loadHandler(handlerHelper);
     l->hasBatch = true;
+    l->batchChain = false;
+}
+
+extern "C" void dvmh_amd_loop_register_chain_handler_C(DvmType curLoop, DvmType handlerHelper) {
+    Guard g;
+    Loop *l = asLoop(curLoop, "dvmh_amd_loop_register_chain_handler_C");
+    l->batchH = loadHandler(handlerHelper);
+    l->hasBatch = true;
+    l->batchChain = true;
 }
 
 extern "C" void dvmh_loop_perform_C(DvmType curLoop) {   // :3769
@@ -983,7 +999,18 @@ extern "C" DvmType dvmh_loop_guess_index_type_C(DvmType curLoop) {   // :4097
     if (l->region)
         for (auto &ra : l->region->arrays) maxElems = std::max(maxElems, ra.data->elemCount());
     for (int i = 0; i < l->rank; i++) maxElems = std::max(maxElems, std::max(std::labs(l->lo[i]), std::labs(l->hi[i])) + 1);
-    return maxElems < 2147483647L ? rt_INT : rt_LLONG;
+    // an index that does not fit int takes the 8-byte type, which is long on
+    // LP64 (the reference tries int, then long, then long long)
+    if (settings.index64 || maxElems >= 2147483647L) return rt_LONG;
+    return rt_INT;
+}
+
+// Every later loop's handlers are told to use 64-bit indices (1) or to pick
+// by the arrays' extents again (0); for tests of the 64-bit kernels on arrays
+// small enough to check.
+extern "C" void dvmh_amd_set_index64_C(DvmType on) {
+    ensureInit();
+    settings.index64 = on != 0;
 }
 
 extern "C" DvmType dvmh_loop_has_element_C(DvmType curLoop, const DvmType dvmDesc[], DvmType rank, ...) {
@@ -1030,17 +1057,17 @@ extern "C" void dvmh_loop_cuda_get_config_C(DvmType curLoop, DvmType sharedPerTh
     int blockRank = 3;
     while (blockRank > 0 && block[blockRank - 1] == 1) blockRank--;
     DVMH_CHECK(blockRank <= std::max(maxBlockRank, 1),
-            "Setting CUDA block with more dimensions than the parallel loop has (minus 1 in case of dependency) is prohibited");
+            "Launch block of %d dimensions exceeds the %d the loop can spread over threads (its rank, less one when it has a dependency)", blockRank, std::max(maxBlockRank, 1));
     static const char *axisNames[3] = {"x", "y", "z"};
     for (int i = 0; i < 3; i++)
-        DVMH_CHECK(block[i] <= gpu.props.maxBlock[i], "Block size on axis %s (%d) is bigger than device can handle (%d)", axisNames[i],
-                block[i], gpu.props.maxBlock[i]);
-    DVMH_CHECK(block[0] * block[1] * block[2] <= gpu.props.maxThreadsPerBlock, "Block size (%d) is bigger than device can handle (%d)",
+        DVMH_CHECK(block[i] <= gpu.props.maxBlock[i], "Block extent %d along %s exceeds the device limit %d", block[i],
+                axisNames[i], gpu.props.maxBlock[i]);
+    DVMH_CHECK(block[0] * block[1] * block[2] <= gpu.props.maxThreadsPerBlock, "Block of %d threads exceeds the device limit of %d per workgroup",
             block[0] * block[1] * block[2], gpu.props.maxThreadsPerBlock);
     if (sharedPerThread < 0) sharedPerThread = 0;
     long needed = (long)block[0] * block[1] * block[2] * sharedPerThread;
     if (needed > gpu.props.ldsPerBlock) {
-        DVMH_CHECK(!l->userBlockFlag, "Shared memory amount is insufficient. Have %ld, needed %ld", gpu.props.ldsPerBlock, needed);
+        DVMH_CHECK(!l->userBlockFlag, "The requested block needs %ld bytes of LDS, more than the %ld a workgroup can have", needed, gpu.props.ldsPerBlock);
         block[0] = std::max(1, (int)std::min<long>(256, gpu.props.ldsPerBlock / sharedPerThread) / 64 * 64);
         block[1] = block[2] = 1;
     }
@@ -1276,6 +1303,11 @@ extern "C" void dvmh_amd_get_stats_C(DvmType out[8]) {
 }
 
 extern "C" DvmType dvmh_amd_layout_changes_C(void) { return layoutChanges(); }
+
+extern "C" DvmType dvmh_amd_host_slots_C(void) {
+    ensureInit();
+    return hostSlotCount();
+}
 
 extern "C" void dvmh_amd_reset_stats_C(void) {
     rt.loopsOnHost = rt.loopsOnGpu = 0;

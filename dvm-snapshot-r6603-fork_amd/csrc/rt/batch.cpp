@@ -9,6 +9,19 @@
 // group.  Each group runs once: a lone loop through its ordinary handler, a
 // larger group through the batch handler, which sees every loop's bounds and
 // headers and launches one persistent kernel over all of their tiles.
+//
+// A loop registered with a chain handler instead joins the last group when
+// that group is a chain of the same handler, whatever arrays they share: the
+// chain's loops stay in program order and the chain handler runs them with
+// their dependencies kept (consecutive ACROSS sweeps of one array, pipelined
+// a few tile-planes apart).  A chain whose loop site the tuner is still
+// exploring runs loop by loop, so that every execution is timed.
+//
+// Anything that needs the queued loops' results or would run before them
+// (a loop without a batch handler, get_actual, actual, a host region) first
+// runs the queue (batchDrain); the batch stays open.  Scalars a queued loop
+// reads are copied to the device when its region starts, so the loops of one
+// batch must see the same value of a scalar they share.
 #include <cstring>
 
 #include "rt.h"
@@ -22,6 +35,7 @@ namespace dvmh {
 
 long graphReplays = 0, batchGroups = 0, batchLoops = 0;
 static bool batching = false;
+static bool draining = false;
 static std::vector<Loop *> queued;
 static long batchEpoch = 0;   // number of the batch being grouped (Data::batchEpoch)
 
@@ -118,22 +132,30 @@ static void releaseLoop(Loop *l) {
     if (r && --r->pendingBatch == 0 && r->endedInBatch) delete r;
 }
 
+long tunerSitePending(const std::string &file, long line);
+
 static void runQueue() {
     struct Group {
         DvmHandlerFunc func;
+        bool chain;
         std::vector<Loop *> loops;
     };
     std::vector<Group> groups;
     batchEpoch++;
     for (Loop *l : queued) {
         int target = -1;
-        for (int g = (int)groups.size() - 1; g >= 0; g--)
-            if (groups[g].func == l->batchH.func) { target = g; break; }
-        // joining group `target` moves l before the groups after it: allowed
-        // when none of target's or the later groups' loops shares an array
-        if (target >= 0 && lastGroupOf(l) >= target) target = -1;
+        if (l->batchChain) {
+            // a chain only grows at the end of the queue: program order kept
+            if (!groups.empty() && groups.back().chain && groups.back().func == l->batchH.func) target = (int)groups.size() - 1;
+        } else {
+            for (int g = (int)groups.size() - 1; g >= 0; g--)
+                if (!groups[g].chain && groups[g].func == l->batchH.func) { target = g; break; }
+            // joining group `target` moves l before the groups after it: allowed
+            // when none of target's or the later groups' loops shares an array
+            if (target >= 0 && lastGroupOf(l) >= target) target = -1;
+        }
         if (target < 0) {
-            groups.push_back(Group{l->batchH.func, {}});
+            groups.push_back(Group{l->batchH.func, l->batchChain, {}});
             target = (int)groups.size() - 1;
         }
         groups[target].loops.push_back(l);
@@ -141,10 +163,12 @@ static void runQueue() {
     }
     queued.clear();
     for (auto &g : groups) {
-        if (g.loops.size() == 1) {
-            Loop *l = g.loops[0];
-            l->queued = false;
-            performLoopNow(l);
+        const bool exploring = g.chain && tunerSitePending(g.loops[0]->file, g.loops[0]->line) > 0;
+        if (g.loops.size() == 1 || exploring) {
+            for (Loop *l : g.loops) {
+                l->queued = false;
+                performLoopNow(l);
+            }
         } else {
             runBatchGroup(g.loops);
         }
@@ -152,6 +176,16 @@ static void runQueue() {
         if (settings.syncMode == 0) flushDeferredPosts();
     }
 }
+
+// Runs what is queued so far (the batch stays open); see the file comment.
+void batchDrain() {
+    if (!batching || draining || queued.empty()) return;
+    draining = true;
+    runQueue();
+    draining = false;
+}
+
+bool batchOpen() { return batching; }
 
 }  // namespace dvmh
 

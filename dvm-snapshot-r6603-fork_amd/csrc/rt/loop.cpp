@@ -1,5 +1,6 @@
 // Loop execution: handler trampoline, dvmh_loop_perform_C's sequence, GPU
 // reductions and their (optionally deferred) posting to host variables.
+#include <algorithm>
 #include <cstring>
 #include <memory>
 #include <utility>
@@ -172,8 +173,11 @@ void addPendingPost(const std::shared_ptr<Reduction> &r, char *val, char *loc, b
 }
 
 // ------------------------------------------------------------ GPU reductions
-// DvmhReductionCuda::prepare (rts/dvmh/loop.cpp:274-297): `count` partial slots,
-// optionally pre-filled with the handler-local initial value.
+// DvmhReductionCuda::prepare (rts/dvmh/loop.cpp:274-297): `count` partial slots
+// per reduction element, optionally pre-filled with the handler-local initial
+// value.  A reduction array of L elements keeps its partials element-major:
+// element e owns slots [e * count, (e + 1) * count) (the layout the reference's
+// array fold walks, rts/dvmh/cuda_reduction.cu:182-199).
 void gpuRedPrepare(SpecLoop *sl, Reduction *r, long count, bool fill) {
     DVMH_CHECK(count > 0, "Reduction partial count must be positive");
     DVMH_CHECK(r->gpuArr == nullptr, "dvmh_loop_cuda_red_prepare_C called twice for one reduction");
@@ -185,8 +189,13 @@ void gpuRedPrepare(SpecLoop *sl, Reduction *r, long count, bool fill) {
         if (r->gpuLocPtr) *r->gpuLocPtr = r->gpuLoc;
     }
     if (fill) {
-        dvmh_hip_fill(r->gpuArr, r->localArr.data(), r->elemCount * r->elemSize, count, sl->stream);
-        if (r->isLoc()) dvmh_hip_fill(r->gpuLoc, r->localLoc.data(), r->elemCount * r->locSize, count, sl->stream);
+        for (long e = 0; e < r->elemCount; e++) {
+            dvmh_hip_fill((char *)r->gpuArr + e * count * r->elemSize, r->localArr.data() + e * r->elemSize, r->elemSize, count,
+                    sl->stream);
+            if (r->isLoc())
+                dvmh_hip_fill((char *)r->gpuLoc + e * count * r->locSize, r->localLoc.data() + e * r->locSize, r->locSize, count,
+                        sl->stream);
+        }
     }
 }
 
@@ -197,7 +206,7 @@ void gpuRedPrepare(SpecLoop *sl, Reduction *r, long count, bool fill) {
 // into mapped pinned host memory (no separate copy).
 void gpuRedFinish(SpecLoop *sl, const std::shared_ptr<Reduction> &r) {
     DVMH_CHECK(r->gpuArr != nullptr, "dvmh_loop_cuda_red_finish_C without dvmh_loop_cuda_red_prepare_C");
-    DVMH_CHECK(r->elemCount == 1, "Reduction arrays are not supported on the GPU");
+    DVMH_CHECK(r->elemCount >= 1, "Reduction of %ld elements", r->elemCount);
     DVMH_CHECK(r->rtType == rt_CHAR || r->rtType == rt_UCHAR || r->rtType == rt_INT || r->rtType == rt_UINT ||
                     r->rtType == rt_LOGICAL || r->rtType == rt_LONG || r->rtType == rt_ULONG || r->rtType == rt_LLONG ||
                     r->rtType == rt_ULLONG || r->rtType == rt_FLOAT || r->rtType == rt_DOUBLE,
@@ -215,51 +224,77 @@ void gpuRedFinish(SpecLoop *sl, const std::shared_ptr<Reduction> &r) {
 
 void gpuRedFinishFlush(SpecLoop *sl) {
     auto &q = sl->finishQueue;
-    while (!q.empty()) {
-        // the first pending reduction's element type, up to DVMH_MULTI_MAX of that type
-        const int type = q[0]->rtType;
-        std::vector<std::shared_ptr<Reduction>> group, rest;
-        for (auto &r : q) {
-            if (r->rtType == type && (int)group.size() < DVMH_MULTI_MAX) group.push_back(r);
-            else rest.push_back(r);
+    if (q.empty()) return;
+    // One folded record per reduction, posted in the order the handler asked
+    // for the finishes (a captured graph's replays re-post them in that order):
+    // its L values, then (16-byte aligned) its L loc records.
+    struct Entry {
+        Reduction *r;
+        long e;
+        char *outVal, *outLoc;
+    };
+    std::vector<Entry> entries;
+    for (auto &r : q) {
+        const long L = r->elemCount;
+        const size_t locOff = ((size_t)(L * r->elemSize) + 15) & ~(size_t)15;
+        const size_t recBytes = r->isLoc() ? locOff + (size_t)(L * r->locSize) : (size_t)(L * r->elemSize);
+        // a captured finish writes into a slot owned by the graph
+        char *host = graphCapturing() ? graphPinnedSlot(recBytes) : pinnedPool.take(recBytes);
+        char *hostLoc = r->isLoc() ? host + locOff : nullptr;
+        if (graphCapturing()) graphNotePost(r, host, hostLoc);
+        char *dev = mappedDevicePtr(host);
+        for (long e = 0; e < L; e++)
+            entries.push_back(Entry{r.get(), e, dev + e * r->elemSize, r->isLoc() ? dev + locOff + e * r->locSize : nullptr});
+        addPendingPost(r, host, hostLoc);
+    }
+    // every element of every reduction is folded by one multi-finish entry;
+    // a launch takes up to DVMH_MULTI_MAX entries of one element type
+    std::vector<int> types;
+    for (auto &r : q)
+        if (std::find(types.begin(), types.end(), r->rtType) == types.end()) types.push_back(r->rtType);
+    std::vector<std::vector<const Entry *>> launches;
+    for (int type : types) {
+        std::vector<const Entry *> cur;
+        for (const Entry &en : entries) {
+            if (en.r->rtType != type) continue;
+            cur.push_back(&en);
+            if ((int)cur.size() == DVMH_MULTI_MAX) { launches.push_back(cur); cur.clear(); }
         }
+        if (!cur.empty()) launches.push_back(cur);
+    }
+    for (size_t li = 0; li < launches.size(); li++) {
+        const auto &group = launches[li];
         DvmhMultiRed d[DVMH_MULTI_MAX];
         std::vector<void *> slots;
         for (size_t i = 0; i < group.size(); i++) {
-            auto &r = group[i];
-            size_t recBytes = (size_t)(r->elemSize + (r->isLoc() ? r->locSize : 0));
-            // a captured finish writes into a slot owned by the graph
-            char *host = graphCapturing() ? graphPinnedSlot(recBytes) : pinnedPool.take(recBytes);
-            char *hostLoc = r->isLoc() ? host + r->elemSize : nullptr;
-            if (graphCapturing()) graphNotePost(r, host, hostLoc);
-            char *dev = mappedDevicePtr(host);
+            const Entry &en = *group[i];
+            Reduction *r = en.r;
             void *sv = gpu.alloc((size_t)dvmh_hip_multi_slot_bytes(r->items));
             slots.push_back(sv);
             d[i].func = r->func;
             d[i].n = r->items;
-            d[i].vals = r->gpuArr;
-            d[i].locs = r->gpuLoc;
+            d[i].vals = (const char *)r->gpuArr + en.e * r->items * r->elemSize;
+            d[i].locs = r->isLoc() ? (const char *)r->gpuLoc + en.e * r->items * r->locSize : nullptr;
             d[i].locSize = r->isLoc() ? r->locSize : 0;
-            d[i].outVal = dev;
-            d[i].outLoc = r->isLoc() ? dev + r->elemSize : nullptr;
+            d[i].outVal = en.outVal;
+            d[i].outLoc = en.outLoc;
             d[i].slots = sv;
-            addPendingPost(r, host, hostLoc);
         }
         // the loop's last finish launch publishes a completion word (not in a
         // captured graph, whose replays would all publish the same value)
-        const bool signal = rest.empty() && !graphCapturing();
+        const bool signal = li + 1 == launches.size() && !graphCapturing();
         const unsigned long long seq = signal ? ++gpu.doneSeq : 0;
-        dvmh_hip_reduce_finish_multi(type, (int)group.size(), d, gpu.finishCounters, signal ? gpu.doneDev : nullptr, seq,
-                sl->stream);
+        dvmh_hip_reduce_finish_multi(group[0]->r->rtType, (int)group.size(), d, gpu.finishCounters, signal ? gpu.doneDev : nullptr,
+                seq, sl->stream);
         if (signal) pendingDone = seq;
         for (void *p : slots) gpu.release(p);
-        for (auto &r : group) {
-            gpu.release(r->gpuArr);
-            r->gpuArr = nullptr;
-            if (r->gpuLoc) { gpu.release(r->gpuLoc); r->gpuLoc = nullptr; }
-        }
-        q.swap(rest);
     }
+    for (auto &r : q) {
+        gpu.release(r->gpuArr);
+        r->gpuArr = nullptr;
+        if (r->gpuLoc) { gpu.release(r->gpuLoc); r->gpuLoc = nullptr; }
+    }
+    q.clear();
 }
 
 // ------------------------------------------------------------ layout policy
@@ -316,6 +351,7 @@ void performLoop(Loop *loop) {
         }
         if (batchQueue(loop)) return;
     }
+    batchDrain();   // loops queued before this one run first
     performLoopNow(loop);
 }
 

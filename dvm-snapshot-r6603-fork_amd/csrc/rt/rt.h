@@ -31,7 +31,7 @@ namespace dvmh {
 
 constexpr int MAX_RANK = 8;         // rts/dvmh/dvmh_types.h:223
 constexpr int MAX_LOOP_RANK = 12;   // rts/dvmh/dvmh_types.h:224
-constexpr int MAX_PARAMS = 64;      // handler parameters supported by the trampoline
+constexpr int MAX_PARAMS = 256;     // handler parameters (rts/dvmh/dvmh_types.h:228 MAX_PARAM_COUNT)
 
 // ---------------------------------------------------------------- errors / log
 [[noreturn]] void fatal(const char *kind, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
@@ -54,6 +54,7 @@ struct Settings {
     int tuneRepeats = 1;        // DVMH_AMD_TUNE_REPEATS: timed executions per candidate
     bool pollDone = true;          // DVMH_AMD_POLL: wait for reduction results on the completion word, not a stream sync
     bool tuneTransforms = false;   // DVMH_AMD_TUNE_TRANSFORMS: candidates may re-lay out arrays of ACROSS loops
+    bool index64 = false;          // DVMH_AMD_INDEX64: every loop's kernels take 64-bit index arithmetic
     std::string tuneDb;         // DVMH_AMD_TUNEDB: file of recorded timings
     std::string modelFile;      // DVMH_AMD_MODEL: trained cost-model weights
     void load();
@@ -258,6 +259,7 @@ struct Loop : Object {
     bool queued = false;            // owned by a lazy region or a batch until it runs
     HandlerDesc batchH;             // handler that runs many loops of this site at once
     bool hasBatch = false;
+    bool batchChain = false;        // batchH runs consecutive loops of the site in order (shared arrays allowed)
     Loop(): Object(Kind::Loop) {}
     ~Loop() override;
     void computeDependencyMask();                       // rts/dvmh/loop.cpp:1117-1150
@@ -312,6 +314,8 @@ void tunerShutdown();
 // batching and graphs (batch.cpp)
 extern long graphReplays, batchGroups, batchLoops;
 bool batchQueue(Loop *loop);
+void batchDrain();                  // run the loops queued in an open batch
+bool batchOpen();
 bool loopIsQueued(Loop *loop);
 int hostSlotCount();
 bool batchKeepsRegion(Region *r);

@@ -486,7 +486,7 @@ long tunerSitePending(const std::string &file, long line) {
 
 extern "C" DvmType dvmh_amd_tuner_site_pending_C(const char *file, DvmType line) { runtimeInit(); return tunerSitePending(file, line); }
 
-extern "C" DvmType dvmh_amd_tuner_query_C(const char *file, DvmType line, DvmType out[10]) {
+extern "C" DvmType dvmh_amd_tuner_query_C(const char *file, DvmType line, DvmType out[13]) {
     runtimeInit();
     std::string prefix = std::string(file) + ":" + std::to_string(line) + "|";
     for (auto &kv : sites) {
@@ -497,7 +497,7 @@ extern "C" DvmType dvmh_amd_tuner_query_C(const char *file, DvmType line, DvmTyp
         out[0] = b;
         out[1] = (DvmType)(kv.second.measured(b) * 1e9);
         out[2] = c.block[0]; out[3] = c.block[1]; out[4] = c.block[2];
-        for (int i = 0; i < 5; i++) out[5 + i] = c.tune[i];
+        for (int i = 0; i < DVMH_AMD_TUNE_N; i++) out[5 + i] = c.tune[i];
         return 1;
     }
     return 0;

@@ -30,7 +30,15 @@ class Workload:
         self.P.fn('teardown', [])()
 
     def points_per_iter(self):
+        """Point-updates per iteration: interior points written by the loops
+        that write the grid (a point written by two loops of one iteration,
+        as in 2-D Jacobi's stencil and copy loops, counts once)."""
         return (self.size - 2) ** 3
+
+    def bytes_per_iter(self):
+        """Algorithmic bytes per iteration: every loop reads each array it uses
+        once and writes each array it writes once, 8 B per FP64 element."""
+        return 16 * self.points_per_iter()
 
 
 class Jac2d(Workload):
@@ -44,6 +52,10 @@ class Jac2d(Workload):
 
     def points_per_iter(self):
         return (self.size - 2) ** 2
+
+    def bytes_per_iter(self):
+        # stencil loop: read A, write B (16 B); copy loop with MAX: read A and B, write A (24 B)
+        return 40 * self.points_per_iter()
 
 
 class Jac3d(Workload):
@@ -76,7 +88,13 @@ class Adi3d(Workload):
         return self.P.fn('iterate', [ctypes.c_int, _dp, _lp], ctypes.c_double)(iters, None, None)
 
     def points_per_iter(self):
-        return self.size ** 3
+        # six line sweeps, each updating (n - 1) points of every line of n^2 lines
+        return 6 * (self.size - 1) * self.size ** 2
+
+    def bytes_per_iter(self):
+        # each sweep reads and writes its points (16 B per point-update); the
+        # MAXLOC loop reads the whole array once (8 B per element)
+        return 16 * self.points_per_iter() + 8 * self.size ** 3
 
 
 ALL = {'jac2d': Jac2d, 'jac3d': Jac3d, 'sor3d': Sor3d, 'adi3d': Adi3d}

@@ -638,6 +638,25 @@ class Converter:
         out.append(self.src[last:toks[j - 1].end])
         return ''.join(out)
 
+    def const_extent(self, dims):
+        """The element count of a one-dimensional C array whose extent is an
+        integer literal or a macro defined as one; None otherwise."""
+        if len(dims) != 1 or not dims[0]:
+            return None
+        text = dims[0].strip()
+        for _ in range(8):
+            if re.fullmatch(r'\d+', text):
+                return int(text)
+            m = None
+            for dfn in self.defines:
+                m = re.match(r'#\s*define\s+' + re.escape(text) + r'\s+\(?\s*(\w+)\s*\)?\s*$', dfn)
+                if m:
+                    break
+            if not m:
+                return None
+            text = m.group(1)
+        return None
+
     # -- directives
     def array_spec(self, d, text):
         """The distributed array declared by `text` under array directive d:
@@ -845,6 +864,7 @@ class Converter:
         line0 = self.src.rfind('\n', 0, self.toks[first].start) + 1
         body = self.dedent(self.src[line0:self.toks[last - 1].end])
         reds, privates, across, shadows, cuda_block = [], [], None, [], None
+        red_lengths = {}
         for name, args in clauses:
             if name == 'reduction':
                 for r in split_top(args):
@@ -853,8 +873,15 @@ class Converter:
                         self.err(d.line, f'reduction {r!r} is not supported')
                     vs = [x.strip() for x in split_top(rm.group(2))]
                     for x in vs:
-                        if x not in self.scope or self.scope[x].dims or self.scope[x].pointer_rank or self.scope[x].dvm:
-                            self.err(d.line, f'reduction variable {x!r} must be a scalar')
+                        v = self.scope.get(x)
+                        if v is None or v.pointer_rank or v.dvm:
+                            self.err(d.line, f'reduction variable {x!r} must be a scalar or a C array')
+                        if v.dims:
+                            # a reduction array: each element reduced on its own
+                            n = self.const_extent(v.dims)
+                            if n is None or rm.group(1) in ('maxloc', 'minloc'):
+                                self.err(d.line, f'reduction array {x!r} needs one constant extent and a value reduction')
+                            red_lengths[x] = n
                     if rm.group(1) in ('maxloc', 'minloc'):
                         if len(vs) != 2:
                             self.err(d.line, f'{rm.group(1)} takes a value and a loc variable')
@@ -925,6 +952,8 @@ class Converter:
                 'steps': [h[2] for h in heads], 'params': params, 'body': '\n' + body + '\n'}
         if reds:
             desc['reductions'] = reds
+        if red_lengths:
+            desc['reduction_lengths'] = red_lengths
         if across:
             desc['across'] = across
         if privates:
@@ -961,6 +990,8 @@ class Converter:
                 self.err(d.line, f'reduction type {r[2]!r} is not supported')
             if len(r) > 3:
                 out.append(f'dvmh_loop_reduction_C({lp}, {RED_FUNC[r[0]]}, &{r[1]}, {RT_TYPE[r[2]]}, 1, &{r[3]}, sizeof({r[3]}));')
+            elif r[1] in red_lengths:
+                out.append(f'dvmh_loop_reduction_C({lp}, {RED_FUNC[r[0]]}, {r[1]}, {RT_TYPE[r[2]]}, {red_lengths[r[1]]}, 0, 0);')
             else:
                 out.append(f'dvmh_loop_reduction_C({lp}, {RED_FUNC[r[0]]}, &{r[1]}, {RT_TYPE[r[2]]}, 1, 0, 0);')
         if cuda_block:

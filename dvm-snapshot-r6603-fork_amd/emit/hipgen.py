@@ -72,6 +72,12 @@ class Loop:
         self.host_only = d.get('host_only', False)  # a loop outside regions: no GPU handler
         self.privates = d.get('privates', [])      # [(decl text)]
         self.reds = d.get('reductions', [])        # [(func, var, ctype)] or [(func, var, ctype, loc, loctype)]
+        # reduction arrays: variable -> element count (a C array of that many
+        # elements, each reduced on its own; value reductions only)
+        self.red_lengths = dict(d.get('reduction_lengths', {}))
+        for red in self.reds:
+            if self.rlen(red) > 1 and len(red) > 3:
+                raise ValueError(f'{self.name}: {red[0]} over the array {red[1]} is not supported')
         self.across = d.get('across')              # None | {'array': name, 'widths': [(lo, hi), ...]}
         self.body = d['body'].strip('\n')
         self.cuda_block = d.get('cuda_block')      # optional default block
@@ -81,6 +87,12 @@ class Loop:
         # arrays read as a radius-1 stencil around the loop point: candidates for
         # the LDS march variant (checked against the body below)
         self.stencil = [a for a in d.get('stencil', []) if self._is_stencil(a)]
+        if self.batch and any(self.rlen(red) > 1 for red in self.reds):
+            raise ValueError(f'{self.name}: a batch handler takes scalar reductions only')
+
+    def rlen(self, red):
+        """Elements of a reduction variable (1 for a scalar)."""
+        return int(self.red_lengths.get(red[1], 1))
 
     def _is_stencil(self, name):
         """True when the body only reads `name` and every access is
@@ -149,6 +161,18 @@ class Loop:
                 return False
         return True
 
+    def chain_eligible(self):
+        """Consecutive executions (sweeps) of the loop may run pipelined by a
+        chain handler: a tiled-wavefront ACROSS loop marked for batching that
+        writes no array but its ACROSS array and has scalar reductions only."""
+        if not self.batch or not self.tile_eligible():
+            return False
+        X = self.across['array']
+        for p in self.arrays:
+            if p[1] != X and re.search(r'\b' + p[1] + r'\s*(?:\[[^\]]*\]\s*)+(?:=(?!=)|\+=|-=|\*=|/=|\+\+|--)', self.body):
+                return False
+        return all(self.rlen(red) == 1 for red in self.reds)
+
     def line_tile_eligible(self):
         """A line along the innermost axis can be staged in LDS and walked by
         one thread (the tiled line sweep)."""
@@ -198,10 +222,10 @@ def host_handler(L):
         formal.append(f'DvmType {p[1]}_hdr[]' if p[0] == 'array' else ('void *' if p[0] == 'carray' else f'{p[2]} *') + f'{p[1]}_ptr')
     out.append(f'void {L.name}_host({", ".join(formal)}) {{')
     b = []
-    b.append('/* Loop reference and device number */')
+    b.append('/* the handle this handler was called with, and the device it runs on */')
     b.append('DvmType loop_ref = *pLoopRef;')
     b.append('DvmType device_num = dvmh_loop_get_device_num_C(loop_ref);')
-    b.append('/* Parameters */')
+    b.append('/* array views and scalar addresses on that device */')
     for p in L.params:
         if p[0] == 'array':
             _, nm, ct, ar = p
@@ -216,18 +240,19 @@ def host_handler(L):
             b.append(f'{ct} (*{nm}){dims} = ({ct} (*){dims}){nm}_ptr;')
         else:
             b.append(f'{p[2]} {p[1]} = *{p[1]}_ptr;')
-    b.append('/* Supplementary variables for loop handling */')
+    b.append('/* portion bounds, host slot count, dependency bits */')
     b.append(f'DvmType boundsLow[{r}], boundsHigh[{r}], loopSteps[{r}];')
     b.append('int slotCount;')
     if L.across:
         b.append('DvmType dependencyMask;')
-    b.append('/* User variables - loop index variables and other private variables */')
+    b.append("/* the loop body's own variables: its indices, privates and reduction partials */")
     for v, ct in L.vars:
         b.append(f'{ct} {v};')
     for pr in L.privates:
         b.append(pr + ';')
     for red in L.reds:
-        b.append(f'{red[2]} {red[1]};')
+        n = L.rlen(red)
+        b.append(f'{red[2]} {red[1]}{f"[{n}]" if n > 1 else ""};')
         if len(red) > 3:
             b.append(f'{red[4]} {red[3]};')
     b.append('')
@@ -235,7 +260,7 @@ def host_handler(L):
     b.append('slotCount = dvmh_loop_get_slot_count_C(loop_ref);')
     if L.across:
         b.append('dependencyMask = dvmh_loop_get_dependency_mask_C(loop_ref);')
-        b.append('int threadSync[slotCount > 0 ? slotCount : 1];')
+        b.append('int slotReady[slotCount > 0 ? slotCount : 1];')
     privs = [v for v, _ in L.vars] + [re.split(r'[\s\*]+', pr.strip())[-1].split('[')[0] for pr in L.privates]
     redvars = []
     for red in L.reds:
@@ -251,10 +276,11 @@ def host_handler(L):
     b.append('{')
     inner = []
     if L.across:
-        inner.append('int currentThread = 0, workingThreads = slotCount;')
+        inner.append('int mySlot = 0, teamSize = slotCount;')
     for i, red in enumerate(L.reds):
         loc = ('&' + red[3]) if len(red) > 3 else '0'
-        inner.append(f'dvmh_loop_red_init_C(loop_ref, {i + 1}, &{red[1]}, {loc});')
+        addr = red[1] if L.rlen(red) > 1 else '&' + red[1]
+        inner.append(f'dvmh_loop_red_init_C(loop_ref, {i + 1}, {addr}, {loc});')
 
     def loop_head(i):
         v = L.vars[i][0]
@@ -291,26 +317,26 @@ def host_handler(L):
         else:
             # every axis dependent: pipelined handshake over the first two axes
             lines.append('{')
-            lines.append(IND + 'DvmType paralIters = (boundsHigh[1] - boundsLow[1]) / loopSteps[1] + 1;')
-            lines.append(IND + 'if (paralIters < workingThreads) workingThreads = paralIters;')
+            lines.append(IND + 'DvmType outerIters = (boundsHigh[1] - boundsLow[1]) / loopSteps[1] + 1;')
+            lines.append(IND + 'if (outerIters < teamSize) teamSize = outerIters;')
             lines.append('#ifdef _OPENMP')
-            lines.append(IND + 'workingThreads = workingThreads < omp_get_num_threads() ? workingThreads : omp_get_num_threads();')
-            lines.append(IND + 'currentThread = omp_get_thread_num();')
+            lines.append(IND + 'teamSize = teamSize < omp_get_num_threads() ? teamSize : omp_get_num_threads();')
+            lines.append(IND + 'mySlot = omp_get_thread_num();')
             lines.append('#endif')
-            lines.append(IND + '__atomic_store_n(&threadSync[currentThread], 0, __ATOMIC_SEQ_CST);')
+            lines.append(IND + '__atomic_store_n(&slotReady[mySlot], 0, __ATOMIC_SEQ_CST);')
             lines.append(IND + '#pragma omp barrier')
             lines.append(IND + loop_head(0) + ' {')
-            lines.append(IND * 2 + 'if (currentThread > 0 && currentThread < workingThreads) {')
-            lines.append(IND * 3 + 'while (!__atomic_load_n(&threadSync[currentThread - 1], __ATOMIC_ACQUIRE)) {}')
-            lines.append(IND * 3 + '__atomic_store_n(&threadSync[currentThread - 1], 0, __ATOMIC_RELEASE);')
+            lines.append(IND * 2 + 'if (mySlot > 0 && mySlot < teamSize) {')
+            lines.append(IND * 3 + 'while (!__atomic_load_n(&slotReady[mySlot - 1], __ATOMIC_ACQUIRE)) {}')
+            lines.append(IND * 3 + '__atomic_store_n(&slotReady[mySlot - 1], 0, __ATOMIC_RELEASE);')
             lines.append(IND * 2 + '}')
             lines.append(IND * 2 + '#pragma omp for schedule(static), nowait')
             for i in range(1, L.rank):
                 lines.append(IND * (i + 1) + loop_head(i))
             lines.append(indent(body, L.rank + 1))
-            lines.append(IND * 2 + 'if (currentThread < workingThreads - 1) {')
-            lines.append(IND * 3 + 'while (__atomic_load_n(&threadSync[currentThread], __ATOMIC_ACQUIRE)) {}')
-            lines.append(IND * 3 + '__atomic_store_n(&threadSync[currentThread], 1, __ATOMIC_RELEASE);')
+            lines.append(IND * 2 + 'if (mySlot < teamSize - 1) {')
+            lines.append(IND * 3 + 'while (__atomic_load_n(&slotReady[mySlot], __ATOMIC_ACQUIRE)) {}')
+            lines.append(IND * 3 + '__atomic_store_n(&slotReady[mySlot], 1, __ATOMIC_RELEASE);')
             lines.append(IND * 2 + '}')
             lines.append(IND + '}')
             lines.append('}')
@@ -320,7 +346,8 @@ def host_handler(L):
         inner.append('}')
     for i, red in enumerate(L.reds):
         loc = ('&' + red[3]) if len(red) > 3 else '0'
-        inner.append(f'dvmh_loop_red_post_C(loop_ref, {i + 1}, &{red[1]}, {loc});')
+        addr = red[1] if L.rlen(red) > 1 else '&' + red[1]
+        inner.append(f'dvmh_loop_red_post_C(loop_ref, {i + 1}, {addr}, {loc});')
     b.append(indent('\n'.join(inner) + '\n', 1).rstrip('\n'))
     b.append('}')
     out.append(indent('\n'.join(b) + '\n', 1).rstrip('\n'))
@@ -363,6 +390,12 @@ def kernel_diag_array_params(L):
 def red_kernel_params(L):
     fp = []
     for red in L.reds:
+        n = L.rlen(red)
+        if n > 1:
+            # a reduction array: its initial values by value; element q's
+            # partial of workgroup slot s at grid[q * items + s]
+            fp += [f'dvmh_dev::RedVec<{red[2]}, {n}> {red[1]}_init', f'{red[2]} *{red[1]}_grid', f'long {red[1]}_items']
+            continue
         fp += [f'{red[2]} {red[1]}_init', f'{red[2]} *{red[1]}_grid']
         if len(red) > 3:
             fp += [f'{red[4]} {red[3]}_init', f'{red[4]} *{red[3]}_grid']
@@ -372,6 +405,11 @@ def red_kernel_params(L):
 def red_kernel_locals(L):
     s = []
     for red in L.reds:
+        n = L.rlen(red)
+        if n > 1:
+            s.append(f'{red[2]} {red[1]}[{n}];')
+            s.append(f'for (int dvmh_q = 0; dvmh_q < {n}; dvmh_q++) {red[1]}[dvmh_q] = {red[1]}_init.v[dvmh_q];')
+            continue
         s.append(f'{red[2]} {red[1]} = {red[1]}_init;')
         if len(red) > 3:
             s.append(f'{red[4]} {red[3]} = {red[3]}_init;')
@@ -384,6 +422,17 @@ def red_kernel_tail(L, slot, accumulate):
     tid0 = '(threadIdx.x | threadIdx.y | threadIdx.z) == 0'
     for red in L.reds:
         f, v, ct = red[0], red[1], red[2]
+        n = L.rlen(red)
+        if n > 1:
+            s.append(f'for (int dvmh_q = 0; dvmh_q < {n}; dvmh_q++) {{')
+            s.append(f'    {ct} dvmh_r = dvmh_dev::blockReduce({v}[dvmh_q], dvmh_dev::{RED_OP[f]}());')
+            cell = f'{v}_grid[(long)dvmh_q * {v}_items + ({slot})]'
+            if accumulate:
+                s.append(f'    if ({tid0}) {cell} = dvmh_dev::{RED_OP[f]}()({cell}, dvmh_r);')
+            else:
+                s.append(f'    if ({tid0}) {cell} = dvmh_r;')
+            s.append('}')
+            continue
         if len(red) > 3:
             loc = red[3]
             mx = 'true' if f == 'maxloc' else 'false'
@@ -417,12 +466,12 @@ def regs_query(kernel, loop_ref='loop_ref'):
 def handler_prologue(L, kind_const, want_ext=False, autotfm=False, regs_kernel=None):
     h = []
     h.append('DvmType tmpVar;')
-    h.append('/* Loop reference and device number */')
+    h.append('/* the handle this handler was called with, and the device it runs on */')
     h.append('DvmType loop_ref = *pLoopRef;')
     h.append('DvmType device_num = dvmh_loop_get_device_num_C(loop_ref);')
     h.append('DvmType tune[DVMH_AMD_TUNE_N];')
     h.append(f'dvmh_amd_loop_get_tuning_C(loop_ref, {kind_const}, tune);')
-    h.append('/* Parameters */')
+    h.append('/* array views and scalar addresses on that device */')
     for p in L.arrays:
         _, nm, ct, ar = p
         if autotfm:
@@ -437,7 +486,7 @@ def handler_prologue(L, kind_const, want_ext=False, autotfm=False, regs_kernel=N
     h.append(f'DvmType boundsLow[{L.rank}], boundsHigh[{L.rank}], loopSteps[{L.rank}];')
     h.append('dim3 threads(0, 0, 0);')
     h.append('hipStream_t stream;')
-    h.append('/* Choose index type for the kernel */')
+    h.append('/* 32- or 64-bit index arithmetic, from the largest extent involved */')
     h.append('int kernelIndexT = (int)dvmh_loop_guess_index_type_C(loop_ref);')
     h.append('if (kernelIndexT == rt_LONG) kernelIndexT = rt_LLONG;')
     h += regs_query(regs_kernel)
@@ -448,6 +497,15 @@ def handler_prologue(L, kind_const, want_ext=False, autotfm=False, regs_kernel=N
 def handler_reds_prepare(L, count_expr, fill):
     h = []
     for i, red in enumerate(L.reds):
+        n = L.rlen(red)
+        if n > 1:
+            h.append(f'{red[2]} {red[1]}[{n}];')
+            h.append(f'{red[2]} *{red[1]}_grid;')
+            h.append(f'long {red[1]}_items = (long)({count_expr});')
+            h.append(f'dvmh_loop_cuda_register_red_C(loop_ref, {i + 1}, (void **)&{red[1]}_grid, 0);')
+            h.append(f'dvmh_loop_red_init_C(loop_ref, {i + 1}, {red[1]}, 0);')
+            h.append(f'dvmh_loop_cuda_red_prepare_C(loop_ref, {i + 1}, {count_expr}, {fill});')
+            continue
         h.append(f'{red[2]} {red[1]};')
         h.append(f'{red[2]} *{red[1]}_grid;')
         if len(red) > 3:
@@ -469,6 +527,10 @@ def handler_reds_finish(L):
 def red_fact_args(L):
     a = []
     for red in L.reds:
+        n = L.rlen(red)
+        if n > 1:
+            a += [f'dvmh_dev::redVec<{red[2]}, {n}>({red[1]})', f'{red[1]}_grid', f'{red[1]}_items']
+            continue
         a += [red[1], f'{red[1]}_grid']
         if len(red) > 3:
             a += [red[3], f'{red[3]}_grid']
@@ -625,7 +687,8 @@ def march_kernel(L):
     # for its workgroup's tile, the batch kernel for every tile it walks
     tp = list(fp) + ['long tile']
     for red in L.reds:
-        tp.append(f'{red[2]} &{red[1]}')
+        n = L.rlen(red)
+        tp.append(f'{red[2]} (&{red[1]})[{n}]' if n > 1 else f'{red[2]} &{red[1]}')
         if len(red) > 3:
             tp.append(f'{red[4]} &{red[3]}')
     k = ['template <typename IndexT, int COLS, int NPF>',
@@ -1195,6 +1258,7 @@ def batch_handler(L):
     lb.append(f'd[{lay["nb"]}] = nbx; d[{lay["nb"] + 1}] = nby; d[{lay["nb"] + 2}] = nbz;')
     for i in range(r):
         lb.append(f'if ((labs(blo[{i}]) + cnt{i} * labs(bst[{i}]) + 64) * maxStride > 2147483647L) wide = 1;')
+    lb.append('if (dvmh_loop_guess_index_type_C(lr) != rt_INT) wide = 1;')
     for ri, red in enumerate(L.reds):
         lb.append(f'{{ {red[2]} v; ')
         if len(red) > 3:
@@ -1569,7 +1633,13 @@ def tile_kernel(L):
     workgroup barrier between them, and writes the tile back.  The body reads
     and writes the ACROSS array through a TileView3 with the usual X[i][j][k]
     syntax; every dependency is met in the sequential order's sense, so the
-    results are those of the lexicographic sweep."""
+    results are those of the lexicographic sweep.
+
+    The work on one tile is a device function, called by the per-loop kernel
+    (one launch per tile-plane) and by the chain kernel, which runs several
+    consecutive sweeps of the loop in one launch sequence: launch t runs
+    tile-plane t - LAG * s of sweep s for every sweep s that has one, so a
+    sweep stays LAG tile-planes behind the sweep before it."""
     X = L.across['array']
     xct = [p for p in L.arrays if p[1] == X][0][2]
     rng = L.stencil_range(X)   # per axis, the offsets at which the body reads X
@@ -1587,22 +1657,28 @@ def tile_kernel(L):
         setup.append(f'{p[2]} &{p[1]} = *{p[1]}_ptr;')
     for i in range(3):
         fp += [f'IndexT lo{i}', f'IndexT cnt{i}', f'IndexT st{i}']
-    fp += ['IndexT dt_TA', 'IndexT dt_TC', 'IndexT dt_ntB', 'IndexT dt_ntC', 'IndexT dt_aLo', 'IndexT dt_tplane']
-    fp += red_kernel_params(L)
+    call = []
+    for p in L.arrays:
+        call += [f'{p[1]}_base'] + [f'{p[1]}_s{q}' for q in range(p[3])]
+    for p in L.scalars:
+        call.append(f'{p[1]}_ptr')
+    for i in range(3):
+        call += [f'lo{i}', f'cnt{i}', f'st{i}']
+    # ---- the tile: stage, local hyperplanes, write back; reductions by reference
+    tp = list(fp) + ['IndexT dt_TA', 'IndexT dt_TC', 'IndexT dt_ta', 'IndexT dt_tb', 'IndexT dt_tc']
+    for red in L.reds:
+        tp.append(f'{red[2]} &{red[1]}')
+        if len(red) > 3:
+            tp.append(f'{red[4]} &{red[3]}')
     k = ['template <typename IndexT>',
-         f'__global__ __launch_bounds__(1024) void {L.name}_tile_kernel({", ".join(fp)}) {{']
+         f'__device__ static inline void {L.name}_tile_work({", ".join(tp)}) {{']
     b = ['extern __shared__ __align__(16) unsigned char dvmh_lds[];']
     b += setup
     for v, ct in L.vars:
         b.append(f'{ct} {v};')
     for pr in L.privates:
         b.append(pr + ';')
-    b += red_kernel_locals(L)
-    b += ['const IndexT dt_ta = dt_aLo + (IndexT)blockIdx.x / dt_ntB;',
-          'const IndexT dt_tb = (IndexT)blockIdx.x - ((IndexT)blockIdx.x / dt_ntB) * dt_ntB;',
-          'const IndexT dt_tc = dt_tplane - dt_ta - dt_tb;',
-          'if (dt_tc < 0 || dt_tc >= dt_ntC) return;   // not on this tile-plane (uniform per workgroup)',
-          'const IndexT dt_a0 = dt_ta * dt_TA, dt_b0 = dt_tb * dt_TA, dt_c0 = dt_tc * dt_TC;',
+    b += ['const IndexT dt_a0 = dt_ta * dt_TA, dt_b0 = dt_tb * dt_TA, dt_c0 = dt_tc * dt_TC;',
           'const IndexT dt_na = cnt0 - dt_a0 < dt_TA ? cnt0 - dt_a0 : dt_TA;',
           'const IndexT dt_nb = cnt1 - dt_b0 < dt_TA ? cnt1 - dt_b0 : dt_TA;',
           'const IndexT dt_nc = cnt2 - dt_c0 < dt_TC ? cnt2 - dt_c0 : dt_TC;',
@@ -1681,11 +1757,63 @@ def tile_kernel(L):
           IND + 'const IndexT dt_p = dt_e / dt_nbc, dt_rem = dt_e - (dt_e / dt_nbc) * dt_nbc;',
           IND + 'const IndexT dt_q = dt_rem / dt_nc, dt_r = dt_rem - (dt_rem / dt_nc) * dt_nc;',
           IND + f'{X}_g[dt_oi + 1 + dt_p][dt_oj + 1 + dt_q][dt_ok + 1 + dt_r] = dt_tile[((dt_p + 1) * dt_R1 + (dt_q + 1)) * dt_R2 + (dt_r + 1)];',
-          '}']
-    b += red_kernel_tail(L, '((long)(dt_ta * dt_ntB + dt_tb) * dt_ntC + dt_tc)', accumulate=False)
+          '}',
+          '__syncthreads();   // the LDS tile is free for the next one']
     k.append(indent('\n'.join(b) + '\n', 1).rstrip('\n'))
     k.append('}')
-    return '\n'.join(k) + '\n'
+    redcall = []
+    for red in L.reds:
+        redcall.append(red[1])
+        if len(red) > 3:
+            redcall.append(red[3])
+    # ---- per-loop kernel: one tile-plane per launch
+    kp = list(fp) + ['IndexT dt_TA', 'IndexT dt_TC', 'IndexT dt_ntB', 'IndexT dt_ntC', 'IndexT dt_aLo', 'IndexT dt_tplane']
+    kp += red_kernel_params(L)
+    g = ['template <typename IndexT>',
+         f'__global__ __launch_bounds__(1024) void {L.name}_tile_kernel({", ".join(kp)}) {{']
+    gb = ['const IndexT dt_ta = dt_aLo + (IndexT)blockIdx.x / dt_ntB;',
+          'const IndexT dt_tb = (IndexT)blockIdx.x - ((IndexT)blockIdx.x / dt_ntB) * dt_ntB;',
+          'const IndexT dt_tc = dt_tplane - dt_ta - dt_tb;',
+          'if (dt_tc < 0 || dt_tc >= dt_ntC) return;   // not on this tile-plane (uniform per workgroup)']
+    gb += red_kernel_locals(L)
+    gb.append(f'{L.name}_tile_work<IndexT>({", ".join(call + ["dt_TA", "dt_TC", "dt_ta", "dt_tb", "dt_tc"] + redcall)});')
+    gb += red_kernel_tail(L, '((long)(dt_ta * dt_ntB + dt_tb) * dt_ntC + dt_tc)', accumulate=False)
+    g.append(indent('\n'.join(gb) + '\n', 1).rstrip('\n'))
+    g.append('}')
+    out = '\n'.join(k) + '\n\n' + '\n'.join(g) + '\n'
+    if L.chain_eligible():
+        # ---- chain kernel: sweep s = dt_sLo + blockIdx.y runs tile-plane
+        # dt_launch - dt_lag * s; each sweep folds its reductions into its own
+        # segment of dt_tiles slots, from its own initial value
+        cp = list(fp) + ['IndexT dt_TA', 'IndexT dt_TC', 'IndexT dt_ntA', 'IndexT dt_ntB', 'IndexT dt_ntC',
+                         'IndexT dt_launch', 'IndexT dt_sLo', 'IndexT dt_lag', 'long dt_tiles']
+        for red in L.reds:
+            cp += [f'const {red[2]} *__restrict__ {red[1]}_inits', f'{red[2]} *{red[1]}_grid']
+            if len(red) > 3:
+                cp += [f'const {red[4]} *__restrict__ {red[3]}_inits', f'{red[4]} *{red[3]}_grid']
+        c = ['template <typename IndexT>',
+             f'__global__ __launch_bounds__(1024) void {L.name}_chain_kernel({", ".join(cp)}) {{']
+        cb = ['const IndexT dt_sw = dt_sLo + (IndexT)blockIdx.y;',
+              'const IndexT dt_tplane = dt_launch - dt_lag * dt_sw;',
+              'const IndexT dt_aLo0 = dt_tplane - (dt_ntB - 1) - (dt_ntC - 1);',
+              'const IndexT dt_aLo = dt_aLo0 > 0 ? dt_aLo0 : 0;',
+              'const IndexT dt_aHi = dt_tplane < dt_ntA - 1 ? dt_tplane : dt_ntA - 1;',
+              'const IndexT dt_ta = dt_aLo + (IndexT)blockIdx.x / dt_ntB;',
+              'const IndexT dt_tb = (IndexT)blockIdx.x - ((IndexT)blockIdx.x / dt_ntB) * dt_ntB;',
+              'const IndexT dt_tc = dt_tplane - dt_ta - dt_tb;',
+              '// this sweep has no tile here (before its first or after its last',
+              '// tile-plane, or off the plane): uniform per workgroup',
+              'if (dt_tplane < 0 || dt_ta > dt_aHi || dt_tc < 0 || dt_tc >= dt_ntC) return;']
+        for red in L.reds:
+            cb.append(f'{red[2]} {red[1]} = {red[1]}_inits[dt_sw];')
+            if len(red) > 3:
+                cb.append(f'{red[4]} {red[3]} = {red[3]}_inits[dt_sw];')
+        cb.append(f'{L.name}_tile_work<IndexT>({", ".join(call + ["dt_TA", "dt_TC", "dt_ta", "dt_tb", "dt_tc"] + redcall)});')
+        cb += red_kernel_tail(L, '(long)dt_sw * dt_tiles + ((long)(dt_ta * dt_ntB + dt_tb) * dt_ntC + dt_tc)', accumulate=False)
+        c.append(indent('\n'.join(cb) + '\n', 1).rstrip('\n'))
+        c.append('}')
+        out += '\n' + '\n'.join(c) + '\n'
+    return out
 
 
 # Chunk lengths (iterations per staged chunk) the tiled line sweep is
@@ -1896,6 +2024,131 @@ def tile_launch(L):
     return b
 
 
+# Tile-planes by which a sweep of a chain trails the sweep before it: tile
+# (a, b, c) reads only the tiles one tile-plane below and above it (every
+# access of the ACROSS array moves along one axis by at most one), so sweep s
+# may run tile-plane p once sweep s - 1 has finished tile-plane p + 1; with a
+# lag of two, the tiles two sweeps run in one launch are never neighbours.
+CHAIN_LAG = 2
+
+
+def chain_handler(L):
+    """Chain handler of a tiled-wavefront ACROSS loop: the group of
+    consecutive sweeps dvmh_amd_batch_end_C hands over runs as one launch
+    sequence, sweep s CHAIN_LAG tile-planes behind sweep s - 1, each sweep with
+    its own reduction segment.  Sweeps that differ in bounds, arrays, scalars
+    or layout, or a configuration other than the tiled wavefront, run one
+    after another through the ordinary handler."""
+    X = L.across['array']
+    xct = [p for p in L.arrays if p[1] == X][0][2]
+    h = [f'extern "C" void {L.name}_cuda_chain(DvmType *pBatchRef) {{']
+    b = ['DvmType batchRef = *pBatchRef;',
+         'long nl = (long)dvmh_amd_batch_count_C(batchRef);',
+         'DvmType loop_ref = dvmh_amd_batch_loop_C(batchRef, 0);',
+         'DvmType tune[DVMH_AMD_TUNE_N];',
+         'dvmh_amd_loop_get_tuning_C(loop_ref, DVMH_AMD_KIND_WAVEFRONT, tune);',
+         'dim3 threads(0, 0, 0);',
+         'hipStream_t stream;',
+         *regs_query(f'{L.name}_chain_kernel<int>'),
+         'DvmType dependency_mask = dvmh_loop_get_dependency_mask_C(loop_ref);',
+         'int kernelIndexT = (int)dvmh_loop_guess_index_type_C(loop_ref);',
+         'if (kernelIndexT == rt_LONG) kernelIndexT = rt_LLONG;',
+         'DvmType boundsLow[3], boundsHigh[3], loopSteps[3];',
+         'dvmh_loop_fill_bounds_C(loop_ref, boundsLow, boundsHigh, loopSteps);',
+         'DvmType TA = tune[DVMH_AMD_T_TILE_A], TC = tune[DVMH_AMD_T_TILE_B];',
+         f'size_t ldsBytes = (size_t)(TA + 2) * (TA + 2) * (TC + 2) * sizeof({xct});',
+         'bool chained = tune[DVMH_AMD_T_VARIANT] == 1 && dependency_mask == 7 && TA >= 2 && TC >= 2 && ldsBytes <= 64512 && '
+         'threads.y == 1 && threads.z == 1 && loopSteps[0] == 1 && loopSteps[1] == 1 && loopSteps[2] == 1;']
+    for pi, p in enumerate(L.params):
+        if p[0] == 'array':
+            _, nm, ct, ar = p
+            b.append(f'DvmType *{nm}_hdr = (DvmType *)dvmh_amd_batch_param_C(batchRef, 0, {pi});')
+            b.append(f'{ct} *{nm} = ({ct} *)dvmh_get_natural_base_C(1, {nm}_hdr);')
+            b.append(f'DvmType {nm}_devHdr[64 + 7];')
+            b.append(f'if (dvmh_fill_header_C(1, {nm}, {nm}_hdr, {nm}_devHdr, 0) != 0) chained = false;')
+        else:
+            b.append(f'{p[2]} *{p[1]} = ({p[2]} *)dvmh_get_device_addr_C(1, dvmh_amd_batch_param_C(batchRef, 0, {pi}));')
+    # every sweep of the chain: same arrays, scalars and bounds as the first
+    b.append('for (long li = 1; li < nl && chained; li++) {')
+    b.append(IND + 'DvmType lr = dvmh_amd_batch_loop_C(batchRef, li);')
+    b.append(IND + 'DvmType lo[3], hi[3], st[3];')
+    b.append(IND + 'dvmh_loop_fill_bounds_C(lr, lo, hi, st);')
+    b.append(IND + 'for (int a = 0; a < 3; a++) if (lo[a] != boundsLow[a] || hi[a] != boundsHigh[a] || st[a] != loopSteps[a]) chained = false;')
+    for pi, p in enumerate(L.params):
+        b.append(IND + f'if (dvmh_amd_batch_param_C(batchRef, li, {pi}) != dvmh_amd_batch_param_C(batchRef, 0, {pi})) chained = false;')
+    b.append('}')
+    b.append('if (!chained) {')
+    b.append(IND + '// one sweep after another through the ordinary handler')
+    b.append(IND + 'for (long li = 0; li < nl; li++) {')
+    b.append(IND * 2 + 'DvmType lr = dvmh_amd_batch_loop_C(batchRef, li);')
+    args = ['&lr']
+    for pi, p in enumerate(L.params):
+        cast = 'DvmType *' if p[0] == 'array' else f'{p[2]} *'
+        args.append(f'({cast})dvmh_amd_batch_param_C(batchRef, li, {pi})')
+    b.append(IND * 2 + f'{L.name}_cuda({", ".join(args)});')
+    b.append(IND + '}')
+    b.append(IND + 'return;')
+    b.append('}')
+    b += ['DvmType cnt0 = boundsHigh[0] - boundsLow[0] + 1, cnt1 = boundsHigh[1] - boundsLow[1] + 1, cnt2 = boundsHigh[2] - boundsLow[2] + 1;',
+          'DvmType ntA = (cnt0 + TA - 1) / TA, ntB = (cnt1 + TA - 1) / TA, ntC = (cnt2 + TC - 1) / TC;',
+          'DvmType tiles = ntA * ntB * ntC, planes = ntA + ntB + ntC - 2;',
+          'DvmType *segStart = (DvmType *)malloc((size_t)(nl + 1) * sizeof(DvmType));',
+          'for (long li = 0; li <= nl; li++) segStart[li] = li * tiles;']
+    ra = []
+    for ri, red in enumerate(L.reds):
+        ct = red[2]
+        b.append(f'{ct} *{red[1]}_grid = ({ct} *)dvmh_amd_batch_scratch_C(batchRef, nl * tiles * (long)sizeof({ct}));')
+        b.append(f'{ct} *{red[1]}_inits = ({ct} *)dvmh_amd_batch_scratch_C(batchRef, nl * (long)sizeof({ct}));')
+        b.append(f'{ct} *{red[1]}_ih = ({ct} *)malloc((size_t)nl * sizeof({ct}));')
+        if len(red) > 3:
+            lt = red[4]
+            b.append(f'{lt} *{red[3]}_grid = ({lt} *)dvmh_amd_batch_scratch_C(batchRef, nl * tiles * (long)sizeof({lt}));')
+            b.append(f'{lt} *{red[3]}_inits = ({lt} *)dvmh_amd_batch_scratch_C(batchRef, nl * (long)sizeof({lt}));')
+            b.append(f'{lt} *{red[3]}_ih = ({lt} *)malloc((size_t)nl * sizeof({lt}));')
+            b.append(f'for (long li = 0; li < nl; li++) dvmh_loop_red_init_C(dvmh_amd_batch_loop_C(batchRef, li), {ri + 1}, &{red[1]}_ih[li], &{red[3]}_ih[li]);')
+            b.append(f'dvmh_amd_batch_upload_C(batchRef, {red[3]}_inits, {red[3]}_ih, nl * (long)sizeof({lt}));')
+        else:
+            b.append(f'for (long li = 0; li < nl; li++) dvmh_loop_red_init_C(dvmh_amd_batch_loop_C(batchRef, li), {ri + 1}, &{red[1]}_ih[li], 0);')
+        b.append(f'dvmh_amd_batch_upload_C(batchRef, {red[1]}_inits, {red[1]}_ih, nl * (long)sizeof({ct}));')
+        ra += [f'{red[1]}_inits', f'{red[1]}_grid']
+        if len(red) > 3:
+            ra += [f'{red[3]}_inits', f'{red[3]}_grid']
+    b += [f'const DvmType lag = {CHAIN_LAG};',
+          'dvmh_amd_mark_kernel_C(loop_ref, 0);',
+          'for (DvmType t = 0; t < planes + lag * (nl - 1); t++) {',
+          IND + '// sweeps with a tile-plane in this launch: 0 <= t - lag * s < planes',
+          IND + 'DvmType sLo = t - (planes - 1) <= 0 ? 0 : (t - (planes - 1) + lag - 1) / lag;',
+          IND + 'DvmType sHi = t / lag < nl - 1 ? t / lag : nl - 1;',
+          IND + 'if (sLo > sHi) continue;',
+          IND + 'DvmType rows = 0;',
+          IND + 'for (DvmType sw = sLo; sw <= sHi; sw++) {',
+          IND * 2 + 'DvmType p = t - lag * sw, aLo = p - (ntB - 1) - (ntC - 1), aHi = p < ntA - 1 ? p : ntA - 1;',
+          IND * 2 + 'if (aLo < 0) aLo = 0;',
+          IND * 2 + 'if (aHi - aLo + 1 > rows) rows = aHi - aLo + 1;',
+          IND + '}',
+          IND + 'dim3 grid((unsigned)(rows * ntB), (unsigned)(sHi - sLo + 1), 1);']
+    for it in ['int', 'long long']:
+        args = array_fact_args(L, it)
+        for i in range(3):
+            args += [f'({it})boundsLow[{i}]', f'({it})cnt{i}', f'({it})loopSteps[{i}]']
+        args += [f'({it})TA', f'({it})TC', f'({it})ntA', f'({it})ntB', f'({it})ntC', f'({it})t', f'({it})sLo', f'({it})lag',
+                 '(long)tiles'] + ra
+        cond = f'kernelIndexT == {"rt_INT" if it == "int" else "rt_LLONG"}'
+        b.append(IND + f'if ({cond}) hipLaunchKernelGGL(({L.name}_chain_kernel<{it}>), grid, threads, ldsBytes, stream, {", ".join(args)});')
+    b.append('}')
+    b.append('dvmh_amd_mark_kernel_C(loop_ref, 1);')
+    for ri, red in enumerate(L.reds):
+        loc = f'{red[3]}_grid' if len(red) > 3 else '0'
+        b.append(f'dvmh_amd_batch_red_finish_C(batchRef, {ri + 1}, {red[1]}_grid, {loc}, segStart);')
+        b.append(f'free({red[1]}_ih);')
+        if len(red) > 3:
+            b.append(f'free({red[3]}_ih);')
+    b.append('free(segStart);')
+    h.append(indent('\n'.join(b) + '\n', 1).rstrip('\n'))
+    h.append('}')
+    return '\n'.join(h) + '\n'
+
+
 def across_handler(L):
     r = L.rank
     ndep = len(L.dep_axes())
@@ -2003,6 +2256,8 @@ def gpu_loop(L):
             if L.line_stage_eligible(dep):
                 text.append(line_reg_kernel(L, dep))
         text.append(across_handler(L))
+        if L.chain_eligible():
+            text.append(chain_handler(L))
     else:
         text.append(parallel_kernel(L))
         if L.stencil:

@@ -73,6 +73,13 @@ void dvmh_amd_reset_stats_C(void);
 /* Device layout changes (axis permutations and diagonal re-layouts of arrays
  * done for ACROSS loops, and their undoing) since the runtime started. */
 DvmType dvmh_amd_layout_changes_C(void);
+/* Host threads the host handlers of a loop run on (DVMH_NUM_THREADS, else the
+ * OpenMP default). */
+DvmType dvmh_amd_host_slots_C(void);
+/* on != 0: every loop's handlers get rt_LONG from dvmh_loop_guess_index_type_C
+ * and run their 64-bit index kernels whatever the array sizes (also
+ * DVMH_AMD_INDEX64=1); 0 restores the choice by extent. */
+void dvmh_amd_set_index64_C(DvmType on);
 /* Device name / architecture string of the GPU in use ("" without one). */
 const char *dvmh_amd_device_name_C(void);
 /* Position of (x, y) in an Rx x Ry plane stored diagonal by diagonal (slash 0:
@@ -96,6 +103,11 @@ DvmType dvmh_amd_block_fits_registers_C(const DvmType block[3], DvmType regsPerT
 void dvmh_amd_batch_begin_C(void);
 void dvmh_amd_batch_end_C(void);
 void dvmh_amd_loop_register_batch_handler_C(DvmType curLoop, DvmType handlerHelper);
+/* A chain handler instead: consecutive queued loops of the site join one group
+ * in program order even when they share arrays (ACROSS sweeps of one array),
+ * and the handler keeps their dependencies.  A group whose loop site the tuner
+ * is still exploring runs loop by loop through the ordinary GPU handler. */
+void dvmh_amd_loop_register_chain_handler_C(DvmType curLoop, DvmType handlerHelper);
 /* Inside a batch handler (called as f(&batchRef)): number of loops, the i-th
  * loop reference (usable with every handler query), and the p-th parameter the
  * i-th loop registered for its ordinary GPU handler. */
@@ -138,8 +150,9 @@ DvmType dvmh_amd_tuner_pending_C(void);
 /* The same for the sites of one source loop (file, line). */
 DvmType dvmh_amd_tuner_site_pending_C(const char *file, DvmType line);
 /* Best candidate so far for the loop site (file, line): out[0] candidate index,
- * out[1] time in ns, out[2..9] block x,y,z and tune[0..4]; returns 0 if unknown. */
-DvmType dvmh_amd_tuner_query_C(const char *file, DvmType line, DvmType out[10]);
+ * out[1] time in ns, out[2..4] block x,y,z, out[5..12] tune[0..7]; returns 0
+ * if unknown. */
+DvmType dvmh_amd_tuner_query_C(const char *file, DvmType line, DvmType out[13]);
 /* Forces the launch configuration (block[3], tune[DVMH_AMD_TUNE_N]) of every
  * loop of the given kind, over whatever the mode would choose; forced
  * executions are not timed.  block == NULL removes the forcing for the kind. */

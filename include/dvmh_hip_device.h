@@ -335,6 +335,18 @@ struct OpXor {
     template <typename T> __device__ __forceinline__ T operator()(T a, T b) const { return a ^ b; }
 };
 
+// Initial values of a reduction array (N elements), passed to a kernel by value.
+template <typename T, int N>
+struct RedVec {
+    T v[N];
+};
+template <typename T, int N>
+__host__ __device__ inline RedVec<T, N> redVec(const T (&a)[N]) {
+    RedVec<T, N> r;
+    for (int i = 0; i < N; i++) r.v[i] = a[i];
+    return r;
+}
+
 // Reduce over the workgroup; the result is valid in thread 0.  Fixed order:
 // shuffle tree inside each wave, then wave results in wave order.
 template <typename T, typename Op>

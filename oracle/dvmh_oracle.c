@@ -149,7 +149,7 @@ void orc_jac3d(double *A, double *B, long n, int iters, double *eps_hist, double
 void orc_sor3d(double *A, long n, int iters, double w, double *eps_hist, int threads) {
     int nt = pick_threads(threads);
     long nn = n * n;
-    int *threadSync = (int *)calloc((size_t)nt, sizeof(int));
+    int *slotReady = (int *)calloc((size_t)nt, sizeof(int));
     for (int it = 0; it < iters; it++) {
         double eps = 0.0;
 #pragma omp parallel num_threads(nt)
@@ -161,14 +161,14 @@ void orc_sor3d(double *A, long n, int iters, double w, double *eps_hist, int thr
             cur = omp_get_thread_num();
             working = omp_get_num_threads();
 #endif
-            long paralIters = (n - 2 - 1) / 1 + 1;
-            if (paralIters < working) working = (int)paralIters;
-            __atomic_store_n(&threadSync[cur], 0, __ATOMIC_SEQ_CST);
+            long outerIters = (n - 2 - 1) / 1 + 1;
+            if (outerIters < working) working = (int)outerIters;
+            __atomic_store_n(&slotReady[cur], 0, __ATOMIC_SEQ_CST);
 #pragma omp barrier
             for (i = 1; i <= n - 2; i++) {
                 if (cur > 0 && cur < working) {
-                    while (!__atomic_load_n(&threadSync[cur - 1], __ATOMIC_ACQUIRE)) {}
-                    __atomic_store_n(&threadSync[cur - 1], 0, __ATOMIC_RELEASE);
+                    while (!__atomic_load_n(&slotReady[cur - 1], __ATOMIC_ACQUIRE)) {}
+                    __atomic_store_n(&slotReady[cur - 1], 0, __ATOMIC_RELEASE);
                 }
 #pragma omp for schedule(static) nowait
                 for (j = 1; j <= n - 2; j++)
@@ -179,8 +179,8 @@ void orc_sor3d(double *A, long n, int iters, double w, double *eps_hist, int thr
                         leps = Max(leps, fabs(A[c] - s));
                     }
                 if (cur < working - 1) {
-                    while (__atomic_load_n(&threadSync[cur], __ATOMIC_ACQUIRE)) {}
-                    __atomic_store_n(&threadSync[cur], 1, __ATOMIC_RELEASE);
+                    while (__atomic_load_n(&slotReady[cur], __ATOMIC_ACQUIRE)) {}
+                    __atomic_store_n(&slotReady[cur], 1, __ATOMIC_RELEASE);
                 }
             }
 #pragma omp critical
@@ -188,7 +188,7 @@ void orc_sor3d(double *A, long n, int iters, double w, double *eps_hist, int thr
         }
         if (eps_hist) eps_hist[it] = eps;
     }
-    free(threadSync);
+    free(slotReady);
 }
 
 /* ------------------------------------------------------------------ C4 */
@@ -258,8 +258,10 @@ void orc_adi3d(double *A, long n, int steps, double c, double *max_hist, long *l
         long loc = -1;
 #pragma omp parallel num_threads(nt)
         {
-            double lmx = mx;
-            long lloc = loc;
+            /* per-thread partials start from the reduction's initial value, never
+             * from the shared result another thread may already have merged */
+            double lmx = -1.0;
+            long lloc = -1;
             long i, j, k;
 #pragma omp for schedule(static) nowait
             for (i = 0; i <= n - 1; i++)

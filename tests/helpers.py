@@ -155,6 +155,21 @@ def run_sor3d(n, iters, w=1.5, seed=None, targets=D.DEVICE_TYPE_HOST | D.DEVICE_
     return {'A': A, 'eps': eps, 'seconds': t}
 
 
+def run_sor3d_chain(n, iters, group, w=1.5, seed=None, targets=D.DEVICE_TYPE_HOST | D.DEVICE_TYPE_CUDA, flags=0):
+    """sor3d with `group` sweeps per batch (sor3d_iterate_chain): each group
+    runs through the chain handler of the sweep loop."""
+    seed = D.SEED_BASE + 3 if seed is None else seed
+    P = _program('sor3d')
+    P.set_targets(targets, flags)
+    P.fn('setup', [ctypes.c_long, ctypes.c_double, ctypes.c_ulonglong])(n, w, seed)
+    eps = np.zeros(iters)
+    t = P.fn('iterate_chain', [ctypes.c_int, ctypes.c_int, _c_dp], ctypes.c_double)(iters, group, dp(eps))
+    A = np.zeros(n ** 3)
+    P.fn('fetch', [_c_dp])(dp(A))
+    P.fn('teardown', [])()
+    return {'A': A, 'eps': eps, 'seconds': t}
+
+
 def run_adi3d(n, steps, c=0.3, seed=None, targets=D.DEVICE_TYPE_HOST | D.DEVICE_TYPE_CUDA, flags=0):
     seed = D.SEED_BASE + 4 if seed is None else seed
     P = _program('adi3d')

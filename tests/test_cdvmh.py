@@ -409,3 +409,97 @@ def test_file_scope_conversion(tmp_path):
         assert call in c, call
     spec = ast.literal_eval(open(tmp_path / 'jacg_cdv.loops').read().split('\n', 1)[1])
     assert '#define MAXOF(a, b) ((a) > (b) ? (a) : (b))' in spec['defines']
+
+
+# ---------------------------------------------------------------- wide: 256 handler parameters
+def _check_wide(n, m):
+    """programs/wide.cdv: one loop whose handlers receive 256 parameters (the
+    array, m and 254 weighted scalars); the result against a Python evaluation
+    of the same expression in the same order."""
+    rng = np.random.default_rng(n * 1000 + m)
+    c = rng.uniform(-1.0, 1.0, 254)
+    acc = c[0] * 1.0
+    for k in range(1, len(c)):
+        acc = acc + c[k] * float(k + 1)
+    want = np.array([acc + float(q) for q in range(n * m)])
+    f = _lib('wide').wide
+    f.argtypes = [ctypes.c_long, ctypes.c_long, _dp, _dp]
+    out = np.zeros(n * m)
+    f(n, m, H.dp(c), H.dp(out))
+    assert H.bits_equal(out, want)
+
+
+def test_wide_handler_host_path():
+    if D.stats()['gpu_present']:
+        pytest.skip('the host path runs where no GPU is present; the GPU tests cover this case')
+    _check_wide(5, 7)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('n,m', [(1, 1), (33, 65)])
+def test_wide_handler_gpu(n, m):
+    before = _gpu_loops()
+    _check_wide(n, m)
+    assert _gpu_loops() > before
+
+
+# ---------------------------------------------------------------- redarr: reduction arrays
+def redarr_reference(n, seed):
+    """Sequential execution of programs/redarr.cdv (numpy for the PARALLEL
+    loop, loops in program order for the ACROSS ones)."""
+    A = _u01(seed, np.arange(n * n, dtype=np.uint64)).reshape(n, n)
+    i, j = np.meshgrid(np.arange(n), np.arange(n), indexing='ij')
+    s = np.bincount(((i + j) % 5).ravel(), weights=A.ravel(), minlength=5)
+    mx = np.full(3, 0.75)
+    for c in range(3):
+        mx[c] = max(mx[c], np.max(A[:, c::3] * float(c + 1), initial=-np.inf))
+    mn = np.full(4, 0.25)
+    for r in range(4):
+        mn[r] = min(mn[r], np.min(A[r::4, :], initial=np.inf))
+    cnt = np.array([0, 10, 20], dtype=np.int64) + np.bincount(((i * n + j) % 3).ravel(), minlength=3)
+    lm = np.array([-1.0, 2.5])
+    for ii in range(1, n):
+        A[ii, :] = A[ii, :] + 0.5 * A[ii - 1, :]
+        for c in range(2):
+            lm[c] = max(lm[c], np.max(A[ii, c::2], initial=-np.inf))
+    wm = np.zeros(3)
+    for ii in range(1, n):
+        for jj in range(1, n):
+            A[ii, jj] = 0.5 * (A[ii - 1, jj] + A[ii, jj - 1]) + 0.25 * A[ii, jj]
+            q = (ii + 2 * jj) % 3
+            wm[q] = max(wm[q], A[ii, jj])
+    return {'sums': s, 'maxs': mx, 'mins': mn, 'counts': cnt, 'lmax': lm, 'wmax': wm, 'A': A.ravel()}
+
+
+def _check_redarr(n):
+    seed = D.SEED_BASE + 17
+    f = _lib('redarr').redarr
+    lp = ctypes.POINTER(ctypes.c_long)
+    f.argtypes = [ctypes.c_long, ctypes.c_ulonglong, _dp, _dp, _dp, lp, _dp, _dp, _dp]
+    got = {'sums': np.zeros(5), 'maxs': np.zeros(3), 'mins': np.zeros(4), 'counts': np.zeros(3, dtype=np.int64),
+           'lmax': np.zeros(2), 'wmax': np.zeros(3), 'A': np.zeros(n * n)}
+    f(n, seed, H.dp(got['sums']), H.dp(got['maxs']), H.dp(got['mins']), got['counts'].ctypes.data_as(lp),
+      H.dp(got['lmax']), H.dp(got['wmax']), H.dp(got['A']))
+    want = redarr_reference(n, seed)
+    assert H.bits_equal(got['A'], want['A'])
+    assert H.sum_close(got['sums'], want['sums'])
+    for k in ('maxs', 'mins', 'counts', 'lmax', 'wmax'):
+        assert np.array_equal(got[k], want[k]), (k, got[k], want[k])
+
+
+@pytest.mark.parametrize('n', [2, 9])
+def test_redarr_host_path(n):
+    if D.stats()['gpu_present']:
+        pytest.skip('the host path runs where no GPU is present; the GPU tests cover this case')
+    _check_redarr(n)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('n', [2, 3, 17, 70, 131])
+def test_redarr_gpu(n):
+    """Reduction arrays on the GPU: per-workgroup partials of every element
+    (element-major), folded per element by the multi-reduction finish; ACROSS
+    loops accumulate into pre-filled element slots."""
+    before = _gpu_loops()
+    _check_redarr(n)
+    assert _gpu_loops() > before

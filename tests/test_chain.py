@@ -1,0 +1,92 @@
+"""Chains of ACROSS sweeps: consecutive executions of the SOR sweep loop
+queued in one batch (programs/sor3d.c sor3d_iterate_chain) run through the
+loop's chain handler, every sweep two tile-planes behind the one before it in
+one launch sequence, each with its own REDUCTION(MAX) segment.  Results must
+equal the sequential sweeps bit for bit, whatever the group size, tile shape
+and grid size; configurations other than the tiled wavefront fall back to one
+sweep after another."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import dvmh_amd as D
+import helpers as H
+
+KIND_WAVEFRONT = 2
+
+
+def _force(block, tune):
+    r = D.runtime()
+    r.dvmh_amd_tuner_force_C.argtypes = [ctypes.c_long, ctypes.c_void_p, ctypes.c_void_p]
+    if block is None:
+        r.dvmh_amd_tuner_force_C(KIND_WAVEFRONT, None, None)
+        return
+    b = np.array(block, dtype=np.int64)
+    t = np.array(tune, dtype=np.int64)
+    r.dvmh_amd_tuner_force_C(KIND_WAVEFRONT, H.vp(b), H.vp(t))
+
+
+def _tile(nt, ta, tc):
+    tune = [0] * 8
+    tune[0] = 1      # DEPTH
+    tune[2] = 1      # VARIANT: tiled wavefront
+    tune[4] = ta     # TILE_A
+    tune[5] = tc     # TILE_B
+    return (nt, 1, 1), tune
+
+
+def test_chain_host_path():
+    """Without a GPU the regions run on the host: the batch queues nothing and
+    every sweep runs in order."""
+    if D.stats()['gpu_present']:
+        pytest.skip('the host path runs where no GPU is present; the GPU tests cover this case')
+    res = H.run_sor3d_chain(12, 5, 3)
+    ref = H.oracle_sor3d(12, 5)
+    assert H.bits_equal(res['A'], ref['A']) and np.array_equal(res['eps'], ref['eps'])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('cfg', [(64, 4, 8), (256, 8, 8), (256, 8, 16), (512, 16, 16)])
+@pytest.mark.parametrize('n,iters,group', [(5, 3, 3), (23, 7, 3), (23, 6, 6), (41, 9, 4), (70, 5, 5)])
+def test_chain_bit_exact(cfg, n, iters, group):
+    """Tiles overhanging the loop, tiles larger than it (n = 5), groups that
+    divide the sweeps and groups that leave a shorter last group."""
+    block, tune = _tile(*cfg)
+    _force(block, tune)
+    before = D.stats()
+    try:
+        res = H.run_sor3d_chain(n, iters, group)
+    finally:
+        _force(None, None)
+    after = D.stats()
+    ref = H.oracle_sor3d(n, iters)
+    assert H.bits_equal(res['A'], ref['A'])
+    assert np.array_equal(res['eps'], ref['eps'])
+    assert after['batch_groups'] > before['batch_groups']
+
+
+@pytest.mark.gpu
+def test_chain_falls_back_without_tiles():
+    """Hyperplane configuration (not the tiled wavefront): the chain handler
+    runs the group's sweeps one after another."""
+    _force((64, 1, 1), [1, 0, 0, 0, 0, 0, 0, 0])
+    try:
+        res = H.run_sor3d_chain(19, 5, 5)
+    finally:
+        _force(None, None)
+    ref = H.oracle_sor3d(19, 5)
+    assert H.bits_equal(res['A'], ref['A']) and np.array_equal(res['eps'], ref['eps'])
+
+
+@pytest.mark.gpu
+def test_chain_full_size():
+    """320^3 (the SOR workload's size), 6 sweeps in one chain, 8^3 tiles."""
+    block, tune = _tile(256, 8, 8)
+    _force(block, tune)
+    try:
+        res = H.run_sor3d_chain(320, 6, 6)
+    finally:
+        _force(None, None)
+    ref = H.oracle_sor3d(320, 6)
+    assert H.bits_equal(res['A'], ref['A']) and np.array_equal(res['eps'], ref['eps'])

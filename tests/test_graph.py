@@ -130,3 +130,40 @@ def test_graph_replay_interleaved_with_eager_work():
     ref3 = H.oracle_jac3d(20, 3)
     assert H.bits_equal(b['A'], ref3['A'])
     assert f.replays() > 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('prog,n', [('sor3d', 320), ('adi3d', 256)])
+def test_cost_model_form_at_full_size(prog, n):
+    """The cost-model form at the size the SOR (320^3) and ADI (256^3)
+    workloads are timed.  A first run explores the model's trial set (every
+    sweep may use another configuration) and must already match the oracle;
+    once no site is pending, a fresh run replays its repeating region as a
+    graph with the configurations the tuner kept, and matches as well."""
+    if CM.rt().dvmh_amd_model_load_C(CM.MODEL.encode()) < 0:
+        pytest.skip('no trained cost model')
+    run = {'sor3d': H.run_sor3d, 'adi3d': H.run_adi3d}[prog]
+    oracle = {'sor3d': H.oracle_sor3d, 'adi3d': H.oracle_adi3d}[prog]
+    explore, final = 10, 4
+
+    def same(res, ref):
+        if prog == 'sor3d':
+            return H.bits_equal(res['A'], ref['A']) and np.array_equal(res['eps'], ref['eps'])
+        return (H.bits_equal(res['A'], ref['A']) and np.array_equal(res['max'], ref['max'])
+                and np.array_equal(res['loc'], ref['loc']))
+
+    r = _rt()
+    with _Form(graph=1, sync=1, mode=2) as f:
+        first = run(n, explore, flags=D.REGION_AMD_GRAPH)
+        for _ in range(6):
+            if r.dvmh_amd_tuner_pending_C() == 0:
+                break
+            run(n, explore, flags=D.REGION_AMD_GRAPH)
+        pending = r.dvmh_amd_tuner_pending_C()
+        replays0 = D.stats()['graph_replays']
+        second = run(n, final, flags=D.REGION_AMD_GRAPH)
+        replays = D.stats()['graph_replays'] - replays0
+    assert same(first, oracle(n, explore))
+    assert pending == 0
+    assert same(second, oracle(n, final))
+    assert replays > 0

@@ -223,6 +223,52 @@ def test_convert_xy_is_a_diagonal_bijection(slash):
                 prev_hi = hi
 
 
+def _diagonal_order_table(Rx, Ry, slash, flip=False):
+    """Storage position of every (x, y) of an Rx x Ry plane, built by sorting
+    rather than by a closed form.  The slash layout is the plain one with y
+    mirrored (y -> Ry - 1 - y).  In the plain layout the cells are sorted by
+    diagonal d = x + y; inside a diagonal they ascend with y when the plane is
+    taller than wide (Rx < Ry), with x when it is wider (Rx > Ry), and for a
+    square plane with y up to the main anti-diagonal (d < Rx) and with x after
+    it.  This is the order rts/dvmh/include/dvmhlib_device.h:56-98 (and
+    cdvmh/include/cdvmh_helpers.h:250-298) store diagonalised planes in.
+    flip=True orders every diagonal by the other coordinate (a planted
+    mistake: x and y exchanged inside a diagonal)."""
+    cells = []
+    for y in range(Ry):
+        for x in range(Rx):
+            ym = Ry - 1 - y if slash else y
+            d = x + ym
+            by_y = (Rx < Ry or (Rx == Ry and d < Rx)) != flip
+            cells.append((d, ym if by_y else x, x, y))
+    cells.sort()
+    table = np.zeros((Ry, Rx), dtype=np.int64)
+    for pos, (_, _, x, y) in enumerate(cells):
+        table[y, x] = pos
+    return table
+
+
+@pytest.mark.parametrize('slash', [0, 1])
+def test_convert_xy_order_inside_each_diagonal(slash):
+    """The position inside a diagonal, not only the diagonal's run, is part of
+    the diagonal storage format emitted handlers index: the runtime's function
+    and the oracle equal the sorted table for every plane up to 40 x 40, and
+    differ from the table with x and y exchanged inside the diagonals on every
+    plane with a diagonal of two or more cells."""
+    r = _rt()
+    o = H.oracle()
+    for Rx in SIZES:
+        for Ry in SIZES:
+            want = _diagonal_order_table(Rx, Ry, slash)
+            for y in range(Ry):
+                for x in range(Rx):
+                    assert r.dvmh_amd_convert_xy_C(x, y, Rx, Ry, slash) == want[y, x], (Rx, Ry, x, y)
+                    assert o.orc_convert_xy(x, y, Rx, Ry, slash) == want[y, x], (Rx, Ry, x, y)
+            if min(Rx, Ry) > 1:
+                got = np.array([[r.dvmh_amd_convert_xy_C(x, y, Rx, Ry, slash) for x in range(Rx)] for y in range(Ry)])
+                assert not np.array_equal(got, _diagonal_order_table(Rx, Ry, slash, flip=True)), (Rx, Ry)
+
+
 # ---------------------------------------------------------------- change_filled_bounds2
 @pytest.mark.parametrize('rank', [1, 2, 3, 4])
 def test_change_filled_bounds2_all_masks(rank):

@@ -208,7 +208,8 @@ def test_adi3d_register_line_sweep_full_size():
     finally:
         _force(KIND_LINE, None, None)
     ref = H.oracle_adi3d(256, 2)
-    assert H.bits_equal(res['A'], ref['A']) and np.array_equal(res['loc'], ref['loc'])
+    assert H.bits_equal(res['A'], ref['A'])
+    assert np.array_equal(res['max'], ref['max']) and np.array_equal(res['loc'], ref['loc'])
 
 
 @pytest.mark.gpu
@@ -221,4 +222,5 @@ def test_adi3d_tiled_line_sweep_full_size():
     finally:
         _force(KIND_LINE, None, None)
     ref = H.oracle_adi3d(256, 2)
-    assert H.bits_equal(res['A'], ref['A']) and np.array_equal(res['loc'], ref['loc'])
+    assert H.bits_equal(res['A'], ref['A'])
+    assert np.array_equal(res['max'], ref['max']) and np.array_equal(res['loc'], ref['loc'])
