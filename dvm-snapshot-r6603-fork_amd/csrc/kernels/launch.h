@@ -91,6 +91,11 @@ void dvmh_hip_mlp_adam(float *P, const float *G, float *M, float *V, float lr, f
 // out[i] = network(X[i]) for n samples.
 void dvmh_hip_mlp_forward(const float *P, const float *X, long n, float *out, hipStream_t s);
 
+// Streaming copy (every byte read once and written once) and read-only pass
+// (every byte read once; a checksum of the data XORed into *sink) of `bytes`.
+void dvmh_hip_stream_copy(void *dst, const void *src, long bytes, hipStream_t s);
+void dvmh_hip_stream_read(const void *src, long bytes, unsigned long long *sink, hipStream_t s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -351,7 +351,9 @@ void performLoop(Loop *loop) {
         }
         if (batchQueue(loop)) return;
     }
-    batchDrain();   // loops queued before this one run first
+    // loops queued before this one run first (a loop without iterations
+    // touches no array: it only posts its reductions' initial values)
+    if (loop->iterations() > 0) batchDrain();
     performLoopNow(loop);
 }
 
@@ -394,3 +396,40 @@ void performLoopNow(Loop *loop) {
 }
 
 }  // namespace dvmh
+
+using namespace dvmh;
+
+namespace dvmh {
+// Arguments of a direct device fold (dvmh_amd_reduce_partials_C and its
+// variant without allocation); a bad argument stops the program.
+void checkFoldArgs(const char *where, long func, long rtType, long n, const void *vals, const void *locs, long locSize,
+        const void *outVal, const void *outLoc) {
+    DVMH_CHECK(n >= 1 && vals && outVal, "%s: %ld partials, values %p, result %p", where, n, vals, outVal);
+    DVMH_CHECK(func >= rf_SUM && func <= rf_MINLOC, "%s: reduction function %ld", where, func);
+    DVMH_CHECK(rtType == rt_CHAR || rtType == rt_UCHAR || rtType == rt_INT || rtType == rt_UINT || rtType == rt_LOGICAL ||
+                    rtType == rt_LONG || rtType == rt_ULONG || rtType == rt_LLONG || rtType == rt_ULLONG || rtType == rt_FLOAT ||
+                    rtType == rt_DOUBLE,
+            "%s: element type %ld has no GPU fold", where, rtType);
+    const bool loc = func == rf_MAXLOC || func == rf_MINLOC;
+    DVMH_CHECK(!loc || (locs && outLoc && locSize > 0), "%s: MAXLOC/MINLOC need loc records", where);
+}
+}  // namespace dvmh
+
+// A reduction's partial values folded on the device, outside a loop: the same
+// deterministic fold the loop finishes use, for handlers and tools that keep
+// their own partial buffers.  Scratch for the two-pass fold of more than
+// 16384 partials comes from the runtime's device cache.
+extern "C" DvmType dvmh_amd_reduce_partials_C(DvmType func, DvmType rtType, DvmType n, const void *vals, const void *locs,
+        DvmType locSize, void *outVal, void *outLoc) {
+    runtimeInit();
+    if (!gpu.present) return -1;
+    checkFoldArgs("dvmh_amd_reduce_partials_C", (long)func, (long)rtType, (long)n, vals, locs, (long)locSize, outVal, outLoc);
+    const bool loc = func == rf_MAXLOC || func == rf_MINLOC;
+    const long sb = dvmh_hip_reduce_scratch_bytes((long)n);
+    void *scratch = sb > 0 ? gpu.alloc((size_t)sb) : nullptr;
+    dvmh_hip_reduce_finish((int)func, (int)rtType, (long)n, vals, loc ? locs : nullptr, loc ? (long)locSize : 0, outVal,
+            loc ? outLoc : nullptr, scratch, gpu.stream);
+    HIP_CHECK(hipStreamSynchronize(gpu.stream));
+    if (scratch) gpu.release(scratch);
+    return 0;
+}

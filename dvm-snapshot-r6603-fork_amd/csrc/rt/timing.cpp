@@ -4,6 +4,7 @@
 #include <vector>
 
 #include "rt.h"
+#include "../kernels/launch.h"
 
 namespace dvmh {
 
@@ -111,3 +112,42 @@ extern "C" void dvmh_amd_set_sync_mode_C(DvmType mode) {
 }
 
 extern "C" void dvmh_amd_set_graph_C(DvmType enable) { settings.graph = enable != 0; }
+
+// Streaming kernels of known traffic, timed with HIP events on the runtime's
+// stream over `reps` back-to-back launches (after one untimed launch); the
+// buffers are allocated here for the measurement and freed after it.
+// Returns seconds per launch, or -1 without a GPU.
+static double streamSeconds(long bytes, long reps, bool copy) {
+    runtimeInit();
+    if (!gpu.present || bytes <= 0 || reps <= 0) return -1.0;
+    void *a = nullptr, *b = nullptr;
+    unsigned long long *sink = nullptr;
+    HIP_CHECK(hipMalloc(&a, (size_t)bytes));
+    if (copy) HIP_CHECK(hipMalloc(&b, (size_t)bytes));
+    HIP_CHECK(hipMalloc((void **)&sink, 64));
+    HIP_CHECK(hipMemsetAsync(a, 1, (size_t)bytes, gpu.stream));
+    HIP_CHECK(hipMemsetAsync(sink, 0, 64, gpu.stream));
+    auto launch = [&]() {
+        if (copy) dvmh_hip_stream_copy(b, a, bytes, gpu.stream);
+        else dvmh_hip_stream_read(a, bytes, sink, gpu.stream);
+    };
+    launch();
+    hipEvent_t e0, e1;
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    HIP_CHECK(hipEventRecord(e0, gpu.stream));
+    for (long r = 0; r < reps; r++) launch();
+    HIP_CHECK(hipEventRecord(e1, gpu.stream));
+    HIP_CHECK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    HIP_CHECK(hipEventDestroy(e0));
+    HIP_CHECK(hipEventDestroy(e1));
+    HIP_CHECK(hipFree(a));
+    if (b) HIP_CHECK(hipFree(b));
+    HIP_CHECK(hipFree(sink));
+    return ms * 1e-3 / (double)reps;
+}
+
+extern "C" double dvmh_amd_stream_copy_seconds_C(DvmType bytes, DvmType reps) { return streamSeconds(bytes, reps, true); }
+extern "C" double dvmh_amd_stream_read_seconds_C(DvmType bytes, DvmType reps) { return streamSeconds(bytes, reps, false); }

@@ -80,6 +80,26 @@ DvmType dvmh_amd_host_slots_C(void);
  * and run their 64-bit index kernels whatever the array sizes (also
  * DVMH_AMD_INDEX64=1); 0 restores the choice by extent. */
 void dvmh_amd_set_index64_C(DvmType on);
+/* Seconds per launch of the runtime's streaming kernels on a device buffer of
+ * `bytes` (allocated for the call): a copy that reads and writes every byte
+ * once, and a read-only pass; timed with HIP events over `reps` launches.
+ * They are the measured ceiling of a read-once/write-once kernel and the
+ * known-traffic dispatches that scale HBM counters.  -1 without a GPU. */
+double dvmh_amd_stream_copy_seconds_C(DvmType bytes, DvmType reps);
+double dvmh_amd_stream_read_seconds_C(DvmType bytes, DvmType reps);
+/* Folds n >= 1 partial values of one reduction, in device memory (vals; for
+ * rf_MAXLOC / rf_MINLOC also locs, n records of locSize bytes), into one value
+ * and its loc record written to outVal / outLoc (device or mapped memory), on
+ * the runtime's stream; returns when the result is written.  func: rf_SUM ..
+ * rf_MINLOC; rtType: rt_CHAR .. rt_DOUBLE element type.  The fold a loop's GPU
+ * reduction finish uses (replacing rts/dvmh/cuda_reduction.cu:168-312's
+ * per-warp results folded on the host): fixed partition and tree, so the
+ * result is deterministic; equal MAX/MIN values keep the lower loc (integer
+ * records of 1, 2, 4 or 8 bytes), else the lower partial index.  More than
+ * 16384 partials fold in two passes through scratch taken from the runtime's
+ * device cache.  Returns 0, -1 without a GPU. */
+DvmType dvmh_amd_reduce_partials_C(DvmType func, DvmType rtType, DvmType n, const void *vals, const void *locs, DvmType locSize,
+        void *outVal, void *outLoc);
 /* Device name / architecture string of the GPU in use ("" without one). */
 const char *dvmh_amd_device_name_C(void);
 /* Position of (x, y) in an Rx x Ry plane stored diagonal by diagonal (slash 0:
