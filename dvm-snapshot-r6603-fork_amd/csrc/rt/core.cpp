@@ -95,6 +95,7 @@ void Gpu::init() {
     snprintf(props.arch, sizeof(props.arch), "%.63s", p.gcnArchName);
     // Default-priority, non-blocking stream (never a prioritised one).
     HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    noallocInit(ordinal);
     // arrival counters of the multi-reduction finish: zeroed once here, every
     // launch leaves them zero again
     HIP_CHECK(hipMalloc((void **)&finishCounters, 64 * sizeof(unsigned)));

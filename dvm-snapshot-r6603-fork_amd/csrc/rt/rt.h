@@ -332,6 +332,7 @@ void graphNotePost(const std::shared_ptr<Reduction> &r, char *val, char *loc);
 void graphRunRegion(Region *region);
 void graphReleaseAll();
 void runtimeInit();   // settings, GPU and tuner set-up on first use
+void noallocInit(int device);   // device address of the no-allocation fold's scratch (noalloc.hip)
 void addPendingPost(const std::shared_ptr<Reduction> &r, char *val, char *loc, bool addInitial = false);
 
 }  // namespace dvmh

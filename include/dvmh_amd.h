@@ -100,6 +100,15 @@ double dvmh_amd_stream_read_seconds_C(DvmType bytes, DvmType reps);
  * device cache.  Returns 0, -1 without a GPU. */
 DvmType dvmh_amd_reduce_partials_C(DvmType func, DvmType rtType, DvmType n, const void *vals, const void *locs, DvmType locSize,
         void *outVal, void *outLoc);
+/* No counterpart in the reference: the same fold without allocating anything,
+ * for callers that cannot call hipMalloc (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid allocation).  Its scratch is a fixed
+ * 1 MiB array in device memory, one per GPU, held until the process ends, so
+ * it folds at most 715,784,192 partials (returns -2 with a message above
+ * that) and calls that overlap on one GPU race.  Same arguments and results
+ * as dvmh_amd_reduce_partials_C; -1 without a GPU. */
+DvmType dvmh_amd_reduce_partials_noalloc_C(DvmType func, DvmType rtType, DvmType n, const void *vals, const void *locs,
+        DvmType locSize, void *outVal, void *outLoc);
 /* Device name / architecture string of the GPU in use ("" without one). */
 const char *dvmh_amd_device_name_C(void);
 /* Position of (x, y) in an Rx x Ry plane stored diagonal by diagonal (slash 0:
