@@ -202,22 +202,23 @@ def batched_leg(rt, iters):
             'speedup': res['batched']['value'] / res['per_grid']['value'], 'parity_bit_exact': bool(ok)}
 
 
-def other_parity(rt, name, n, iters):
+def other_parity(rt, name, n, iters, chain=False):
     """The chosen form (cost-model configuration settled by the timed legs,
-    results synchronised only when needed, repeating regions replayed as HIP
-    graphs) at the size it is timed, for `iters` iterations from a fresh
-    setup, against the oracle: grids bit-exact, MAX / MAXLOC values and
-    indices exact."""
+    results synchronised only when needed; repeating regions replayed as HIP
+    graphs, or SOR's sweeps run as one chain) at the size and iteration count
+    it is timed, from a fresh setup, against the oracle: grids bit-exact, MAX /
+    MAXLOC values and indices exact."""
     import helpers as H
-    set_form(rt, 2, 1, 1)
-    flags = D.REGION_AMD_GRAPH
-    replays0 = D.stats()['graph_replays']
+    set_form(rt, 2, 1, 0 if chain else 1)
+    flags = 0 if chain else D.REGION_AMD_GRAPH
+    st0 = D.stats()
     if name == 'jac2d':
         res, ref = H.run_jac2d(n, iters, flags=flags), H.oracle_jac2d(n, iters)
         ok = {'grid_bit_exact': bool(H.bits_equal(res['A'], ref['A']) and H.bits_equal(res['B'], ref['B'])),
               'max_exact': bool(np.array_equal(res['eps'], ref['eps']))}
     elif name == 'sor3d':
-        res, ref = H.run_sor3d(n, iters, flags=flags), H.oracle_sor3d(n, iters)
+        res = H.run_sor3d_chain(n, iters, iters) if chain else H.run_sor3d(n, iters, flags=flags)
+        ref = H.oracle_sor3d(n, iters)
         ok = {'grid_bit_exact': bool(H.bits_equal(res['A'], ref['A'])),
               'max_exact': bool(np.array_equal(res['eps'], ref['eps']))}
     else:
@@ -225,70 +226,92 @@ def other_parity(rt, name, n, iters):
         ok = {'grid_bit_exact': bool(H.bits_equal(res['A'], ref['A'])),
               'max_exact': bool(np.array_equal(res['max'], ref['max'])),
               'loc_exact': bool(np.array_equal(res['loc'], ref['loc']))}
+    st1 = D.stats()
     ok['iterations'] = iters
-    ok['graph_replays'] = D.stats()['graph_replays'] - replays0
+    ok['form'] = 'chain of %d sweeps' % iters if chain else 'graph replay'
+    ok['graph_replays'] = st1['graph_replays'] - st0['graph_replays']
+    ok['batch_groups'] = st1['batch_groups'] - st0['batch_groups']
     set_form(rt, 0, 0, 0)
     return ok
 
 
-def chosen_layouts(rt, w):
+def chosen_layouts(rt, w, chain=False):
     """Kernel variant and device layout the tuner kept at each loop site of the
     workload (from the tuner's query; transform 0 = natural layout)."""
     out = {}
     q = (ctypes.c_long * 13)()
+    f = rt.dvmh_amd_tuner_query_chain_C if chain else rt.dvmh_amd_tuner_query_C
+    f.argtypes = [ctypes.c_char_p, ctypes.c_long, ctypes.POINTER(ctypes.c_long)]
+    f.restype = ctypes.c_long
     for line in w.lines:
-        if rt.dvmh_amd_tuner_query_C(w.cdv, line, q) > 0:
+        if f(w.cdv, line, q) > 0:
             out[str(line)] = {'block': [q[2], q[3], q[4]], 'variant': q[5 + 2], 'tile_a': q[5 + 4], 'tile_b': q[5 + 5],
                               'chunk': q[5 + 3], 'transform': q[5 + 6]}
     return out
 
 
-def other_configs(rt, parity=True):
-    """The remaining configurations, plain against the chosen (cost-model)
-    form.  Each form: wall time of a run without timing events, then the
-    kernels' time from HIP events on a second run whose launches are issued
-    eagerly (graph replays carry no timing marks), so that `frac` is the
-    kernels' algorithmic bytes over their own time.  The chosen form is then
-    checked against the oracle at the same size."""
+def time_form(rt, name, n, iters, mode, sync, graph):
+    """One form of a workload: tuner settled first, then the wall time of a run
+    without timing events, then the kernels' time from HIP events on a second
+    run whose launches are issued eagerly (graph replays carry no timing
+    marks), so that `frac` is the kernels' algorithmic bytes over their own
+    time."""
     from dvmh_amd import workloads as W
-    specs = [('jac2d', 2048, 100, 4, 'C1 2-D Jacobi 2048^2 x 100 (two loops per iteration)'),
-             ('sor3d', 320, 30, 3, 'C3 SOR 320^3 x 30 (ACROSS on three axes, REDUCTION(MAX))'),
-             ('adi3d', 256, 20, 3, 'C4 ADI 256^3 x 20 (six line sweeps + MAXLOC; 6 x 255 x 256^2 point-updates per step)')]
+    set_form(rt, mode, sync, graph)
+    w = W.ALL[name](n, flags=D.REGION_AMD_GRAPH if graph else 0)
+    w.setup()
+    for _ in range(200):
+        w.run(max(2, iters // 10) if name != 'sor3d_chain' else iters)
+        if mode == 0 or sum(rt.dvmh_amd_tuner_site_pending_C(w.cdv, ln) for ln in w.lines) == 0:
+            break
+    w.run(3 if name != 'sor3d_chain' else iters)   # settled: a repeating region is captured here, before timing
+    sec = w.run(iters)
+    rt.dvmh_amd_set_graph_C(0)
+    rt.dvmh_amd_reset_kernel_stats_C()
+    rt.dvmh_amd_set_kernel_timing_C(1)
+    w.run(iters)
+    rt.dvmh_amd_set_kernel_timing_C(0)
+    rt.dvmh_amd_set_graph_C(graph)
+    launches, ns = kernel_stats(rt, w.cdv, w.lines)
+    layouts = chosen_layouts(rt, w, chain=name == 'sor3d_chain') if mode == 2 else None
+    w.teardown()
+    upd, byt = w.points_per_iter(), w.bytes_per_iter()
+    r = {'value': upd * iters / sec, 'ms': sec * 1e3, 'ms_per_iteration': sec * 1e3 / iters,
+         'kernel_ms': ns * 1e-6 if launches else None, 'kernel_launch_marks': launches,
+         'frac': byt * iters / (ns * 1e-9) / HBM_PEAK if ns else None,
+         'frac_wall': byt * iters / sec / HBM_PEAK,
+         'point_updates_per_iteration': upd, 'bytes_per_iteration': byt}
+    if layouts is not None:
+        r['sites'] = layouts
+    return r
+
+
+def other_configs(rt, parity=True):
+    """The remaining configurations, plain (default configuration, per-loop
+    launches) against the chosen form (cost model, results synchronised only
+    when needed, repeating time steps replayed as graphs; SOR's sweeps chained
+    30 at a time), each chosen form then checked against the oracle for the
+    whole timed run."""
+    specs = [('jac2d', 2048, 100, 'C1 2-D Jacobi 2048^2 x 100 (two loops per iteration)'),
+             ('sor3d', 320, 30, 'C3 SOR 320^3 x 30 (ACROSS on three axes, REDUCTION(MAX))'),
+             ('adi3d', 256, 20, 'C4 ADI 256^3 x 20 (six line sweeps + MAXLOC; 6 x 255 x 256^2 point-updates per step)')]
     out = {}
-    for name, n, iters, piters, desc in specs:
+    for name, n, iters, desc in specs:
         r = {'workload': desc}
-        for form, mode, sync, graph in (('plain', 0, 0, 0), ('chosen', 2, 1, 1)):
-            set_form(rt, mode, sync, graph)
-            w = W.ALL[name](n, flags=D.REGION_AMD_GRAPH if graph else 0)
-            w.setup()
-            for _ in range(40):
-                w.run(max(2, iters // 10))
-                if mode == 0 or sum(rt.dvmh_amd_tuner_site_pending_C(w.cdv, ln) for ln in w.lines) == 0:
-                    break
-            w.run(3)   # settled: the repeating region is captured here, before timing
-            sec = w.run(iters)
-            rt.dvmh_amd_set_graph_C(0)
-            rt.dvmh_amd_reset_kernel_stats_C()
-            rt.dvmh_amd_set_kernel_timing_C(1)
-            w.run(iters)
-            rt.dvmh_amd_set_kernel_timing_C(0)
-            rt.dvmh_amd_set_graph_C(graph)
-            launches, ns = kernel_stats(rt, w.cdv, w.lines)
-            layouts = chosen_layouts(rt, w) if mode == 2 else None
-            w.teardown()
-            upd, byt = w.points_per_iter(), w.bytes_per_iter()
-            r[form] = {'value': upd * iters / sec, 'ms': sec * 1e3,
-                       'kernel_ms': ns * 1e-6 if launches else None, 'kernel_launch_marks': launches,
-                       'frac': byt * iters / (ns * 1e-9) / HBM_PEAK if ns else None,
-                       'frac_wall': byt * iters / sec / HBM_PEAK,
-                       'point_updates_per_iteration': upd, 'bytes_per_iteration': byt}
-            if layouts is not None:
-                r[form]['sites'] = layouts
+        r['plain'] = time_form(rt, name, n, iters, 0, 0, 0)
+        if name == 'sor3d':
+            # per-sweep chosen form (each sweep its own launch sequence), then the chain
+            r['chosen_per_sweep'] = time_form(rt, name, n, iters, 2, 1, 1)
+            r['chosen'] = time_form(rt, 'sor3d_chain', n, iters, 2, 1, 0)
+            r['chosen']['form'] = 'cost model per loop, the 30 sweeps as one chain (chain tile shape tuned per sweep)'
+        else:
+            r['chosen'] = time_form(rt, name, n, iters, 2, 1, 1)
         r['speedup'] = r['chosen']['value'] / r['plain']['value']
         if parity:
-            r['parity'] = other_parity(rt, name, n, piters)
+            r['parity'] = other_parity(rt, name, n, iters, chain=name == 'sor3d')
             log(name, 'parity', r['parity'])
         out[name] = r
+        log(name, {k: (v if not isinstance(v, dict) else {kk: vv for kk, vv in v.items() if kk != 'sites'}) for k, v in r.items()})
     set_form(rt, 0, 0, 0)
     return out
 

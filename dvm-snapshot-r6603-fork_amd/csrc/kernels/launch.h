@@ -4,6 +4,7 @@
 #pragma once
 
 #include <hip/hip_runtime_api.h>
+#include <stdint.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -92,9 +93,11 @@ void dvmh_hip_mlp_adam(float *P, const float *G, float *M, float *V, float lr, f
 void dvmh_hip_mlp_forward(const float *P, const float *X, long n, float *out, hipStream_t s);
 
 // Streaming copy (every byte read once and written once) and read-only pass
-// (every byte read once; a checksum of the data XORed into *sink) of `bytes`.
+// (every byte read once; per-wave checksums of the data stored into `sink`,
+// dvmh_hip_stream_sink_bytes(bytes) bytes of device memory) of `bytes`.
 void dvmh_hip_stream_copy(void *dst, const void *src, long bytes, hipStream_t s);
-void dvmh_hip_stream_read(const void *src, long bytes, unsigned long long *sink, hipStream_t s);
+long dvmh_hip_stream_sink_bytes(long bytes);
+void dvmh_hip_stream_read(const void *src, long bytes, uint64_t *sink, hipStream_t s);
 
 #ifdef __cplusplus
 }

@@ -22,7 +22,9 @@
 // runs the queue (batchDrain); the batch stays open.  Scalars a queued loop
 // reads are copied to the device when its region starts, so the loops of one
 // batch must see the same value of a scalar they share.
+#include <algorithm>
 #include <cstring>
+#include <map>
 
 #include "rt.h"
 #include "../kernels/launch.h"
@@ -112,6 +114,17 @@ static void runBatchGroup(std::vector<Loop *> &group) {
     DvmType ref = (DvmType)&ctx;
     callHandler(group[0]->batchH, &ref);
     for (SpecLoop *sl : ctx.specs) gpuRedFinishFlush(sl);   // finishes requested per loop, if any
+    // a configuration the tuner explores on this group (picked through the
+    // first loop's reference) is timed from its pick to here, per loop
+    SpecLoop *s0 = ctx.specs[0];
+    if (s0->explore && s0->tuneStart && !graphCapturing()) {
+        hipEvent_t ev1 = tunerEvent(1);
+        HIP_CHECK(hipEventRecord(ev1, s0->stream));
+        HIP_CHECK(hipEventSynchronize(ev1));
+        float ms = 0;
+        HIP_CHECK(hipEventElapsedTime(&ms, s0->tuneStart, ev1));
+        if (s0->candidate >= 0) tunerRecord(s0, ms * 1e-3 / (double)group.size());
+    }
     rt.loopsOnGpu += (long)group.size();
     batchGroups++;
     batchLoops += (long)group.size();
@@ -132,7 +145,7 @@ static void releaseLoop(Loop *l) {
     if (r && --r->pendingBatch == 0 && r->endedInBatch) delete r;
 }
 
-long tunerSitePending(const std::string &file, long line);
+long tunerSitePendingExcept(const std::string &file, long line, int kind);
 
 static void runQueue() {
     struct Group {
@@ -163,7 +176,7 @@ static void runQueue() {
     }
     queued.clear();
     for (auto &g : groups) {
-        const bool exploring = g.chain && tunerSitePending(g.loops[0]->file, g.loops[0]->line) > 0;
+        const bool exploring = g.chain && tunerSitePendingExcept(g.loops[0]->file, g.loops[0]->line, DVMH_AMD_KIND_CHAIN) > 0;
         if (g.loops.size() == 1 || exploring) {
             for (Loop *l : g.loops) {
                 l->queued = false;
@@ -227,12 +240,68 @@ extern "C" void *dvmh_amd_batch_scratch_C(DvmType batchRef, DvmType bytes) {
     return p;
 }
 
+// Pinned staging of batch uploads.  Copies are queued on the runtime's stream
+// without waiting; staging space is handed out in order and only reused after
+// a stream synchronisation, which happens when the space runs out (so every
+// copy staged before it has completed).
+namespace {
+struct UploadRing {
+    char *buf = nullptr;
+    size_t cap = 0, used = 0;
+    char *take(size_t bytes) {
+        bytes = (bytes + 255) & ~(size_t)255;
+        if (used + bytes > cap) {
+            if (buf) HIP_CHECK(hipStreamSynchronize(gpu.stream));
+            if (bytes > cap) {
+                if (buf) HIP_CHECK(hipHostFree(buf));
+                cap = std::max(bytes, (size_t)8 << 20);
+                HIP_CHECK(hipHostMalloc((void **)&buf, cap, hipHostMallocDefault));
+            }
+            used = 0;
+        }
+        char *p = buf + used;
+        used += bytes;
+        return p;
+    }
+};
+UploadRing uploadRing;
+
+// Device copies a batch handler keeps from one execution to the next, per
+// (tag, slot): re-uploaded only when the host bytes differ from the last ones.
+struct Persisted {
+    void *dev = nullptr;
+    size_t cap = 0;
+    std::vector<char> last;
+};
+std::map<std::pair<const void *, long>, Persisted> persisted;
+}  // namespace
+
+static void uploadAsync(void *dst, const void *src, size_t bytes) {
+    if (bytes == 0) return;
+    char *stage = uploadRing.take(bytes);
+    memcpy(stage, src, bytes);
+    HIP_CHECK(hipMemcpyAsync(dst, stage, bytes, hipMemcpyHostToDevice, gpu.stream));
+}
+
 extern "C" void dvmh_amd_batch_upload_C(DvmType batchRef, void *dst, const void *src, DvmType bytes) {
     asBatch(batchRef, "dvmh_amd_batch_upload_C");
-    void *stage = gpu.pinnedStaging((size_t)bytes);
-    memcpy(stage, src, (size_t)bytes);
-    HIP_CHECK(hipMemcpyAsync(dst, stage, (size_t)bytes, hipMemcpyHostToDevice, gpu.stream));
-    HIP_CHECK(hipStreamSynchronize(gpu.stream));   // the staging buffer is reused
+    uploadAsync(dst, src, (size_t)bytes);
+}
+
+extern "C" void *dvmh_amd_batch_persist_C(DvmType batchRef, const void *tag, DvmType slot, const void *src, DvmType bytes) {
+    asBatch(batchRef, "dvmh_amd_batch_persist_C");
+    DVMH_CHECK(bytes >= 0, "dvmh_amd_batch_persist_C: %ld bytes", (long)bytes);
+    Persisted &p = persisted[std::make_pair(tag, (long)slot)];
+    const size_t n = (size_t)bytes;
+    if (p.dev && p.last.size() == n && (n == 0 || memcmp(p.last.data(), src, n) == 0)) return p.dev;
+    if (!p.dev || p.cap < n) {
+        if (p.dev) gpu.release(p.dev);   // stream-ordered: earlier launches reading it come first
+        p.cap = std::max(n, (size_t)64);
+        p.dev = gpu.alloc(p.cap);
+    }
+    p.last.assign((const char *)src, (const char *)src + n);
+    uploadAsync(p.dev, src, n);
+    return p.dev;
 }
 
 // Folds each loop's partial segment and posts the results (one kernel, one D2H).
@@ -249,8 +318,9 @@ extern "C" void dvmh_amd_batch_red_finish_C(DvmType batchRef, DvmType redIndex, 
         DVMH_CHECK(r->func == r0->func && r->rtType == r0->rtType && r->elemCount == 1 && (r->isLoc() ? r->locSize : 0) == ls,
                 "Batched loops disagree on reduction %ld", redIndex);
     }
-    void *dOff = dvmh_amd_batch_scratch_C(batchRef, (nl + 1) * (long)sizeof(long));
-    dvmh_amd_batch_upload_C(batchRef, dOff, offsets, (nl + 1) * (long)sizeof(long));
+    // the segment offsets usually repeat from one execution of the handler to the next
+    void *dOff = dvmh_amd_batch_persist_C(batchRef, (const void *)b->loops[0]->batchH.func, 1000 + redIndex, offsets,
+            (nl + 1) * (long)sizeof(long));
     void *dRes = dvmh_amd_batch_scratch_C(batchRef, nl * (es + ls) + 64);
     void *dLoc = ls ? (char *)dRes + ((nl * es + 15) & ~15L) : nullptr;
     dvmh_hip_reduce_segments(r0->func, r0->rtType, nl, (const long *)dOff, partials, locPartials, ls, dRes, dLoc, gpu.stream);

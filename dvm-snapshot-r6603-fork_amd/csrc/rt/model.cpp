@@ -176,7 +176,8 @@ const SiteInfo *findSite(const char *keyPart) {
 // Ranking hook of the tuner: candidate indices by predicted time, fastest first.
 int modelRank(SiteInfo &s, std::vector<int> &order) {
     order.clear();
-    if (!model.ready || !gpu.present || graphCapturing()) return 0;
+    // chain sites are timed exhaustively: the records hold none of them
+    if (!model.ready || !gpu.present || graphCapturing() || s.kind == DVMH_AMD_KIND_CHAIN) return 0;
     std::vector<double> pred;
     predictSite(s, pred);
     order.resize(pred.size());

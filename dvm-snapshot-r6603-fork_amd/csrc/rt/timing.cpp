@@ -121,12 +121,11 @@ static double streamSeconds(long bytes, long reps, bool copy) {
     runtimeInit();
     if (!gpu.present || bytes <= 0 || reps <= 0) return -1.0;
     void *a = nullptr, *b = nullptr;
-    unsigned long long *sink = nullptr;
+    uint64_t *sink = nullptr;
     HIP_CHECK(hipMalloc(&a, (size_t)bytes));
     if (copy) HIP_CHECK(hipMalloc(&b, (size_t)bytes));
-    HIP_CHECK(hipMalloc((void **)&sink, 64));
+    else HIP_CHECK(hipMalloc((void **)&sink, (size_t)dvmh_hip_stream_sink_bytes(bytes) + 64));
     HIP_CHECK(hipMemsetAsync(a, 1, (size_t)bytes, gpu.stream));
-    HIP_CHECK(hipMemsetAsync(sink, 0, 64, gpu.stream));
     auto launch = [&]() {
         if (copy) dvmh_hip_stream_copy(b, a, bytes, gpu.stream);
         else dvmh_hip_stream_read(a, bytes, sink, gpu.stream);
@@ -145,7 +144,7 @@ static double streamSeconds(long bytes, long reps, bool copy) {
     HIP_CHECK(hipEventDestroy(e1));
     HIP_CHECK(hipFree(a));
     if (b) HIP_CHECK(hipFree(b));
-    HIP_CHECK(hipFree(sink));
+    if (sink) HIP_CHECK(hipFree(sink));
     return ms * 1e-3 / (double)reps;
 }
 

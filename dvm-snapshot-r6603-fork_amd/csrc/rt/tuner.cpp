@@ -143,6 +143,20 @@ std::vector<Candidate> makeCandidates(int kind, int loopRank, int depCount, cons
                         addCand(v, tl, 1, 1, {{DVMH_AMD_T_VARIANT, 1}, {DVMH_AMD_T_TILE_A, ck}, {DVMH_AMD_T_TILE_B, reg}});
                 }
         }
+    } else if (kind == DVMH_AMD_KIND_CHAIN) {
+        // a chain of sweeps of a tiled-wavefront loop: tile shapes only (the
+        // chain handler runs nothing else); times are per sweep of the chain
+        addCand(v, 256, 1, 1, {{DVMH_AMD_T_VARIANT, 1}, {DVMH_AMD_T_TILE_A, 8}, {DVMH_AMD_T_TILE_B, 8}});
+        const int nts[] = {128, 256, 512};
+        const int tas[] = {8, 12, 16};
+        const int tcs[] = {8, 16, 32};
+        for (int nt : nts)
+            for (int ta : tas)
+                for (int tc : tcs) {
+                    if ((long)(ta + 2) * (ta + 2) * (tc + 2) * 8 > 64512) continue;
+                    if (nt > 4 * ta * ta) continue;
+                    addCand(v, nt, 1, 1, {{DVMH_AMD_T_VARIANT, 1}, {DVMH_AMD_T_TILE_A, ta}, {DVMH_AMD_T_TILE_B, tc}});
+                }
     } else {
         // one launch per hyperplane: threads over the free dependent axes (y only
         // with three or more dependent axes), optional diagonal layout
@@ -482,15 +496,27 @@ long tunerSitePending(const std::string &file, long line) {
     }
     return pending;
 }
+
+// The same without the sites of one kind (a chain's own site, say).
+long tunerSitePendingExcept(const std::string &file, long line, int kind) {
+    std::string prefix = file + ":" + std::to_string(line) + "|";
+    long pending = 0;
+    for (auto &kv : sites) {
+        if (kv.first.compare(0, prefix.size(), prefix) != 0 || kv.second.kind == kind) continue;
+        pending += kv.second.pending();
+    }
+    return pending;
+}
 }  // namespace dvmh
 
 extern "C" DvmType dvmh_amd_tuner_site_pending_C(const char *file, DvmType line) { runtimeInit(); return tunerSitePending(file, line); }
 
-extern "C" DvmType dvmh_amd_tuner_query_C(const char *file, DvmType line, DvmType out[13]) {
-    runtimeInit();
+static DvmType tunerQuery(const char *file, DvmType line, int kind, DvmType out[13]) {
     std::string prefix = std::string(file) + ":" + std::to_string(line) + "|";
     for (auto &kv : sites) {
         if (kv.first.compare(0, prefix.size(), prefix) != 0) continue;
+        if (kv.second.kind == DVMH_AMD_KIND_CHAIN && kind != DVMH_AMD_KIND_CHAIN) continue;
+        if (kind == DVMH_AMD_KIND_CHAIN && kv.second.kind != DVMH_AMD_KIND_CHAIN) continue;
         int b = kv.second.bestMeasured();
         if (b < 0) return 0;
         const Candidate &c = kv.second.cands[b];
@@ -501,4 +527,14 @@ extern "C" DvmType dvmh_amd_tuner_query_C(const char *file, DvmType line, DvmTyp
         return 1;
     }
     return 0;
+}
+
+extern "C" DvmType dvmh_amd_tuner_query_C(const char *file, DvmType line, DvmType out[13]) {
+    runtimeInit();
+    return tunerQuery(file, line, -1, out);
+}
+
+extern "C" DvmType dvmh_amd_tuner_query_chain_C(const char *file, DvmType line, DvmType out[13]) {
+    runtimeInit();
+    return tunerQuery(file, line, DVMH_AMD_KIND_CHAIN, out);
 }

@@ -78,6 +78,15 @@ class Sor3d(Workload):
         return self.P.fn('iterate', [ctypes.c_int, _dp], ctypes.c_double)(iters, None)
 
 
+class Sor3dChain(Sor3d):
+    """The SOR sweeps `group` at a time in one batch (sor3d_iterate_chain):
+    each group runs through the sweep loop's chain handler."""
+    group = 30
+
+    def run(self, iters):
+        return self.P.fn('iterate_chain', [ctypes.c_int, ctypes.c_int, _dp], ctypes.c_double)(iters, self.group, None)
+
+
 class Adi3d(Workload):
     name, cdv, lines, seed_offset = 'adi3d', b'adi3d.cdv', (34, 39, 44, 49, 54, 59, 64), 4
 
@@ -97,4 +106,4 @@ class Adi3d(Workload):
         return 16 * self.points_per_iter() + 8 * self.size ** 3
 
 
-ALL = {'jac2d': Jac2d, 'jac3d': Jac3d, 'sor3d': Sor3d, 'adi3d': Adi3d}
+ALL = {'jac2d': Jac2d, 'jac3d': Jac3d, 'sor3d': Sor3d, 'sor3d_chain': Sor3dChain, 'adi3d': Adi3d}

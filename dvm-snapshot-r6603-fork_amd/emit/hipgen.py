@@ -1297,10 +1297,11 @@ def batch_handler(L):
         b.append(IND + '}')
         b.append(IND + 'tstart[nl] = tiles;')
         b.append('}')
-    b.append(f'long *dDesc = (long *)dvmh_amd_batch_scratch_C(batchRef, nl * {dlen} * (long)sizeof(long));')
-    b.append(f'dvmh_amd_batch_upload_C(batchRef, dDesc, desc, nl * {dlen} * (long)sizeof(long));')
-    b.append('long *dStart = (long *)dvmh_amd_batch_scratch_C(batchRef, (nl + 1) * (long)sizeof(long));')
-    b.append('dvmh_amd_batch_upload_C(batchRef, dStart, tstart, (nl + 1) * (long)sizeof(long));')
+    b.append('// descriptors and tile starts usually repeat from one time step to the next:')
+    b.append('// the runtime keeps their device copies and uploads only what changed')
+    b.append('static const char dvmh_tag = 0;')
+    b.append(f'long *dDesc = (long *)dvmh_amd_batch_persist_C(batchRef, &dvmh_tag, 0, desc, nl * {dlen} * (long)sizeof(long));')
+    b.append('long *dStart = (long *)dvmh_amd_batch_persist_C(batchRef, &dvmh_tag, 1, tstart, (nl + 1) * (long)sizeof(long));')
     for red in L.reds:
         b.append(f'{red[2]} *{red[1]}_grid = ({red[2]} *)dvmh_amd_batch_scratch_C(batchRef, (tiles > 0 ? tiles : 1) * (long)sizeof({red[2]}));')
         if len(red) > 3:
@@ -2046,7 +2047,8 @@ def chain_handler(L):
          'long nl = (long)dvmh_amd_batch_count_C(batchRef);',
          'DvmType loop_ref = dvmh_amd_batch_loop_C(batchRef, 0);',
          'DvmType tune[DVMH_AMD_TUNE_N];',
-         'dvmh_amd_loop_get_tuning_C(loop_ref, DVMH_AMD_KIND_WAVEFRONT, tune);',
+         '// the chain has its own tuner site: tile shapes timed per sweep of a chain',
+         'dvmh_amd_loop_get_tuning_C(loop_ref, DVMH_AMD_KIND_CHAIN, tune);',
          'dim3 threads(0, 0, 0);',
          'hipStream_t stream;',
          *regs_query(f'{L.name}_chain_kernel<int>'),

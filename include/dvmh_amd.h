@@ -31,6 +31,7 @@ extern "C" {
                                       (VARIANT 1: TILE_A = points per thread on the innermost
                                       axis, CHUNK = planes per workgroup on the outermost) */
 #define DVMH_AMD_KIND_BATCH 4      /* PARALLEL loops of many instances run by one batch kernel */
+#define DVMH_AMD_KIND_CHAIN 5      /* consecutive sweeps of a tiled-wavefront ACROSS loop run by its chain handler */
 /* tune[] slots */
 #define DVMH_AMD_T_DEPTH 0     /* iterations of the slowest mapped axis per thread */
 #define DVMH_AMD_T_XCD 1       /* 1: XCD-aware block-to-tile order */
@@ -145,6 +146,11 @@ DvmType dvmh_amd_batch_loop_C(DvmType batchRef, DvmType i);
 void *dvmh_amd_batch_param_C(DvmType batchRef, DvmType i, DvmType p);
 /* Copies bytes from host memory to device memory dst on the runtime's stream. */
 void dvmh_amd_batch_upload_C(DvmType batchRef, void *dst, const void *src, DvmType bytes);
+/* A device copy of `bytes` host bytes that the runtime keeps for the caller's
+ * (tag, slot) across executions of a batch handler: uploaded (asynchronously,
+ * stream-ordered) only when the bytes differ from the last ones given for that
+ * key.  For descriptor tables that repeat from one time step to the next. */
+void *dvmh_amd_batch_persist_C(DvmType batchRef, const void *tag, DvmType slot, const void *src, DvmType bytes);
 /* Inside a batch handler: finish reduction redIndex of every loop in the batch.
  * partials holds per-loop segments [offsets[i], offsets[i+1]) of values (and
  * locs) in device memory; one kernel folds each segment and the results are
@@ -182,6 +188,8 @@ DvmType dvmh_amd_tuner_site_pending_C(const char *file, DvmType line);
  * out[1] time in ns, out[2..4] block x,y,z, out[5..12] tune[0..7]; returns 0
  * if unknown. */
 DvmType dvmh_amd_tuner_query_C(const char *file, DvmType line, DvmType out[13]);
+/* The same for the chain site of that loop (DVMH_AMD_KIND_CHAIN). */
+DvmType dvmh_amd_tuner_query_chain_C(const char *file, DvmType line, DvmType out[13]);
 /* Forces the launch configuration (block[3], tune[DVMH_AMD_TUNE_N]) of every
  * loop of the given kind, over whatever the mode would choose; forced
  * executions are not timed.  block == NULL removes the forcing for the kind. */

@@ -90,3 +90,39 @@ def test_chain_full_size():
         _force(None, None)
     ref = H.oracle_sor3d(320, 6)
     assert H.bits_equal(res['A'], ref['A']) and np.array_equal(res['eps'], ref['eps'])
+
+
+@pytest.mark.gpu
+def test_chain_tuned_bit_exact():
+    """Cost-model mode: the per-loop site is explored sweep by sweep inside the
+    first groups, then the chain's own site times its tile shapes group by
+    group (exhaustively: no records of chains train the model).  Every run is
+    bit-exact and the chain site ends settled with a recorded best shape."""
+    from dvmh_amd import costmodel as CM
+    r = CM.rt()
+    if r.dvmh_amd_model_load_C(CM.MODEL.encode()) < 0:
+        pytest.skip('no trained cost model')
+    r.dvmh_amd_set_sync_mode_C.argtypes = [ctypes.c_long]
+    r.dvmh_amd_tuner_query_chain_C.argtypes = [ctypes.c_char_p, ctypes.c_long, ctypes.POINTER(ctypes.c_long)]
+    r.dvmh_amd_tuner_query_chain_C.restype = ctypes.c_long
+    n, iters, group = 40, 10, 5
+    ref = H.oracle_sor3d(n, iters)
+    r.dvmh_amd_tuner_clear_C()
+    r.dvmh_amd_tuner_set_mode_C(2)
+    r.dvmh_amd_set_sync_mode_C(1)
+    runs = 0
+    try:
+        for runs in range(1, 80):
+            res = H.run_sor3d_chain(n, iters, group)
+            assert H.bits_equal(res['A'], ref['A']) and np.array_equal(res['eps'], ref['eps']), runs
+            if r.dvmh_amd_tuner_site_pending_C(b'sor3d.cdv', 32) == 0:
+                break
+        pending = r.dvmh_amd_tuner_site_pending_C(b'sor3d.cdv', 32)
+        q = (ctypes.c_long * 13)()
+        found = r.dvmh_amd_tuner_query_chain_C(b'sor3d.cdv', 32, q)
+    finally:
+        r.dvmh_amd_tuner_set_mode_C(0)
+        r.dvmh_amd_set_sync_mode_C(0)
+        r.dvmh_amd_tuner_clear_C()
+    assert pending == 0, (pending, runs)
+    assert found == 1 and q[5 + 2] == 1   # a tiled-wavefront shape kept for the chain
