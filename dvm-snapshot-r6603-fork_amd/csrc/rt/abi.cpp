@@ -86,8 +86,7 @@ void allocateHost(Data *d) {
     d->host = (char *)calloc(nb ? nb : 1, 1);
     DVMH_CHECK(d->host != nullptr, "Out of host memory allocating %zu bytes", nb);
     d->ownsHost = true;
-    d->hostActual = true;
-    d->devActual = false;
+    d->hostOwnsAll();
 }
 
 void fillUserHeader(Data *d) {
@@ -341,6 +340,8 @@ extern "C" void dvmh_forget_header_(DvmType dvmDesc[]) {
     if (dvmDesc) dvmDesc[0] = 0;
 }
 
+static void fetchMissing(Data *d, const Box &b);
+
 // ============================================================ regular variables
 static Data *registerRegular(const void *addr, long rank, long typeSize, const long sizes[]) {
     Data *d = regularVarOf(addr);
@@ -352,6 +353,7 @@ static Data *registerRegular(const void *addr, long rank, long typeSize, const l
     d->host = (char *)addr;
     d->ownsHost = false;
     d->isRegular = true;
+    d->hostOwnsAll();
     rt.regularVars[addr] = d;
     rt.allData.push_back(d);
     return d;
@@ -392,7 +394,7 @@ extern "C" void dvmh_data_exit_C(const void *addr, DvmType saveFlag) {
     Guard g;
     Data *d = regularVarOf(addr);
     if (!d) return;
-    if (saveFlag && d->hasDev && !d->hostActual) d->toHost();
+    if (saveFlag) fetchMissing(d, d->whole());
     deleteData(d);
 }
 
@@ -440,10 +442,53 @@ extern "C" DvmType dvmh_fill_header_C(DvmType deviceNum, const void *baseAddr, c
 }
 
 // ============================================================ actual state
+static void moveBlock(Data *d, const long lo[], const long hi[], bool toHost);
+
+static Box boxOf(const Data *d, const long lo[], const long hi[]) {
+    Box b;
+    for (int a = 0; a < d->rank; a++) { b.lo[a] = lo[a]; b.hi[a] = hi[a]; }
+    return b;
+}
+
+// The parts of b whose host copy is stale come from the device (where they
+// are current); b is then current on the host.
+static void fetchMissing(Data *d, const Box &b) {
+    std::vector<Box> need = d->hostPieces.missing(b);
+    if (need.empty()) return;
+    DVMH_INTERNAL(d->hasDev, "array elements current nowhere");
+    bool whole = d->hostPieces.empty();
+    const Box w = d->whole();
+    for (int a = 0; a < d->rank && whole; a++) whole = b.lo[a] == w.lo[a] && b.hi[a] == w.hi[a];
+    if (whole) {
+        d->toHost();   // the whole array, in one copy
+        return;
+    }
+    for (const Box &x : need) moveBlock(d, x.lo, x.hi, true);
+    d->hostPieces.add(b);
+}
+
+// The device copy is made current everywhere: the whole array when the device
+// holds none of it, else only the pieces the host wrote since.
+static void uploadMissing(Data *d) {
+    d->ensureDev();
+    std::vector<Box> need = d->devPieces.missing(d->whole());
+    if (need.empty()) return;
+    if (d->devPieces.empty()) {
+        d->toDevice();
+        return;
+    }
+    for (const Box &x : need) moveBlock(d, x.lo, x.hi, false);
+    d->devPieces.add(d->whole());
+}
+
+// Each array keeps the pieces where its host copy and its device copy are
+// current (Data::hostPieces / devPieces); transfers move only the pieces a
+// request finds stale, sub-arrays as packed blocks (rts/dvmh/dvmh_data.cpp:339-600,
+// getActualBase / setActual on DvmhPieces).
 static void getActual(Data *d) {
     batchDrain();
     flushDeferredPosts();
-    if (d->hasDev && d->devActual && !d->hostActual) d->toHost();
+    fetchMissing(d, d->whole());
 }
 
 extern "C" void dvmh_get_actual_variable2_(void *addr) {   // :2325
@@ -537,21 +582,20 @@ extern "C" void dvmh_get_actual_subarray_C(const DvmType dvmDesc[], DvmType rank
     va_end(ap);
     batchDrain();
     flushDeferredPosts();
-    if (d->hasDev && d->devActual && !d->hostActual) moveBlock(d, lo, hi, true);
+    fetchMissing(d, boxOf(d, lo, hi));
 }
 
 extern "C" void dvmh_get_actual_all2_() {
     Guard g;
     batchDrain();
     flushDeferredPosts();
-    for (Data *d : rt.allData)
-        if (d->hasDev && d->devActual && !d->hostActual) d->toHost();
+    for (Data *d : rt.allData) fetchMissing(d, d->whole());
 }
 
 extern "C" void dvmh_actual_variable2_(const void *addr) {   // host copy is the newest
     Guard g;
     Data *d = regularVarOf(addr);
-    if (d) { d->hostActual = true; d->devActual = false; }
+    if (d) d->hostOwnsAll();
 }
 
 extern "C" void dvmh_actual_subvariable_C(const void *addr, DvmType rank, ...) {
@@ -564,9 +608,8 @@ extern "C" void dvmh_actual_array2_(const DvmType dvmDesc[]) {
     Guard g;
     Data *d = asData(dvmDesc, "dvmh_actual_array2_");
     batchDrain();
-    if (d->hasDev && d->devActual && !d->hostActual) d->toHost();   // keep elements the host did not write
-    d->hostActual = true;
-    d->devActual = false;
+    fetchMissing(d, d->whole());   // keep elements the host did not write
+    d->hostOwnsAll();
 }
 
 extern "C" void dvmh_actual_subarray_C(const DvmType dvmDesc[], DvmType rank, ...) {   // :120
@@ -579,23 +622,19 @@ extern "C" void dvmh_actual_subarray_C(const DvmType dvmDesc[], DvmType rank, ..
     va_start(ap, rank);
     subarrayBounds(d, ap, lo, hi);
     va_end(ap);
-    if (d->hasDev && d->devActual) {
-        // the device stays the owner of everything but the block written on the host
-        if (!d->hostActual) moveBlock(d, lo, hi, false);
-        else d->devActual = false;
-    } else {
-        d->hostActual = true;
-        d->devActual = false;
-    }
+    // the host wrote the block: its host copy is current there, the device's is
+    // not; nothing moves until a region needs the block on the device
+    const Box b = boxOf(d, lo, hi);
+    d->hostPieces.add(b);
+    d->devPieces.remove(b);
 }
 
 extern "C" void dvmh_actual_all2_() {
     Guard g;
     batchDrain();
     for (Data *d : rt.allData) {
-        if (d->hasDev && d->devActual && !d->hostActual) d->toHost();
-        d->hostActual = true;
-        d->devActual = false;
+        fetchMissing(d, d->whole());
+        d->hostOwnsAll();
     }
 }
 
@@ -627,11 +666,12 @@ extern "C" void dvmh_remote_access_C(DvmType[], const DvmType[], DvmType, ...) {
     fatal("error", "REMOTE_ACCESS is not supported by this single-process runtime");
 }
 
-static long linearIndex(Data *d, va_list &ap, bool *inside) {
+static long linearIndex(Data *d, va_list &ap, bool *inside, long *point = nullptr) {
     long off = 0;
     *inside = true;
     for (int a = 0; a < d->rank; a++) {
         DvmType idx = va_arg(ap, DvmType);
+        if (point) point[a] = idx;
         if (idx < d->space[a].lo || idx > d->space[a].hi) *inside = false;
         off = off * d->space[a].size() + (idx - d->space[a].lo);
     }
@@ -663,10 +703,12 @@ extern "C" DvmType dvmh_calc_linear_C(const DvmType dvmDesc[], DvmType rank, ...
 
 static void *elementAddr(Data *d, va_list &ap) {
     bool inside;
-    long off = linearIndex(d, ap, &inside);
+    long point[MAX_RANK];
+    long off = linearIndex(d, ap, &inside, point);
     DVMH_CHECK(inside, "Element index is out of the array bounds");
-    // host code touches the element: the host copy must be current
-    if (d->hasDev && d->devActual && !d->hostActual) d->toHost();
+    // host code touches the element: when the host copy of that element is
+    // stale (the program did not get_actual it), the stale pieces come down
+    if (!d->hostPieces.holds(point)) fetchMissing(d, d->whole());
     return d->host + off * d->typeSize;
 }
 
@@ -753,12 +795,15 @@ extern "C" void dvmh_region_execute_on_targets_C(DvmType curRegion, DvmType devi
     for (auto &ra : r->arrays) {
         Data *d = ra.data;
         if (ra.intent & INTENT_LOCAL) { d->ensureDev(); continue; }
-        if (!d->devActual) d->toDevice();
+        uploadMissing(d);
     }
     for (auto &rs : r->scalars) {
         Data *d = rs.data;
-        if (!d->hasDev) { d->ensureDev(); d->devActual = false; }
-        if (!d->devActual || d->hostActual) { gpu.h2d(d->dev.ptr, d->host, (size_t)d->typeSize); d->devActual = true; }
+        d->ensureDev();
+        if (!d->devWhole() || d->hostWhole()) {
+            gpu.h2d(d->dev.ptr, d->host, (size_t)d->typeSize);
+            d->devPieces.add(d->whole());
+        }
     }
 }
 
@@ -766,12 +811,8 @@ extern "C" void dvmh_region_end_C(DvmType curRegion) {   // :3122
     Guard g;
     Region *r = asRegion(curRegion, "dvmh_region_end_C");
     if (r->onGpu) {
-        for (auto &ra : r->arrays) {
-            if (ra.intent & (INTENT_OUT)) {
-                ra.data->devActual = true;
-                ra.data->hostActual = false;
-            }
-        }
+        for (auto &ra : r->arrays)
+            if (ra.intent & (INTENT_OUT)) ra.data->devOwnsAll();   // written on the device
         bool needSync = false;
         for (auto &rs : r->scalars)
             if (rs.intent & INTENT_OUT) needSync = true;
@@ -781,6 +822,11 @@ extern "C" void dvmh_region_end_C(DvmType curRegion) {   // :3122
         }
         if (!r->deferred.empty()) graphRunRegion(r);
         if (settings.syncMode == 1 && !(r->flags & REGION_ASYNC)) flushDeferredPosts();
+    }
+    if (!r->onGpu) {
+        // a region run on the host wrote its output arrays there
+        for (auto &ra : r->arrays)
+            if (ra.intent & (INTENT_OUT)) ra.data->hostOwnsAll();
     }
     if (rt.curRegion == r) rt.curRegion = nullptr;
     if (!batchKeepsRegion(r)) delete r;   // else freed when its queued loops have run
@@ -1304,12 +1350,48 @@ extern "C" void dvmh_amd_get_stats_C(DvmType out[8]) {
 
 extern "C" DvmType dvmh_amd_layout_changes_C(void) { return layoutChanges(); }
 
+extern "C" void dvmh_amd_transfer_stats_C(DvmType out[2]) {
+    ensureInit();
+    out[0] = gpu.h2dBytes;
+    out[1] = gpu.d2hBytes;
+}
+
+// The piece algebra the actual-state tracking uses, on its own: `nops`
+// operations (code 1 add, 2 remove, then lo[rank], hi[rank]) applied to an
+// empty set, then the parts of `query` the set misses, written as boxes
+// (lo[rank], hi[rank]) into outBoxes; returns their count (written up to maxBoxes).
+extern "C" DvmType dvmh_amd_pieces_eval_C(DvmType rank, const DvmType *ops, DvmType nops, const DvmType *query, DvmType *outBoxes,
+        DvmType maxBoxes) {
+    DVMH_CHECK(rank >= 0 && rank <= MAX_RANK, "dvmh_amd_pieces_eval_C: rank %ld", (long)rank);
+    Pieces p;
+    p.rank = (int)rank;
+    const long rec = 1 + 2 * rank;
+    for (long i = 0; i < nops; i++) {
+        const DvmType *o = ops + i * rec;
+        Box b;
+        for (int a = 0; a < rank; a++) { b.lo[a] = o[1 + a]; b.hi[a] = o[1 + rank + a]; }
+        if (o[0] == 1) p.add(b);
+        else if (o[0] == 2) p.remove(b);
+        else DVMH_CHECK(false, "dvmh_amd_pieces_eval_C: operation %ld", (long)o[0]);
+    }
+    Box q;
+    for (int a = 0; a < rank; a++) { q.lo[a] = query[a]; q.hi[a] = query[rank + a]; }
+    std::vector<Box> miss = p.missing(q);
+    for (long i = 0; i < (long)miss.size() && i < maxBoxes; i++)
+        for (int a = 0; a < rank; a++) {
+            outBoxes[i * 2 * rank + a] = miss[i].lo[a];
+            outBoxes[i * 2 * rank + rank + a] = miss[i].hi[a];
+        }
+    return (DvmType)miss.size();
+}
+
 extern "C" DvmType dvmh_amd_host_slots_C(void) {
     ensureInit();
     return hostSlotCount();
 }
 
 extern "C" void dvmh_amd_reset_stats_C(void) {
+    gpu.h2dBytes = gpu.d2hBytes = 0;
     rt.loopsOnHost = rt.loopsOnGpu = 0;
     rt.gpuLoopSeconds = 0;
     graphReplays = batchGroups = batchLoops = 0;

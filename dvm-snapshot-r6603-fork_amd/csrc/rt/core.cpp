@@ -183,11 +183,13 @@ void *Gpu::pinnedStaging(size_t bytes) {
 }
 
 void Gpu::h2d(void *dst, const void *src, size_t bytes) {
+    h2dBytes += (long)bytes;
     HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream));
     HIP_CHECK(hipStreamSynchronize(stream));
 }
 
 void Gpu::d2h(void *dst, const void *src, size_t bytes) {
+    d2hBytes += (long)bytes;
     HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream));
     HIP_CHECK(hipStreamSynchronize(stream));
 }
@@ -249,6 +251,126 @@ Data::~Data() {
     magic = 0;
 }
 
+// ------------------------------------------------------------ pieces
+static bool disjoint(int rank, const Box &a, const Box &b) {
+    for (int i = 0; i < rank; i++)
+        if (a.hi[i] < b.lo[i] || b.hi[i] < a.lo[i]) return true;
+    return false;
+}
+
+static bool boxEmpty(int rank, const Box &a) {
+    for (int i = 0; i < rank; i++)
+        if (a.hi[i] < a.lo[i]) return true;
+    return false;
+}
+
+// x minus b as at most 2 * rank disjoint boxes: slabs below and above b on
+// each axis in turn, the remaining core clipped to b on the axes done so far.
+static void subtractBox(int rank, const Box &x, const Box &b, std::vector<Box> &out) {
+    if (disjoint(rank, x, b)) {
+        out.push_back(x);
+        return;
+    }
+    Box cur = x;
+    for (int a = 0; a < rank; a++) {
+        if (cur.lo[a] < b.lo[a]) {
+            Box s = cur;
+            s.hi[a] = b.lo[a] - 1;
+            out.push_back(s);
+            cur.lo[a] = b.lo[a];
+        }
+        if (cur.hi[a] > b.hi[a]) {
+            Box s = cur;
+            s.lo[a] = b.hi[a] + 1;
+            out.push_back(s);
+            cur.hi[a] = b.hi[a];
+        }
+    }
+}
+
+void Pieces::remove(const Box &b) {
+    if (boxEmpty(rank, b)) return;
+    std::vector<Box> out;
+    for (const Box &x : boxes) subtractBox(rank, x, b, out);
+    boxes.swap(out);
+}
+
+void Pieces::add(const Box &b) {
+    if (boxEmpty(rank, b)) return;
+    remove(b);
+    boxes.push_back(b);
+    // merge boxes that differ on one axis only and touch there
+    for (bool merged = true; merged;) {
+        merged = false;
+        for (size_t i = 0; i < boxes.size() && !merged; i++)
+            for (size_t j = i + 1; j < boxes.size() && !merged; j++) {
+                int diff = -1, ndiff = 0;
+                for (int a = 0; a < rank; a++)
+                    if (boxes[i].lo[a] != boxes[j].lo[a] || boxes[i].hi[a] != boxes[j].hi[a]) { diff = a; ndiff++; }
+                if (ndiff != 1) continue;
+                Box &x = boxes[i];
+                const Box &y = boxes[j];
+                if (x.hi[diff] + 1 == y.lo[diff]) x.hi[diff] = y.hi[diff];
+                else if (y.hi[diff] + 1 == x.lo[diff]) x.lo[diff] = y.lo[diff];
+                else continue;
+                boxes.erase(boxes.begin() + (long)j);
+                merged = true;
+            }
+    }
+}
+
+std::vector<Box> Pieces::missing(const Box &b) const {
+    std::vector<Box> rest;
+    if (boxEmpty(rank, b)) return rest;
+    rest.push_back(b);
+    for (const Box &x : boxes) {
+        std::vector<Box> next;
+        for (const Box &r : rest) subtractBox(rank, r, x, next);
+        rest.swap(next);
+        if (rest.empty()) break;
+    }
+    return rest;
+}
+
+long Pieces::elements() const {
+    long n = 0;
+    for (const Box &x : boxes) {
+        long c = 1;
+        for (int a = 0; a < rank; a++) c *= x.hi[a] - x.lo[a] + 1;
+        n += c;
+    }
+    return n;
+}
+
+bool Pieces::holds(const long point[]) const {
+    for (const Box &x : boxes) {
+        int a = 0;
+        while (a < rank && point[a] >= x.lo[a] && point[a] <= x.hi[a]) a++;
+        if (a == rank) return true;
+    }
+    return false;
+}
+
+Box Data::whole() const {
+    Box w;
+    for (int a = 0; a < rank; a++) { w.lo[a] = space[a].lo; w.hi[a] = space[a].hi; }
+    return w;
+}
+
+void Data::hostOwnsAll() {
+    hostPieces.rank = devPieces.rank = rank;
+    hostPieces.clear();
+    hostPieces.add(whole());
+    devPieces.clear();
+}
+
+void Data::devOwnsAll() {
+    hostPieces.rank = devPieces.rank = rank;
+    devPieces.clear();
+    devPieces.add(whole());
+    hostPieces.clear();
+}
+
 long Data::elemCount() const {
     long n = 1;
     for (int i = 0; i < rank; i++) n *= space[i].size();
@@ -276,7 +398,8 @@ void Data::ensureDev() {
     for (int i = 0; i < rank; i++) dev.axisPerm[i] = i + 1;
     dev.diagState = 0;
     hasDev = true;
-    devActual = false;
+    devPieces.rank = rank;
+    devPieces.clear();
 }
 
 void Data::toDevice() {
@@ -287,7 +410,9 @@ void Data::toDevice() {
         dev.diagState = 0;
     }
     gpu.h2d(dev.ptr, host, bytes());
-    devActual = true;
+    devPieces.rank = rank;
+    devPieces.clear();
+    devPieces.add(whole());
 }
 
 // Sizes of storage positions of the current device layout.
@@ -374,7 +499,9 @@ void Data::toHost() {
         for (int i = 0; i < rank; i++) dev.axisPerm[i] = keepPerm[i];
         dev.diagState = keepDiag;
     }
-    hostActual = true;
+    hostPieces.rank = rank;
+    hostPieces.clear();
+    hostPieces.add(whole());
 }
 
 // DvmhBuffer::fillHeader (rts/dvmh/dvmh_buffer.cpp:149-182) for the device copy.

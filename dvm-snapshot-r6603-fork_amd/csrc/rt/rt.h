@@ -81,6 +81,7 @@ public:
     volatile unsigned long long *doneHost = nullptr;
     unsigned long long *doneDev = nullptr;
     unsigned long long doneSeq = 0;
+    long h2dBytes = 0, d2hBytes = 0;   // array and variable bytes moved by h2d / d2h
     void waitDone(unsigned long long seq);   // until the word reaches seq (or the stream drains)
     void init();
     void *alloc(size_t bytes);          // size-keyed cache in front of hipMalloc
@@ -128,6 +129,27 @@ struct DevBuffer {
     bool isDiagonalized() const { return diagState != 0; }
 };
 
+// A set of disjoint boxes of array indices (inclusive bounds per axis): where
+// a copy of an array is current (the role of the reference's DvmhPieces,
+// rts/dvmh/dvmh_pieces.cpp, used by its actual-state tracking,
+// rts/dvmh/dvmh_data.cpp:339-600).
+struct Box {
+    long lo[MAX_RANK], hi[MAX_RANK];
+};
+
+struct Pieces {
+    int rank = 0;
+    std::vector<Box> boxes;
+    bool empty() const { return boxes.empty(); }
+    void clear() { boxes.clear(); }
+    void add(const Box &b);                     // union with b
+    void remove(const Box &b);                  // difference with b
+    std::vector<Box> missing(const Box &b) const;   // parts of b not in the set, disjoint
+    bool covers(const Box &b) const { return missing(b).empty(); }
+    bool holds(const long point[]) const;       // the point lies in one of the boxes
+    long elements() const;
+};
+
 struct Data : Object {
     int rank = 0;
     long typeSize = 0;
@@ -140,7 +162,9 @@ struct Data : Object {
     DvmType *userHeader = nullptr;
     DevBuffer dev;
     bool hasDev = false;
-    bool hostActual = true, devActual = false;
+    // where each copy is current; every element is current in at least one
+    // (a device copy's pieces are in natural indices whatever its layout)
+    Pieces hostPieces, devPieces;
     // grouping of a closing batch: the last group (of batch number batchEpoch)
     // holding a loop whose region registers this array
     long batchEpoch = 0;
@@ -148,6 +172,11 @@ struct Data : Object {
     Data(): Object(Kind::Data) {}
     ~Data() override;
     long elemCount() const;
+    Box whole() const;
+    bool hostWhole() const { return hostPieces.covers(whole()); }
+    bool devWhole() const { return devPieces.covers(whole()); }
+    void hostOwnsAll();             // host copy current everywhere, device copy nowhere
+    void devOwnsAll();              // the reverse (a device write of the whole array)
     size_t bytes() const { return (size_t)elemCount() * (size_t)typeSize; }
     long hostStride(int axis /*1-based*/) const;
     void ensureDev();

@@ -77,6 +77,17 @@ DvmType dvmh_amd_layout_changes_C(void);
 /* Host threads the host handlers of a loop run on (DVMH_NUM_THREADS, else the
  * OpenMP default). */
 DvmType dvmh_amd_host_slots_C(void);
+/* Bytes of array and variable data moved host -> device (out[0]) and device ->
+ * host (out[1]) by actualisation since start-up or the last
+ * dvmh_amd_reset_stats_C. */
+void dvmh_amd_transfer_stats_C(DvmType out[2]);
+/* The box-set algebra of the actual-state tracking, for inspection: applies
+ * nops operations (records of 1 + 2 * rank values: 1 add / 2 remove, lo[rank],
+ * hi[rank], inclusive) to an empty set and writes the parts of the box query
+ * (lo[rank], hi[rank]) the set misses to outBoxes (at most maxBoxes records of
+ * 2 * rank values); returns how many there are. */
+DvmType dvmh_amd_pieces_eval_C(DvmType rank, const DvmType *ops, DvmType nops, const DvmType *query, DvmType *outBoxes,
+        DvmType maxBoxes);
 /* on != 0: every loop's handlers get rt_LONG from dvmh_loop_guess_index_type_C
  * and run their 64-bit index kernels whatever the array sizes (also
  * DVMH_AMD_INDEX64=1); 0 restores the choice by extent. */
