@@ -740,6 +740,7 @@ extern "C" DvmType dvmh_region_create_C(DvmType regionFlags) {   // :3002
     ensureInit();
     Region *r = new Region();
     r->flags = regionFlags;
+    r->async = settings.allowAsync && (regionFlags & REGION_ASYNC);   // rts/dvmh/region.cpp:442
     rt.curRegion = r;
     return (DvmType)r;
 }
@@ -773,6 +774,7 @@ extern "C" void dvmh_region_register_scalar_C(DvmType curRegion, DvmType intent,
     long sizes[1] = {1};
     Data *d = registerRegular(addr, 0, typeSize, sizes);
     d->rank = 0;
+    flushPostsFor(addr);   // a reduction result of an earlier async region lands first
     r->scalars.push_back(RegionArray{d, intent, varName ? varName : ""});
 }
 
@@ -821,7 +823,7 @@ extern "C" void dvmh_region_end_C(DvmType curRegion) {   // :3122
                 if (rs.intent & INTENT_OUT) gpu.d2h(rs.data->host, rs.data->dev.ptr, (size_t)rs.data->typeSize);
         }
         if (!r->deferred.empty()) graphRunRegion(r);
-        if (settings.syncMode == 1 && !(r->flags & REGION_ASYNC)) flushDeferredPosts();
+        if (settings.syncMode == 1 && !r->async) flushDeferredPosts();
     }
     if (!r->onGpu) {
         // a region run on the host wrote its output arrays there
@@ -1349,6 +1351,16 @@ extern "C" void dvmh_amd_get_stats_C(DvmType out[8]) {
 }
 
 extern "C" DvmType dvmh_amd_layout_changes_C(void) { return layoutChanges(); }
+
+extern "C" DvmType dvmh_amd_pending_posts_C(void) {
+    ensureInit();
+    return pendingPostCount();
+}
+
+extern "C" void dvmh_amd_set_allow_async_C(DvmType allow) {
+    ensureInit();
+    settings.allowAsync = allow != 0;
+}
 
 extern "C" void dvmh_amd_transfer_stats_C(DvmType out[2]) {
     ensureInit();

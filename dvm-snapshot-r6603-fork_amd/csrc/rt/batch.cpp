@@ -185,8 +185,10 @@ static void runQueue() {
         } else {
             runBatchGroup(g.loops);
         }
+        bool sync = false;
+        for (Loop *l : g.loops) sync = sync || syncAtLoopEnd(l->region);
         for (Loop *l : g.loops) releaseLoop(l);
-        if (settings.syncMode == 0) flushDeferredPosts();
+        if (sync) flushDeferredPosts();
     }
 }
 

@@ -56,6 +56,7 @@ void Settings::load() {
     tuneMode = (int)envLong("DVMH_AMD_TUNE", envLong("DVMH_OPT_PARAMS", 0) ? 1 : 0);
     DVMH_CHECK(tuneMode >= 0 && tuneMode <= 3, "DVMH_AMD_TUNE must be 0..3");
     syncMode = (int)envLong("DVMH_AMD_SYNC", 0);
+    allowAsync = envLong("DVMH_ALLOW_ASYNC", 0) != 0;   // rts/dvmh/settings.cpp:361
     pollDone = envLong("DVMH_AMD_POLL", 1) != 0;
     graph = envLong("DVMH_AMD_GRAPH", 0) != 0;
     tuneRepeats = (int)envLong("DVMH_AMD_TUNE_REPEATS", 1);

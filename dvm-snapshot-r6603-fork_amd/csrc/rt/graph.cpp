@@ -139,7 +139,7 @@ void graphRunRegion(Region *region) {
             }
         for (Loop *l : region->deferred) delete l;
         region->deferred.clear();
-        if (settings.syncMode == 0) flushDeferredPosts();
+        if (syncAtLoopEnd(region)) flushDeferredPosts();
         return;
     }
     if (g.noCapture || g.eagerStable < 1 || !tunerSettled(region)) {
@@ -173,7 +173,7 @@ void graphRunRegion(Region *region) {
     HIP_CHECK(hipGraphLaunch(g.exec, gpu.stream));
     (void)nLoops;
     // the captured loops' records were queued as pending posts by the finishes
-    if (settings.syncMode == 0) flushDeferredPosts();
+    if (syncAtLoopEnd(region)) flushDeferredPosts();
 }
 
 void graphReleaseAll() {

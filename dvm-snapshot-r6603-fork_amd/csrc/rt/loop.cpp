@@ -154,6 +154,16 @@ static char *mappedDevicePtr(char *h) {
 // synchronised.
 static unsigned long long pendingDone = 0;
 
+void flushPostsFor(const void *addr) {
+    for (auto &p : pending)
+        if (p.red->arrayAddr == addr || p.red->locAddr == addr) {
+            flushDeferredPosts();
+            return;
+        }
+}
+
+long pendingPostCount() { return (long)pending.size(); }
+
 void flushDeferredPosts() {
     if (pending.empty()) return;
     if (pendingDone && settings.pollDone) gpu.waitDone(pendingDone);
@@ -386,7 +396,7 @@ void performLoopNow(Loop *loop) {
         }
         for (void *p : sl.toFree) gpu.release(p);
         finishLoopReductions(loop);
-        if (settings.syncMode == 0 && !graphCapturing()) flushDeferredPosts();
+        if (syncAtLoopEnd(loop->region) && !graphCapturing()) flushDeferredPosts();
     } else {
         callHandler(loop->hostH, &ref);
         rt.loopsOnHost++;

@@ -50,6 +50,7 @@ struct Settings {
     bool requireGpu = false;    // DVMH_AMD_REQUIRE_GPU: abort instead of running a region on the host
     int tuneMode = 0;           // DVMH_AMD_TUNE: 0 default config, 1 exhaustive sweep, 2 cost model, 3 fixed (from DB)
     int syncMode = 0;           // DVMH_AMD_SYNC: 0 synchronise after every loop, 1 only when results are needed
+    bool allowAsync = false;    // DVMH_ALLOW_ASYNC: regions created with REGION_ASYNC synchronise only when results are needed
     bool graph = false;         // DVMH_AMD_GRAPH: capture repeating regions into HIP graphs
     int tuneRepeats = 1;        // DVMH_AMD_TUNE_REPEATS: timed executions per candidate
     bool pollDone = true;          // DVMH_AMD_POLL: wait for reduction results on the completion word, not a stream sync
@@ -204,6 +205,7 @@ struct Region : Object {
     long flags = 0;
     long targets = DEVICE_TYPE_HOST | DEVICE_TYPE_CUDA;
     bool onGpu = false;
+    bool async = false;                    // REGION_ASYNC with DVMH_ALLOW_ASYNC: no synchronisation at loop or region end
     std::vector<RegionArray> arrays;
     std::vector<RegionArray> scalars;
     std::vector<struct Loop *> deferred;   // GPU loops queued by a lazy (graph) region
@@ -328,6 +330,12 @@ void gpuRedPrepare(SpecLoop *sl, Reduction *r, long count, bool fill);
 void gpuRedFinish(SpecLoop *sl, const std::shared_ptr<Reduction> &r);
 void gpuRedFinishFlush(SpecLoop *sl);
 void flushDeferredPosts();
+void flushPostsFor(const void *addr);   // when a pending post targets addr
+long pendingPostCount();
+// Whether loops of region r synchronise when they end: by default yes, not in
+// DVMH_AMD_SYNC=1 and not in an async region (results merge at the next
+// get_actual / actual or at a region that registers their variables).
+inline bool syncAtLoopEnd(const Region *r) { return settings.syncMode == 0 && !(r && r->async); }
 
 // tuner
 struct TuneChoice {

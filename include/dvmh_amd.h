@@ -81,6 +81,13 @@ DvmType dvmh_amd_host_slots_C(void);
  * host (out[1]) by actualisation since start-up or the last
  * dvmh_amd_reset_stats_C. */
 void dvmh_amd_transfer_stats_C(DvmType out[2]);
+/* GPU reduction results computed but not yet merged into the program's
+ * variables (they merge at the next synchronisation point: the loop's end by
+ * default, get_actual / actual in an async region or with DVMH_AMD_SYNC=1). */
+DvmType dvmh_amd_pending_posts_C(void);
+/* Honour REGION_ASYNC (as DVMH_ALLOW_ASYNC does at start-up, rts/dvmh/settings.cpp:361):
+ * loops of regions created with it afterwards end without a synchronisation. */
+void dvmh_amd_set_allow_async_C(DvmType allow);
 /* The box-set algebra of the actual-state tracking, for inspection: applies
  * nops operations (records of 1 + 2 * rank values: 1 add / 2 remove, lo[rank],
  * hi[rank], inclusive) to an empty set and writes the parts of the box query
