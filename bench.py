@@ -243,10 +243,45 @@ def chosen_layouts(rt, w, chain=False):
     f = rt.dvmh_amd_tuner_query_chain_C if chain else rt.dvmh_amd_tuner_query_C
     f.argtypes = [ctypes.c_char_p, ctypes.c_long, ctypes.POINTER(ctypes.c_long)]
     f.restype = ctypes.c_long
+    ts = rt.dvmh_amd_tuner_transform_stats_C
+    ts.argtypes = [ctypes.c_char_p, ctypes.c_long, ctypes.POINTER(ctypes.c_long)]
+    ts.restype = ctypes.c_long
+    t = (ctypes.c_long * 4)()
     for line in w.lines:
         if f(w.cdv, line, q) > 0:
             out[str(line)] = {'block': [q[2], q[3], q[4]], 'variant': q[5 + 2], 'tile_a': q[5 + 4], 'tile_b': q[5 + 5],
                               'chunk': q[5 + 3], 'transform': q[5 + 6]}
+            if not chain and ts(w.cdv, line, t) > 0 and t[0] > 0:
+                # the site's layout-transform candidates: how many the tuner timed
+                # (the model's trial set) and the best of them against the best
+                # natural-layout one (transform round trip included)
+                out[str(line)]['transform_candidates'] = {'count': t[0], 'timed': t[1],
+                                                          'best_us': t[2] / 1e3 if t[2] >= 0 else None,
+                                                          'best_natural_us': t[3] / 1e3 if t[3] >= 0 else None}
+    return out
+
+
+def layout_transforms(reps=5):
+    """The layout-transform kernels (csrc/kernels/transform.hip) at the shapes
+    the ACROSS configurations would re-lay out: the ADI 256^3 array (exchange of
+    the two fastest axes: tiled transpose; axis rotation: row gather) and the
+    SOR 320^3 array (diagonal planes).  Each transform reads and writes the
+    array once: algorithmic bytes = 2 x 8 B x elements."""
+    rt = D.runtime()
+    f = rt.dvmh_amd_layout_transform_seconds_C
+    f.argtypes = [ctypes.c_long] * 5 + [ctypes.POINTER(ctypes.c_double)]
+    f.restype = ctypes.c_long
+    o = (ctypes.c_double * 2)()
+    out = {}
+    for key, which, n, desc in (('transpose_256', 0, 256, 'exchange of the two fastest axes, FP64 256^3 (transposeTiles)'),
+                                ('diagonal_320', 1, 320, 'natural <-> diagonal planes, FP64 320^3 (diagonalTiles)'),
+                                ('rotate_256', 2, 256, 'axis rotation, FP64 256^3 (permuteGather)')):
+        if f(which, n, n, n, reps, o) != 0:
+            continue
+        nbytes = 2.0 * 8 * n ** 3
+        out[key] = {'what': desc, 'forward_us': o[0] * 1e6, 'backward_us': o[1] * 1e6,
+                    'forward_GBps': nbytes / o[0] / 1e9, 'backward_GBps': nbytes / o[1] / 1e9,
+                    'frac_forward': nbytes / o[0] / HBM_PEAK, 'frac_backward': nbytes / o[1] / HBM_PEAK}
     return out
 
 
@@ -442,7 +477,9 @@ def main():
         log('batched', batched)
         others = other_configs(rt)
         log('others', others)
-        print(json.dumps({'batched_grids': batched, 'other_configs': others}), flush=True)
+        transforms = layout_transforms()
+        log('layout transforms', transforms)
+        print(json.dumps({'batched_grids': batched, 'other_configs': others, 'layout_transforms': transforms}), flush=True)
         return
     bench = Jac3d(rt)
     plain = exhaustive = None
@@ -457,12 +494,14 @@ def main():
     log('chosen', chosen)
     parity = None if (args.skip_parity or args.chosen_only) else bench.parity(2)
     log('parity', parity)
-    batched = others = None
+    batched = others = transforms = None
     if not (args.skip_others or args.chosen_only):
         batched = batched_leg(rt, 20)
         log('batched', batched)
         others = other_configs(rt)
         log('others', others)
+        transforms = layout_transforms()
+        log('layout transforms', transforms)
     cpu = None if (args.skip_cpu or args.chosen_only) else cpu_baseline()
     log('cpu', cpu)
     kernel_s = chosen['kernel_us'] * 1e-6
@@ -499,6 +538,7 @@ def main():
         'parity': parity,
         'batched_grids': batched,
         'other_configs': others,
+        'layout_transforms': transforms,
         'cpu_baseline': cpu,
     }
     print(json.dumps(out), flush=True)

@@ -143,9 +143,13 @@ void graphRunRegion(Region *region) {
         return;
     }
     if (g.noCapture || g.eagerStable < 1 || !tunerSettled(region)) {
+        // layout changes while the tuner still explores (transform candidates)
+        // say nothing about the settled region
+        const bool settled = tunerSettled(region);
         long before = layoutChanges();
         runEager(region);
-        if (layoutChanges() != before) g.noCapture = true;
+        if (!settled) g.eagerStable = 0;
+        else if (layoutChanges() != before) g.noCapture = true;
         else g.eagerStable++;
         return;
     }

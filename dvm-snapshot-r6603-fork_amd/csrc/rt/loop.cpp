@@ -327,6 +327,14 @@ static void prepareLayouts(Loop *loop) {
     }
 }
 
+static void restoreNaturalLayouts(Loop *loop) {
+    if (!loop->region) return;
+    for (auto &ra : loop->region->arrays) {
+        Data *d = ra.data;
+        if (d->hasDev && d->dev.isTransformed(d->rank)) d->setLayout(nullptr, 0);
+    }
+}
+
 // ------------------------------------------------------------ perform
 
 // After the handler: allreduce is trivial for one process; then the
@@ -386,6 +394,10 @@ void performLoopNow(Loop *loop) {
         // executions the tuner explores are timed from its pick to here; settled
         // sites run without events or synchronisation
         if (sl.explore && sl.tuneStart && !graphCapturing()) {
+            // a candidate that re-lays out arrays is charged the round trip: the
+            // arrays return to the natural layout before the end mark, as the
+            // next loop that needs that layout would do
+            if (sl.tune[DVMH_AMD_T_TRANSFORM]) restoreNaturalLayouts(loop);
             hipEvent_t ev1 = tunerEvent(1);
             HIP_CHECK(hipEventRecord(ev1, sl.stream));
             HIP_CHECK(hipEventSynchronize(ev1));

@@ -54,7 +54,7 @@ struct Settings {
     bool graph = false;         // DVMH_AMD_GRAPH: capture repeating regions into HIP graphs
     int tuneRepeats = 1;        // DVMH_AMD_TUNE_REPEATS: timed executions per candidate
     bool pollDone = true;          // DVMH_AMD_POLL: wait for reduction results on the completion word, not a stream sync
-    bool tuneTransforms = false;   // DVMH_AMD_TUNE_TRANSFORMS: candidates may re-lay out arrays of ACROSS loops
+    bool tuneTransforms = true;    // DVMH_AMD_TUNE_TRANSFORMS (default 1): candidates may re-lay out arrays of ACROSS loops
     bool index64 = false;          // DVMH_AMD_INDEX64: every loop's kernels take 64-bit index arithmetic
     std::string tuneDb;         // DVMH_AMD_TUNEDB: file of recorded timings
     std::string modelFile;      // DVMH_AMD_MODEL: trained cost-model weights

@@ -150,3 +150,54 @@ static double streamSeconds(long bytes, long reps, bool copy) {
 
 extern "C" double dvmh_amd_stream_copy_seconds_C(DvmType bytes, DvmType reps) { return streamSeconds(bytes, reps, true); }
 extern "C" double dvmh_amd_stream_read_seconds_C(DvmType bytes, DvmType reps) { return streamSeconds(bytes, reps, false); }
+
+// The layout-transform kernels on an FP64 array of n0 x n1 x n2 elements,
+// timed with HIP events on the runtime's stream (`reps` round trips after one
+// untimed round trip): which 0 exchanges the two fastest axes (the tiled
+// transpose), 1 goes to the diagonal layout of the (n1, n2) planes, 2 rotates
+// the axes to n2 x n0 x n1 (the general gather).  out[0] / out[1] receive the
+// seconds of one forward / backward transform; returns 0, or -1 without a GPU.
+extern "C" DvmType dvmh_amd_layout_transform_seconds_C(DvmType which, DvmType n0, DvmType n1, DvmType n2, DvmType reps,
+        double out[2]) {
+    runtimeInit();
+    if (!gpu.present || which < 0 || which > 2 || n0 <= 0 || n1 <= 0 || n2 <= 0 || reps <= 0) return -1;
+    const size_t bytes = (size_t)n0 * n1 * n2 * sizeof(double);
+    void *a = nullptr, *b = nullptr;
+    HIP_CHECK(hipMalloc(&a, bytes));
+    HIP_CHECK(hipMalloc(&b, bytes));
+    HIP_CHECK(hipMemsetAsync(a, 1, bytes, gpu.stream));
+    const long fwdSizes[3] = {n0, n1, n2};
+    const long swapBack[3] = {n0, n2, n1}, rotBack[3] = {n2, n0, n1};
+    const int swapPerm[3] = {0, 2, 1}, rotPerm[3] = {2, 0, 1}, rotInv[3] = {1, 2, 0};
+    auto step = [&](bool back) {
+        const void *src = back ? b : a;
+        void *dst = back ? a : b;
+        if (which == 0) dvmh_hip_permute(src, dst, 8, 3, back ? swapBack : fwdSizes, swapPerm, gpu.stream);
+        else if (which == 1) dvmh_hip_diagonal(src, dst, 8, n2, n1, n0, back ? 1 : 0, 0, gpu.stream);
+        else dvmh_hip_permute(src, dst, 8, 3, back ? rotBack : fwdSizes, back ? rotInv : rotPerm, gpu.stream);
+    };
+    step(false);
+    step(true);
+    hipEvent_t ev[3];
+    for (auto &e : ev) HIP_CHECK(hipEventCreate(&e));
+    float fwd = 0, bwd = 0;
+    for (long r = 0; r < reps; r++) {
+        float ms0 = 0, ms1 = 0;
+        HIP_CHECK(hipEventRecord(ev[0], gpu.stream));
+        step(false);
+        HIP_CHECK(hipEventRecord(ev[1], gpu.stream));
+        step(true);
+        HIP_CHECK(hipEventRecord(ev[2], gpu.stream));
+        HIP_CHECK(hipEventSynchronize(ev[2]));
+        HIP_CHECK(hipEventElapsedTime(&ms0, ev[0], ev[1]));
+        HIP_CHECK(hipEventElapsedTime(&ms1, ev[1], ev[2]));
+        fwd += ms0;
+        bwd += ms1;
+    }
+    for (auto &e : ev) HIP_CHECK(hipEventDestroy(e));
+    HIP_CHECK(hipFree(a));
+    HIP_CHECK(hipFree(b));
+    out[0] = fwd * 1e-3 / (double)reps;
+    out[1] = bwd * 1e-3 / (double)reps;
+    return 0;
+}

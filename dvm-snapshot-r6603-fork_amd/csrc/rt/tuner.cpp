@@ -534,6 +534,34 @@ extern "C" DvmType dvmh_amd_tuner_query_C(const char *file, DvmType line, DvmTyp
     return tunerQuery(file, line, -1, out);
 }
 
+// How the layout-transform candidates of a loop's sites fared (summed over its
+// sites other than a chain's): out[0] transform candidates, out[1] of them
+// timed, out[2] best timed transform candidate in ns (-1 if none), out[3] best
+// timed natural-layout candidate in ns (-1 if none).  Returns the sites seen.
+extern "C" DvmType dvmh_amd_tuner_transform_stats_C(const char *file, DvmType line, DvmType out[4]) {
+    runtimeInit();
+    std::string prefix = std::string(file) + ":" + std::to_string(line) + "|";
+    out[0] = out[1] = 0;
+    out[2] = out[3] = -1;
+    long seen = 0;
+    for (auto &kv : sites) {
+        if (kv.first.compare(0, prefix.size(), prefix) != 0 || kv.second.kind == DVMH_AMD_KIND_CHAIN) continue;
+        seen++;
+        const SiteInfo &si = kv.second;
+        for (size_t c = 0; c < si.cands.size(); c++) {
+            const bool tf = si.cands[c].tune[DVMH_AMD_T_TRANSFORM] != 0;
+            if (tf) out[0]++;
+            const double t = si.measured((int)c);
+            if (t < 0) continue;
+            if (tf) out[1]++;
+            DvmType &best = out[tf ? 2 : 3];
+            const DvmType ns = (DvmType)(t * 1e9);
+            if (best < 0 || ns < best) best = ns;
+        }
+    }
+    return seen;
+}
+
 extern "C" DvmType dvmh_amd_tuner_query_chain_C(const char *file, DvmType line, DvmType out[13]) {
     runtimeInit();
     return tunerQuery(file, line, DVMH_AMD_KIND_CHAIN, out);

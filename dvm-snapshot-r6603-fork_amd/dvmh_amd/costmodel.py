@@ -9,6 +9,7 @@
           the default configuration and k random candidates.
 
     python -m dvmh_amd.costmodel sweep --out records.tsv
+    python -m dvmh_amd.costmodel sweep --merge --spec adi3d:64,128 --out records.tsv
     python -m dvmh_amd.costmodel train --records records.tsv --out costmodel.bin --holdout '|e382x382x382'
     python -m dvmh_amd.costmodel eval --records records.tsv --model costmodel.bin --site 'jac3d.cdv:41|k3|r3|m0|e382x382x382'
 """
@@ -113,13 +114,32 @@ def sweep_workload(name, size, repeats=2, iters_per_run=20, max_runs=400, log=No
         log(f'swept {name} {size}: {runs} runs, {time.perf_counter() - t0:.1f} s')
 
 
-def sweep(specs=DEFAULT_SPECS, out=RECORDS, repeats=2, log=None):
+def sweep(specs=DEFAULT_SPECS, out=RECORDS, repeats=2, log=None, merge=False):
+    """Sweeps the workloads from a clean tuner and saves the records to `out`.
+    With `merge`, `out` keeps its records of every other program: those of the
+    swept workloads' programs are replaced by the new sweep."""
     lib = rt()
+    lib.dvmh_amd_tuner_clear_C()
+    programs = set()
     for name, sizes in specs:
         for size in sizes:
             sweep_workload(name, size, repeats=repeats, log=log)
+            programs.add(workloads.ALL[name].cdv.decode() + ':')
     os.makedirs(os.path.dirname(out) or '.', exist_ok=True)
-    return lib.dvmh_amd_tuner_save_C(out.encode())
+    kept = []
+    if merge and os.path.exists(out):
+        with open(out) as f:
+            kept = [ln for ln in f if not any(ln.startswith(p) for p in programs)]
+    tmp = out + '.new'
+    n = lib.dvmh_amd_tuner_save_C(tmp.encode())
+    with open(tmp) as f:
+        fresh = f.readlines()
+    with open(out, 'w') as f:
+        f.writelines(kept + fresh)
+    os.remove(tmp)
+    if log and merge:
+        log(f'kept {len(kept)} records of other programs')
+    return len(kept) + n
 
 
 def load_records(path=RECORDS, clear=True):
@@ -212,6 +232,7 @@ def main(argv=None):
     s.add_argument('--out', default=RECORDS)
     s.add_argument('--repeats', type=int, default=2)
     s.add_argument('--spec', action='append', help='name:size,size,...  (default: the training sweep)')
+    s.add_argument('--merge', action='store_true', help='replace only the swept programs\' records in --out')
     t = sub.add_parser('train')
     t.add_argument('--records', default=RECORDS)
     t.add_argument('--out', default=MODEL)
@@ -233,7 +254,7 @@ def main(argv=None):
         specs = DEFAULT_SPECS
         if a.spec:
             specs = [(x.split(':')[0], [int(v) for v in x.split(':')[1].split(',')]) for x in a.spec]
-        n = sweep(specs, a.out, a.repeats, log)
+        n = sweep(specs, a.out, a.repeats, log, merge=a.merge)
         log(f'{n} records -> {a.out}')
     elif a.cmd == 'train':
         load_records(a.records)

@@ -106,6 +106,12 @@ void dvmh_amd_set_index64_C(DvmType on);
  * known-traffic dispatches that scale HBM counters.  -1 without a GPU. */
 double dvmh_amd_stream_copy_seconds_C(DvmType bytes, DvmType reps);
 double dvmh_amd_stream_read_seconds_C(DvmType bytes, DvmType reps);
+/* The layout-transform kernels timed on an FP64 n0 x n1 x n2 array (buffers
+ * allocated for the measurement): which 0 swaps the two fastest axes (tiled
+ * transpose), 1 converts the (n1, n2) planes to the diagonal layout, 2 rotates
+ * the axes (general gather).  out[0] / out[1]: seconds of one forward /
+ * backward transform, averaged over `reps` round trips; returns -1 without a GPU. */
+DvmType dvmh_amd_layout_transform_seconds_C(DvmType which, DvmType n0, DvmType n1, DvmType n2, DvmType reps, double out[2]);
 /* Folds n >= 1 partial values of one reduction, in device memory (vals; for
  * rf_MAXLOC / rf_MINLOC also locs, n records of locSize bytes), into one value
  * and its loc record written to outVal / outLoc (device or mapped memory), on
@@ -208,6 +214,11 @@ DvmType dvmh_amd_tuner_site_pending_C(const char *file, DvmType line);
 DvmType dvmh_amd_tuner_query_C(const char *file, DvmType line, DvmType out[13]);
 /* The same for the chain site of that loop (DVMH_AMD_KIND_CHAIN). */
 DvmType dvmh_amd_tuner_query_chain_C(const char *file, DvmType line, DvmType out[13]);
+/* How the layout-transform candidates of the loop's sites fared: out[0]
+ * transform candidates, out[1] of them timed, out[2] best timed one in ns and
+ * out[3] best timed natural-layout candidate in ns (-1 where none was timed);
+ * returns the number of sites. */
+DvmType dvmh_amd_tuner_transform_stats_C(const char *file, DvmType line, DvmType out[4]);
 /* Forces the launch configuration (block[3], tune[DVMH_AMD_TUNE_N]) of every
  * loop of the given kind, over whatever the mode would choose; forced
  * executions are not timed.  block == NULL removes the forcing for the kind. */
