@@ -165,26 +165,36 @@ def batched_leg(rt, iters):
     setup(count, None, D.SEED_BASE + 5)
     sides = np.array([side(g) for g in range(count)])
     pts = int(np.sum((sides - 2) ** 2))
-    for name, batched, sync in (('per_grid', 0, 0), ('batched', 1, 1)):
-        rt.dvmh_amd_set_sync_mode_C(sync)
+    for name, batched, sync, graph in (('per_grid', 0, 0, 0), ('batched', 1, 1, 1)):
+        set_form(rt, 0, sync, graph)
+        # (batched: the first iteration runs eagerly, the second is captured as
+        # one graph, every later one replays it)
         it(2, batched, None)
         # wall time without events (the per-grid form would pay for thousands of
-        # event pairs per iteration), then the kernel time on a second run
+        # event pairs per iteration), then the kernel time on a second run,
+        # issued eagerly (graph replays carry no timing marks)
+        replays0 = D.stats()['graph_replays']
         sec = it(iters, batched, None)
+        replays = D.stats()['graph_replays'] - replays0
+        rt.dvmh_amd_set_graph_C(0)
         rt.dvmh_amd_reset_kernel_stats_C()
         rt.dvmh_amd_set_kernel_timing_C(1)
         it(iters, batched, None)
         rt.dvmh_amd_set_kernel_timing_C(0)
         launches, ns = kernel_stats(rt, b'batched.cdv', (31, 36))
-        res[name] = {'value': pts * iters / sec, 'ms_per_iteration': sec / iters * 1e3, 'kernel_launches': launches,
+        res[name] = {'value': pts * iters / sec, 'ms_per_iteration': sec / iters * 1e3, 'graph_replays': replays,
+                     'kernel_launches': launches,
                      'kernel_ms_per_iteration': ns * 1e-6 / iters,
                      'achieved_GBps': pts * 40 * iters / (ns * 1e-9) / 1e9 if ns else None}
     P.fn('teardown', [])()
-    # parity: a fresh batched run of 5 iterations, smallest and largest grid
-    set_form(rt, 0, 1)
+    # parity: a fresh batched run of 5 iterations in the timed form (eager,
+    # captured, then 3 replays), smallest and largest grid
+    set_form(rt, 0, 1, 1)
     setup(count, None, D.SEED_BASE + 5)
     eps = np.zeros(5 * count)
+    replays0 = D.stats()['graph_replays']
     it(5, 1, eps.ctypes.data_as(_dp))
+    parity_replays = D.stats()['graph_replays'] - replays0
     eps = eps.reshape(5, count)
     picks = [int(np.argmin(sides)), int(np.argmax(sides))]
     ok = True
@@ -195,8 +205,10 @@ def batched_leg(rt, iters):
         fetch(r['g'], A.ctypes.data_as(_dp), Bb.ctypes.data_as(_dp))
         ok = ok and H.bits_equal(A, r['A']) and H.bits_equal(Bb, r['B']) and np.array_equal(eps[:, r['g']], r['eps'])
     P.fn('teardown', [])()
-    rt.dvmh_amd_set_sync_mode_C(0)
+    set_form(rt, 0, 0, 0)
     return {'workload': '512 uneven 2-D Jacobi grids (sides 64..1024 by seed), FP64, 5-point + REDUCTION(MAX) per grid',
+            'form': 'batched: one persistent kernel per loop site per iteration, the time step replayed as one graph',
+            'parity_graph_replays': parity_replays,
             'points_per_iteration': pts, 'iterations': iters, 'bytes_per_point_update': 40,
             'per_grid': res['per_grid'], 'batched': res['batched'],
             'speedup': res['batched']['value'] / res['per_grid']['value'], 'parity_bit_exact': bool(ok)}

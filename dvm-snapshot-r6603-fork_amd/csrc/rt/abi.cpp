@@ -312,7 +312,9 @@ extern "C" void dvmh_realign_C(DvmType dvmDesc[], DvmType, const DvmType templDe
 }
 
 static void deleteData(Data *d) {
-    if (d->hasDev) gpu.sync();
+    // captured graphs may hold its device address (or reuse it for another
+    // object later): they are dropped with it
+    if (d->hasDev) graphReleaseAll();
     alignments.erase(d);
     rt.allData.erase(std::remove(rt.allData.begin(), rt.allData.end(), d), rt.allData.end());
     for (auto it = rt.regularVars.begin(); it != rt.regularVars.end();)

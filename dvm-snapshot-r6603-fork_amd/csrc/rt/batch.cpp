@@ -146,12 +146,31 @@ static void releaseLoop(Loop *l) {
 }
 
 long tunerSitePendingExcept(const std::string &file, long line, int kind);
+long tunerSitePending(const std::string &file, long line);
+
+// A queue may be captured as a graph when graphs are on and no tuner site of
+// its loops is still being explored (every execution of an explored site is
+// timed, which a replay cannot do).
+static bool queueSettled(const std::vector<Loop *> &loops) {
+    if (settings.tuneMode == 0 || settings.tuneMode == 3) return true;
+    std::vector<std::pair<const std::string *, long>> seen;
+    for (Loop *l : loops) {
+        bool known = false;
+        for (auto &s : seen)
+            if (s.second == l->line && *s.first == l->file) { known = true; break; }
+        if (known) continue;
+        if (tunerSitePending(l->file, l->line) > 0) return false;
+        seen.push_back({&l->file, l->line});
+    }
+    return true;
+}
 
 static void runQueue() {
     struct Group {
         DvmHandlerFunc func;
         bool chain;
         std::vector<Loop *> loops;
+        bool explore = false;
     };
     std::vector<Group> groups;
     batchEpoch++;
@@ -168,28 +187,40 @@ static void runQueue() {
             if (target >= 0 && lastGroupOf(l) >= target) target = -1;
         }
         if (target < 0) {
-            groups.push_back(Group{l->batchH.func, l->batchChain, {}});
+            groups.push_back(Group{l->batchH.func, l->batchChain, {}, false});
             target = (int)groups.size() - 1;
         }
         groups[target].loops.push_back(l);
         markGroup(l, target);
     }
-    queued.clear();
-    for (auto &g : groups) {
-        const bool exploring = g.chain && tunerSitePendingExcept(g.loops[0]->file, g.loops[0]->line, DVMH_AMD_KIND_CHAIN) > 0;
-        if (g.loops.size() == 1 || exploring) {
-            for (Loop *l : g.loops) {
-                l->queued = false;
-                performLoopNow(l);
+    std::vector<Loop *> all;
+    all.swap(queued);
+    bool settled = settings.graph && queueSettled(all);
+    for (auto &g : groups)
+        g.explore = g.chain && tunerSitePendingExcept(g.loops[0]->file, g.loops[0]->line, DVMH_AMD_KIND_CHAIN) > 0;
+    auto runGroups = [&]() {
+        for (auto &g : groups) {
+            if (g.loops.size() == 1 || g.explore) {
+                for (Loop *l : g.loops) {
+                    l->queued = false;
+                    performLoopNow(l);
+                }
+            } else {
+                runBatchGroup(g.loops);
             }
-        } else {
-            runBatchGroup(g.loops);
+            bool sync = false;
+            for (Loop *l : g.loops) sync = sync || syncAtLoopEnd(l->region);
+            for (Loop *l : g.loops) releaseLoop(l);
+            if (sync && !graphCapturing()) flushDeferredPosts();
         }
-        bool sync = false;
-        for (Loop *l : g.loops) sync = sync || syncAtLoopEnd(l->region);
-        for (Loop *l : g.loops) releaseLoop(l);
-        if (sync) flushDeferredPosts();
-    }
+    };
+    bool sync = false;
+    for (Loop *l : all) sync = sync || syncAtLoopEnd(l->region);
+    if (batchGraphRun(all, settled, runGroups))
+        for (Loop *l : all) releaseLoop(l);
+    // records of a replay or of a captured queue's first launch (an eager run
+    // has posted its own group by group)
+    if (sync) flushDeferredPosts();
 }
 
 // Runs what is queued so far (the batch stays open); see the file comment.
@@ -280,7 +311,8 @@ std::map<std::pair<const void *, long>, Persisted> persisted;
 
 static void uploadAsync(void *dst, const void *src, size_t bytes) {
     if (bytes == 0) return;
-    char *stage = uploadRing.take(bytes);
+    // a captured copy reads a pinned copy the graph owns (the ring is reused)
+    char *stage = graphCapturing() ? graphPinnedSlot(bytes) : uploadRing.take(bytes);
     memcpy(stage, src, bytes);
     HIP_CHECK(hipMemcpyAsync(dst, stage, bytes, hipMemcpyHostToDevice, gpu.stream));
 }
@@ -293,8 +325,16 @@ extern "C" void dvmh_amd_batch_upload_C(DvmType batchRef, void *dst, const void 
 extern "C" void *dvmh_amd_batch_persist_C(DvmType batchRef, const void *tag, DvmType slot, const void *src, DvmType bytes) {
     asBatch(batchRef, "dvmh_amd_batch_persist_C");
     DVMH_CHECK(bytes >= 0, "dvmh_amd_batch_persist_C: %ld bytes", (long)bytes);
-    Persisted &p = persisted[std::make_pair(tag, (long)slot)];
     const size_t n = (size_t)bytes;
+    if (graphCapturing()) {
+        // a graph keeps its own copy: later executions of other queues may
+        // change the persisted one while the graph is still replayed
+        void *d = gpu.alloc(std::max(n, (size_t)64));
+        graphNoteAlloc(d);
+        uploadAsync(d, src, n);
+        return d;
+    }
+    Persisted &p = persisted[std::make_pair(tag, (long)slot)];
     if (p.dev && p.last.size() == n && (n == 0 || memcmp(p.last.data(), src, n) == 0)) return p.dev;
     if (!p.dev || p.cap < n) {
         if (p.dev) gpu.release(p.dev);   // stream-ordered: earlier launches reading it come first
@@ -327,12 +367,13 @@ extern "C" void dvmh_amd_batch_red_finish_C(DvmType batchRef, DvmType redIndex, 
     void *dLoc = ls ? (char *)dRes + ((nl * es + 15) & ~15L) : nullptr;
     dvmh_hip_reduce_segments(r0->func, r0->rtType, nl, (const long *)dOff, partials, locPartials, ls, dRes, dLoc, gpu.stream);
     size_t total = (size_t)(((nl * es + 15) & ~15L) + nl * ls + 64);
-    char *host = pinnedTake(total);
+    char *host = graphCapturing() ? graphPinnedSlot(total) : pinnedTake(total);
     HIP_CHECK(hipMemcpyAsync(host, dRes, total - 64, hipMemcpyDeviceToHost, gpu.stream));
     for (long i = 0; i < nl; i++) {
         auto &r = b->loops[i]->reds[redIndex - 1];
         char *val = host + i * es;
         char *loc = ls ? host + ((nl * es + 15) & ~15L) + i * ls : nullptr;
+        if (graphCapturing()) graphNotePost(r, val, loc);
         addPendingPost(r, val, loc, false);
     }
 }

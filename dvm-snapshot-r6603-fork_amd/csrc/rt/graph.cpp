@@ -16,6 +16,7 @@
 #include <cstring>
 #include <functional>
 #include <sstream>
+#include <unordered_map>
 
 #include "rt.h"
 
@@ -180,16 +181,191 @@ void graphRunRegion(Region *region) {
     if (syncAtLoopEnd(region)) flushDeferredPosts();
 }
 
+// ------------------------------------------------------------ batch queues
+// The loops of a batch (dvmh_amd_batch_begin_C .. _end_C) form one time step
+// of a program that repeats it: when DVMH_AMD_GRAPH is on, every tuner site
+// involved is settled and the queue equals an earlier one word for word (its
+// signature below), the whole queue (all fused groups, lone loops, reduction
+// finishes and their D2H copies) is captured once and then replayed with one
+// graph launch.  Device memory the captured handlers upload to (descriptor
+// tables) is allocated for the graph and filled from pinned copies the graph
+// owns, so graphs of different queues never share mutable buffers.
+
+struct BatchGraph {
+    std::vector<long> sig;
+    GraphEntry g;
+    std::vector<std::pair<long, long>> postAt;   // (queue position, reduction index) of each captured post
+};
+static std::unordered_map<size_t, BatchGraph> batchGraphs;
+constexpr size_t MAX_BATCH_GRAPHS = 16;
+
+static void destroyEntry(GraphEntry &g) {
+    if (g.exec) HIP_CHECK(hipGraphExecDestroy(g.exec));
+    if (g.graph) HIP_CHECK(hipGraphDestroy(g.graph));
+    for (char *p : g.pinned) HIP_CHECK(hipHostFree(p));
+    for (void *p : g.devBlocks) gpu.release(p);
+    g = GraphEntry();
+}
+
+// Everything a captured queue depends on, as words: per loop its site, bounds,
+// handlers and parameter addresses, the arrays its region registers (object,
+// device address, layout), reductions (variables, initial bytes) and ACROSS
+// widths.  Scalars are read by the kernels from device memory, so only their
+// addresses enter.
+static std::vector<long> batchSignature(const std::vector<Loop *> &loops) {
+    std::vector<long> w;
+    w.reserve(loops.size() * 32);
+    auto bytes = [&w](const std::vector<char> &v) {
+        w.push_back((long)v.size());
+        for (size_t i = 0; i < v.size(); i += sizeof(long)) {
+            long x = 0;
+            memcpy(&x, v.data() + i, std::min(sizeof(long), v.size() - i));
+            w.push_back(x);
+        }
+    };
+    for (Loop *l : loops) {
+        w.push_back((long)std::hash<std::string>()(l->file));
+        w.push_back(l->line);
+        w.push_back(l->rank);
+        for (int i = 0; i < l->rank; i++) {
+            w.push_back(l->lo[i]);
+            w.push_back(l->hi[i]);
+            w.push_back(l->step[i]);
+        }
+        w.push_back((long)l->batchH.func);
+        w.push_back((long)l->gpuH.func);
+        w.push_back(l->batchChain);
+        for (void *p : l->gpuH.params) w.push_back((long)p);
+        if (l->region) {
+            for (auto &ra : l->region->arrays) {
+                const Data *d = ra.data;
+                w.push_back((long)d);
+                w.push_back((long)d->dev.ptr);
+                w.push_back(d->dev.diagState);
+                for (int i = 0; i < d->rank; i++) w.push_back(d->dev.axisPerm[i]);
+            }
+            for (auto &rs : l->region->scalars) {
+                w.push_back((long)rs.data);
+                w.push_back((long)rs.data->dev.ptr);
+            }
+        }
+        for (auto &r : l->reds) {
+            w.push_back(r->func);
+            w.push_back(r->rtType);
+            w.push_back((long)r->arrayAddr);
+            w.push_back((long)r->locAddr);
+            bytes(r->initArr);
+            bytes(r->initLoc);
+        }
+        for (auto &a : l->acrosses) {
+            w.push_back((long)a.data);
+            for (int i = 0; i < a.data->rank; i++) {
+                w.push_back(a.low[i]);
+                w.push_back(a.high[i]);
+            }
+        }
+        w.push_back(l->userBlock[0]);
+        w.push_back(l->userBlock[1]);
+        w.push_back(l->userBlock[2]);
+        w.push_back(-1);
+    }
+    return w;
+}
+
+static size_t hashWords(const std::vector<long> &w) {
+    unsigned long long h = 1469598103934665603ULL;   // FNV-1a over the words
+    for (long x : w) {
+        h ^= (unsigned long long)x;
+        h *= 1099511628211ULL;
+    }
+    return (size_t)h;
+}
+
+bool batchGraphRun(const std::vector<Loop *> &loops, bool eligible, const std::function<void()> &runGroups) {
+    if (!eligible || loops.empty()) {
+        runGroups();
+        return false;
+    }
+    std::vector<long> sig = batchSignature(loops);
+    const size_t h = hashWords(sig);
+    auto it = batchGraphs.find(h);
+    if (it != batchGraphs.end() && it->second.sig != sig) {
+        gpu.sync();
+        destroyEntry(it->second.g);
+        batchGraphs.erase(it);
+        it = batchGraphs.end();
+    }
+    if (it == batchGraphs.end()) {
+        if (batchGraphs.size() >= MAX_BATCH_GRAPHS) {
+            gpu.sync();
+            for (auto &kv : batchGraphs) destroyEntry(kv.second.g);
+            batchGraphs.clear();
+        }
+        it = batchGraphs.emplace(h, BatchGraph()).first;
+        it->second.sig = std::move(sig);
+    }
+    BatchGraph &bg = it->second;
+    GraphEntry &g = bg.g;
+    if (g.exec) {
+        HIP_CHECK(hipGraphLaunch(g.exec, gpu.stream));
+        graphReplays++;
+        rt.loopsOnGpu += (long)loops.size();
+        // each variable is reset to its identity as perform would do, the
+        // graph's records are posted in capture order, then the initial
+        // values are folded in
+        for (Loop *l : loops)
+            for (auto &r : l->reds) r->initValues(r->arrayAddr, r->locAddr);
+        for (size_t k = 0; k < g.posts.size(); k++) {
+            auto &r = loops[bg.postAt[k].first]->reds[bg.postAt[k].second];
+            addPendingPost(r, g.posts[k].val, g.posts[k].loc, false);
+        }
+        for (Loop *l : loops) finishLoopReductionsExt(l);
+        return true;
+    }
+    if (g.noCapture || g.eagerStable < 1) {
+        long before = layoutChanges();
+        runGroups();
+        if (layoutChanges() != before) g.noCapture = true;
+        else g.eagerStable++;
+        return false;
+    }
+    // capture this occurrence (the queue's loops map their reductions to
+    // positions first: the loops are released group by group as they run)
+    std::unordered_map<const Reduction *, std::pair<long, long>> where;
+    for (size_t i = 0; i < loops.size(); i++)
+        for (size_t k = 0; k < loops[i]->reds.size(); k++) where[loops[i]->reds[k].get()] = {(long)i, (long)k};
+    flushDeferredPosts();
+    capturing = true;
+    captureBlocks = &g.devBlocks;
+    capturePosts = &g.posts;
+    capturePinned = &g.pinned;
+    int savedSync = settings.syncMode;
+    settings.syncMode = 1;
+    long before = layoutChanges();
+    HIP_CHECK(hipStreamBeginCapture(gpu.stream, hipStreamCaptureModeRelaxed));
+    runGroups();
+    HIP_CHECK(hipStreamEndCapture(gpu.stream, &g.graph));
+    capturing = false;
+    settings.syncMode = savedSync;
+    DVMH_INTERNAL(layoutChanges() == before, "array layout changed inside a captured batch");
+    HIP_CHECK(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
+    HIP_CHECK(hipGraphLaunch(g.exec, gpu.stream));
+    for (auto &p : g.posts) {
+        auto w = where.find(p.red.get());
+        DVMH_INTERNAL(w != where.end(), "a captured reduction record belongs to no queued loop");
+        bg.postAt.push_back(w->second);
+        p.red.reset();   // the replays post to the loops of their own queue
+    }
+    // the captured run's records were queued as pending posts by the finishes
+    return false;
+}
+
 void graphReleaseAll() {
     gpu.sync();
-    for (auto &kv : graphs) {
-        GraphEntry &g = kv.second;
-        if (g.exec) HIP_CHECK(hipGraphExecDestroy(g.exec));
-        if (g.graph) HIP_CHECK(hipGraphDestroy(g.graph));
-        for (char *p : g.pinned) HIP_CHECK(hipHostFree(p));
-        for (void *p : g.devBlocks) gpu.release(p);
-    }
+    for (auto &kv : graphs) destroyEntry(kv.second);
     graphs.clear();
+    for (auto &kv : batchGraphs) destroyEntry(kv.second.g);
+    batchGraphs.clear();
 }
 
 }  // namespace dvmh

@@ -18,6 +18,7 @@
 #include <atomic>
 #include <cstdint>
 #include <cstdio>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -367,6 +368,10 @@ void graphNoteAlloc(void *p);
 char *graphPinnedSlot(size_t bytes);
 void graphNotePost(const std::shared_ptr<Reduction> &r, char *val, char *loc);
 void graphRunRegion(Region *region);
+// Runs a batch queue through runGroups, or replays the graph captured from an
+// identical earlier queue (returns true then); see graph.cpp.
+bool batchGraphRun(const std::vector<Loop *> &loops, bool eligible, const std::function<void()> &runGroups);
+void finishLoopReductionsExt(Loop *loop);
 void graphReleaseAll();
 void runtimeInit();   // settings, GPU and tuner set-up on first use
 void noallocInit(int device);   // device address of the no-allocation fold's scratch (noalloc.hip)

@@ -4,6 +4,8 @@ their ordinary handlers; the batched form queues one iteration's loops between
 dvmh_amd_batch_begin_C / dvmh_amd_batch_end_C and runs each loop site as one
 persistent kernel over all grids' tiles.  Both are compared bit for bit with
 the oracle (grid values) and exactly (per-grid MAX eps)."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -109,14 +111,68 @@ def test_gpu_batched_matches_per_grid_seeded():
     _check(b, H.oracle_batched(a['sides'], iters, grids=sample), iters)
 
 
+class _Graphs:
+    """Graph replay of repeating batch queues on (DVMH_AMD_GRAPH) for a block."""
+
+    def __init__(self):
+        self.f = D.runtime().dvmh_amd_set_graph_C
+        self.f.argtypes = [ctypes.c_long]
+
+    def __enter__(self):
+        self.f(1)
+
+    def __exit__(self, *exc):
+        self.f(0)
+
+
 @pytest.mark.gpu
-def test_gpu_full_workload_properties():
-    """512 seeded grids x 200 iterations (the workload's size), batched: every
-    grid's eps is non-increasing, the boundaries stay 1.0, and the smallest and
-    largest grids match the oracle bit for bit."""
+def test_gpu_batched_graph_replay_bit_exact():
+    """The batched time step captured as one graph: the first iteration runs
+    eagerly, the second is captured (and launched), the rest replay the graph.
+    Results equal the eager batched form and the oracle bit for bit."""
+    count, iters = 64, 8
+    a, _ = _gpu_run(count, iters, 1)
+    with _Graphs():
+        b, d = _gpu_run(count, iters, 1)
+    assert d['graph_replays'] == iters - 2
+    assert d['batch_groups'] == 2 * 2   # only the eager and the captured iteration run the batch handlers
+    assert np.array_equal(a['eps'], b['eps'])
+    for g in range(count):
+        assert H.bits_equal(a['grids'][g]['A'], b['grids'][g]['A'])
+        assert H.bits_equal(a['grids'][g]['B'], b['grids'][g]['B'])
+    sample = [0, 5, 63, int(np.argmax(a['sides'])), int(np.argmin(a['sides']))]
+    _check(b, H.oracle_batched(a['sides'], iters, grids=sample), iters)
+
+
+@pytest.mark.gpu
+def test_gpu_batched_graph_ragged():
+    """Ragged sides (empty and one-point interiors) with graph replay, twice in
+    a row on fresh arrays (the graphs of the first run are dropped with its
+    arrays)."""
+    sides = [2, 3, 67, 5, 130, 64, 2, 257, 33, 1024]
+    iters = 6
+    with _Graphs():
+        for _ in range(2):
+            res, d = _gpu_run(len(sides), iters, 1, sides=sides)
+            assert d['graph_replays'] == iters - 2
+            _check(res, H.oracle_batched(sides, iters), iters)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('graph', [0, 1])
+def test_gpu_full_workload_properties(graph):
+    """512 seeded grids x 200 iterations (the workload's size), batched (and
+    with the time step replayed as a graph): every grid's eps is
+    non-increasing, the boundaries stay 1.0, and the smallest and largest
+    grids match the oracle bit for bit."""
     count, iters = 512, 200
-    res, d = _gpu_run(count, iters, 1, fetch=True)
-    assert d['batch_loops'] == 2 * iters * count
+    if graph:
+        with _Graphs():
+            res, d = _gpu_run(count, iters, 1, fetch=True)
+        assert d['graph_replays'] == iters - 2
+    else:
+        res, d = _gpu_run(count, iters, 1, fetch=True)
+        assert d['batch_loops'] == 2 * iters * count
     assert np.all(np.diff(res['eps'], axis=0) <= 0)
     for g in range(count):
         n = int(res['sides'][g])
