@@ -269,9 +269,17 @@ extern "C" void dvmh_distribute_C(DvmType dvmDesc[], DvmType rank, ...) {   // :
     fillUserHeader(d);
 }
 
-extern "C" void dvmh_redistribute_C(DvmType dvmDesc[], DvmType rank, ...) {
+// A single process keeps every element in place; an array declared without a
+// distribution (`#pragma dvm array`) gets its storage at its first one.
+extern "C" void dvmh_redistribute_C(DvmType dvmDesc[], DvmType rank, ...) {   // :63
     Guard g;
-    (void)dvmDesc; (void)rank;   // a single process keeps everything in place
+    Object *o = asObject(dvmDesc[0]);
+    if (o->kind == Kind::Template) return;
+    Data *d = asData(dvmDesc, "dvmh_redistribute_C");
+    DVMH_CHECK(rank == d->rank, "dvmh_redistribute_C: rank %ld does not match the array rank %d", rank, d->rank);
+    DVMH_CHECK(d->space[0].size() >= 1, "dvmh_redistribute_C: array has no size (missing dvmh_array_alloc_C)");
+    allocateHost(d);
+    fillUserHeader(d);
 }
 
 extern "C" DvmType dvmh_alignment_linear_C(DvmType axis, DvmType multiplier, DvmType summand) {   // :1364
@@ -309,6 +317,10 @@ extern "C" void dvmh_realign_C(DvmType dvmDesc[], DvmType, const DvmType templDe
     for (int i = 0; i < templRank; i++) ai.rules[i] = loadAlign(va_arg(ap, DvmType));
     va_end(ap);
     alignments[d] = ai;
+    // an array declared without a distribution gets its storage here
+    DVMH_CHECK(d->space[0].size() >= 1, "dvmh_realign_C: array has no size (missing dvmh_array_alloc_C)");
+    allocateHost(d);
+    fillUserHeader(d);
 }
 
 static void deleteData(Data *d) {
@@ -481,6 +493,15 @@ static void uploadMissing(Data *d) {
     }
     for (const Box &x : need) moveBlock(d, x.lo, x.hi, false);
     d->devPieces.add(d->whole());
+}
+
+// The same for one box of the array.
+static void uploadMissingBox(Data *d, const Box &b) {
+    d->ensureDev();
+    std::vector<Box> need = d->devPieces.missing(b);
+    if (need.empty()) return;
+    for (const Box &x : need) moveBlock(d, x.lo, x.hi, false);
+    d->devPieces.add(b);
 }
 
 // Each array keeps the pieces where its host copy and its device copy are
@@ -664,8 +685,47 @@ extern "C" void dvmh_indirect_shadow_renew_C(const DvmType[], DvmType, const cha
     fatal("error", "Indirect shadows need several processes; this runtime is single-process");
 }
 
-extern "C" void dvmh_remote_access_C(DvmType[], const DvmType[], DvmType, ...) {
-    fatal("error", "REMOTE_ACCESS is not supported by this single-process runtime");
+// The box a remote-access or sub-array rule list names: a constant rule one
+// index, any other rule the whole axis.
+static Box ruleBox(Data *d, va_list &ap) {
+    Box b = d->whole();
+    for (int a = 0; a < d->rank; a++) {
+        AlignRule r = loadAlign(va_arg(ap, DvmType));
+        if (r.axis == 0) {
+            DVMH_CHECK(r.summand >= d->space[a].lo && r.summand <= d->space[a].hi, "Remote index %ld is out of the array axis %d",
+                    r.summand, a + 1);
+            b.lo[a] = b.hi[a] = r.summand;
+        }
+    }
+    return b;
+}
+
+// REMOTE_ACCESS outside loops (rts/dvmh/dvmh2.cpp dvmh_remote_access_C): with
+// one process every element is local, so the "buffer" rmaDesc names is a
+// read-only alias of the array's host copy, whose referenced elements are made
+// current first.  Deleting the alias leaves the array alone.
+extern "C" void dvmh_remote_access_C(DvmType rmaDesc[], const DvmType dvmDesc[], DvmType rank, ...) {   // :133
+    Guard g;
+    Data *d = asData(dvmDesc, "dvmh_remote_access_C");
+    DVMH_CHECK(rank == d->rank, "dvmh_remote_access_C: rank mismatch");
+    va_list ap;
+    va_start(ap, rank);
+    const Box b = ruleBox(d, ap);
+    va_end(ap);
+    batchDrain();
+    flushDeferredPosts();
+    fetchMissing(d, b);
+    Data *alias = new Data();
+    alias->rank = d->rank;
+    alias->typeSize = d->typeSize;
+    alias->rtType = d->rtType;
+    for (int a = 0; a < d->rank; a++) alias->space[a] = d->space[a];
+    alias->host = d->host;
+    alias->ownsHost = false;
+    alias->hostOwnsAll();
+    alias->userHeader = rmaDesc;
+    fillUserHeader(alias);
+    rt.allData.push_back(alias);
 }
 
 static long linearIndex(Data *d, va_list &ap, bool *inside, long *point = nullptr) {
@@ -762,11 +822,24 @@ extern "C" void dvmh_region_register_array_C(DvmType curRegion, DvmType intent, 
     r->arrays.push_back(RegionArray{d, intent, varName ? varName : ""});
 }
 
+// A sub-array in a region's clause (rts/dvmh/dvmh2.cpp:3080): only that box
+// is made current where the region runs, and with an out intent only that box
+// is owned by the device afterwards.
 extern "C" void dvmh_region_register_subarray_C(DvmType curRegion, DvmType intent, const DvmType dvmDesc[], const char varName[],
         DvmType rank, ...) {
     Guard g;
-    (void)rank;
-    dvmh_region_register_array_C(curRegion, intent, dvmDesc, varName);
+    Region *r = asRegion(curRegion, "dvmh_region_register_subarray_C");
+    Data *d = asData(dvmDesc, "dvmh_region_register_subarray_C");
+    DVMH_CHECK(rank == d->rank, "dvmh_region_register_subarray_C: rank mismatch");
+    long lo[MAX_RANK], hi[MAX_RANK];
+    va_list ap;
+    va_start(ap, rank);
+    subarrayBounds(d, ap, lo, hi);
+    va_end(ap);
+    RegionArray ra{d, intent, varName ? varName : ""};
+    ra.part = true;
+    ra.box = boxOf(d, lo, hi);
+    r->arrays.push_back(ra);
 }
 
 extern "C" void dvmh_region_register_scalar_C(DvmType curRegion, DvmType intent, const void *addr, DvmType typeSize,
@@ -792,14 +865,23 @@ extern "C" void dvmh_region_execute_on_targets_C(DvmType curRegion, DvmType devi
         DVMH_CHECK(deviceTypes & DEVICE_TYPE_HOST, "No usable device for the region (targets %ld, GPU %s)", deviceTypes,
                 gpu.present ? "present" : "absent");
         // host execution: host copies must be current
-        for (auto &ra : r->arrays) getActual(ra.data);
+        for (auto &ra : r->arrays) {
+            if (!ra.part) {
+                getActual(ra.data);
+                continue;
+            }
+            batchDrain();
+            flushDeferredPosts();
+            fetchMissing(ra.data, ra.box);
+        }
         for (auto &rs : r->scalars) getActual(rs.data);
         return;
     }
     for (auto &ra : r->arrays) {
         Data *d = ra.data;
         if (ra.intent & INTENT_LOCAL) { d->ensureDev(); continue; }
-        uploadMissing(d);
+        if (ra.part) uploadMissingBox(d, ra.box);
+        else uploadMissing(d);
     }
     for (auto &rs : r->scalars) {
         Data *d = rs.data;
@@ -815,8 +897,15 @@ extern "C" void dvmh_region_end_C(DvmType curRegion) {   // :3122
     Guard g;
     Region *r = asRegion(curRegion, "dvmh_region_end_C");
     if (r->onGpu) {
-        for (auto &ra : r->arrays)
-            if (ra.intent & (INTENT_OUT)) ra.data->devOwnsAll();   // written on the device
+        for (auto &ra : r->arrays) {
+            if (!(ra.intent & INTENT_OUT)) continue;
+            if (!ra.part) {
+                ra.data->devOwnsAll();   // written on the device
+            } else {
+                ra.data->devPieces.add(ra.box);
+                ra.data->hostPieces.remove(ra.box);
+            }
+        }
         bool needSync = false;
         for (auto &rs : r->scalars)
             if (rs.intent & INTENT_OUT) needSync = true;
@@ -829,8 +918,15 @@ extern "C" void dvmh_region_end_C(DvmType curRegion) {   // :3122
     }
     if (!r->onGpu) {
         // a region run on the host wrote its output arrays there
-        for (auto &ra : r->arrays)
-            if (ra.intent & (INTENT_OUT)) ra.data->hostOwnsAll();
+        for (auto &ra : r->arrays) {
+            if (!(ra.intent & INTENT_OUT)) continue;
+            if (!ra.part) {
+                ra.data->hostOwnsAll();
+            } else {
+                ra.data->hostPieces.add(ra.box);
+                ra.data->devPieces.remove(ra.box);
+            }
+        }
     }
     if (rt.curRegion == r) rt.curRegion = nullptr;
     if (!batchKeepsRegion(r)) delete r;   // else freed when its queued loops have run
@@ -946,8 +1042,17 @@ extern "C" void dvmh_loop_across_C(DvmType curLoop, DvmType isOut, const DvmType
 extern "C" void dvmh_loop_shadow_compute_C(DvmType curLoop, const DvmType[], DvmType, ...) { Guard g; (void)curLoop; }
 extern "C" void dvmh_loop_shadow_compute_array_C(DvmType curLoop, const DvmType[]) { Guard g; (void)curLoop; }
 extern "C" void dvmh_loop_consistent_C(DvmType curLoop, const DvmType[], DvmType, ...) { Guard g; (void)curLoop; }
-extern "C" void dvmh_loop_remote_access_C(DvmType, const DvmType[], DvmType, ...) {
-    fatal("error", "REMOTE_ACCESS is not supported by this single-process runtime");
+// REMOTE_ACCESS of a parallel loop: every element is local to the one
+// process, so only the arguments are checked.
+extern "C" void dvmh_loop_remote_access_C(DvmType curLoop, const DvmType dvmDesc[], DvmType rank, ...) {   // :177
+    Guard g;
+    asLoop(curLoop, "dvmh_loop_remote_access_C");
+    Data *d = asData(dvmDesc, "dvmh_loop_remote_access_C");
+    DVMH_CHECK(rank == d->rank, "dvmh_loop_remote_access_C: rank mismatch");
+    va_list ap;
+    va_start(ap, rank);
+    for (int a = 0; a < rank; a++) loadAlign(va_arg(ap, DvmType));
+    va_end(ap);
 }
 
 extern "C" void dvmh_loop_array_correspondence_C(DvmType curLoop, const DvmType dvmDesc[], DvmType rank, ...) {   // :179

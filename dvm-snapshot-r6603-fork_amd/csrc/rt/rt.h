@@ -200,6 +200,8 @@ struct RegionArray {
     Data *data;
     long intent;
     std::string name;
+    bool part = false;   // registered as a sub-array: only `box` is moved and owned
+    Box box{};
 };
 
 struct Region : Object {

@@ -426,9 +426,57 @@ class Converter:
             out.append((p, b))
         return out
 
+    def empty_macros(self):
+        """Function-like macros the program defines as nothing (`#define DVM(dir)`,
+        the old C-DVM directive form, which cdvmh leaves to the preprocessor):
+        an invocation of one is not C; it is skipped where statements are split
+        and dropped where declarations are read."""
+        out = []
+        for dfn in self.defines:
+            m = re.match(r'#\s*define\s+([A-Za-z_]\w*)\(([^)]*)\)\s*$', dfn)
+            if m:
+                out.append(m.group(1))
+        return out
+
+    def header_macros(self):
+        """Function-like macros whose replacement is a loop or if header
+        (`#define DO(v,l,h,s) for(v=(l); v<=(h); v+=(s))`): an invocation heads
+        the statement that follows it."""
+        out = []
+        for dfn in self.defines:
+            m = re.match(r'#\s*define\s+([A-Za-z_]\w*)\([^)]*\)\s*(for|while|if)\s*\(', dfn)
+            if m:
+                out.append(m.group(1))
+        return out
+
+    def macro_head(self, i, b):
+        """Index of the ')' closing the invocation at toks[i] of an empty or
+        header macro, else None."""
+        t = self.toks[i]
+        if t.kind == 'id' and i + 1 < b and self.toks[i + 1].text == '(' and \
+                (t.text in self.empty_macros() or t.text in self.header_macros()):
+            return matching(self.toks, i + 1)
+        return None
+
+    def strip_empty_macros(self, text):
+        for name in self.empty_macros():
+            while True:
+                m = re.search(r'\b' + name + r'\s*\(', text)
+                if not m:
+                    break
+                depth, k = 0, m.end() - 1
+                while k < len(text):
+                    depth += {'(': 1, ')': -1}.get(text[k], 0)
+                    if depth == 0:
+                        break
+                    k += 1
+                text = text[:m.start()] + text[k + 1:]
+        return text
+
     def function(self, name, lparen, rparen, lbrace, rbrace, inherited=()):
         self.scope = dict(self.globals)
         for p in split_top(self.text(lparen + 1, rparen)):
+            p = self.strip_empty_macros(p).strip()
             if p in ('', 'void'):
                 continue
             d = parse_declaration(p)
@@ -473,7 +521,11 @@ class Converter:
                 out.append(gap)
                 if d.kind in ('actual', 'get_actual'):
                     out.append(self.actual(d, self.indent_of(i)))
-                elif d.kind in ('array', 'region', 'parallel'):
+                elif d.kind == 'realign':
+                    out.append(self.realign(d, self.indent_of(i)))
+                elif d.kind == 'redistribute':
+                    out.append(self.redistribute(d, self.indent_of(i)))
+                elif d.kind in ('array', 'region', 'parallel', 'remote_access'):
                     pending.append(d)
                 else:
                     self.err(t, f'directive {d.kind!r} is not supported')
@@ -490,6 +542,8 @@ class Converter:
                     out.append(self.array_decl(d, i, j))
                 elif d.kind == 'region':
                     out.append(self.region(d, i, j, ctx))
+                elif d.kind == 'remote_access':
+                    out.append(self.remote_block(d, i, j, ctx))
                 else:
                     out.append(self.parallel(d, i, j, ctx))
             else:
@@ -508,6 +562,9 @@ class Converter:
             return matching(self.toks, i) + 1
         if t.kind == 'pp':
             return i + 1
+        k = self.macro_head(i, b)
+        if k is not None:
+            return self.statement_end(k + 1, b)
         if t.text in ('for', 'while', 'switch'):
             k = matching(self.toks, i + 1)
             return self.statement_end(k + 1, b)
@@ -540,6 +597,10 @@ class Converter:
         if t.text == '{':
             k = matching(toks, i)
             return '{' + self.block(i + 1, k, ctx) + '}'
+        k = self.macro_head(i, j)
+        if k is not None and k + 1 < j:
+            # DVM(...) or a loop-header macro: kept as written, the statement after it converted
+            return self.host_expr(i, k + 1, ctx) + self.src[toks[k].end:toks[k + 1].start] + self.plain_statement(k + 1, j, ctx)
         if t.text in ('for', 'while', 'switch', 'if'):
             k = matching(toks, i + 1)
             head = self.host_expr(i, k + 1, ctx)
@@ -581,8 +642,52 @@ class Converter:
         if m and m.group(1) in self.scope and self.scope[m.group(1)].dvm:
             return self.line_call(t.line) + '\n' + self.indent_of(i) + f'dvmh_array_free_C({m.group(1)});'
         if ctx['region'] is not None and self.uses_dvm_array(i, j):
-            self.err(t, 'a distributed array used in a region outside a parallel loop is not supported')
+            return self.sequential_part(i, j, ctx)
         return self.host_expr(i, j, ctx)
+
+    def sequential_part(self, i, j, ctx):
+        """A region's statement outside parallel loops that uses distributed
+        arrays (cdvmh/src/converter_main.cpp:1310-1330): a loop of rank 0 whose
+        handlers run the statement once where the region runs."""
+        t = self.toks[i]
+        line0 = self.src.rfind('\n', 0, t.start) + 1
+        body = self.dedent(self.src[line0:self.toks[j - 1].end])
+        params = []
+        for x in tokenize(body):
+            if x.kind != 'id' or x.text not in self.scope or any(p[1] == x.text for p in params):
+                continue
+            v = self.scope[x.text]
+            if v.dvm:
+                params.append(('array', x.text, v.ctype, v.dvm.rank))
+            elif not v.dims and not v.pointer_rank:
+                params.append(('scalar', x.text, v.ctype))
+            else:
+                self.err(t, f'{x.text}: C arrays and pointers in a region\'s sequential part are not supported')
+        name = f'sequence_{self.prog}_{t.line}'
+        while any(L['name'] == name for L in self.loops):
+            name += '_'
+        self.loops.append({'name': name, 'vars': [], 'steps': [], 'params': params, 'body': '\n' + body + '\n',
+                           'sequential': True})
+        self.handlers.append((name, params, False))
+        for p in params:
+            written = re.search(r'\b' + p[1] + r'\s*(?:\[[^;]*?\]\s*)*(?:=(?!=)|\+=|-=|\*=|/=|\+\+|--)', body) or \
+                re.search(r'(?:\+\+|--)\s*' + p[1] + r'\b', body)
+            if p[0] == 'array':
+                ctx['arrays'][p[1]] = ctx['arrays'].get(p[1], False) or bool(written)
+            else:
+                ctx['scalars'][p[1]] = p[2]
+                if written:
+                    ctx.setdefault('scalars_out', set()).add(p[1])
+        lp = ctx['loop']
+        pargs = ', '.join(p[1] if p[0] == 'array' else '&' + p[1] for p in params)
+        out = [self.line_call(t.line),
+               f'{lp} = dvmh_loop_create_C({ctx["region"]}, 0);',
+               f'dvmh_loop_register_handler_C({lp}, DEVICE_TYPE_HOST, 0, '
+               f'dvmh_handler_func_C((DvmHandlerFunc){name}_host, {len(params)}, {pargs}));',
+               f'dvmh_loop_register_handler_C({lp}, DEVICE_TYPE_CUDA, 0, '
+               f'dvmh_handler_func_C((DvmHandlerFunc){name}_cuda, {len(params)}, {pargs}));',
+               f'dvmh_loop_perform_C({lp});']
+        return ('\n' + self.indent_of(i)).join(out)
 
     def indent_of(self, i):
         s = self.src
@@ -677,9 +782,7 @@ class Converter:
             cl = cl.strip()
             m = re.match(r'distribute\s*((?:\[[^\]]*\]\s*)+)$', cl)
             if m:
-                fm = ['' if f.strip() == '*' else f.strip() for f in brackets(m.group(1))]   # [*]: not distributed
-                if len(fm) != rank or any(f not in ('block', '') for f in fm):
-                    self.err(d.line, 'distribute accepts [block], [*] or [] per axis')
+                fm = self.dist_formats(brackets(m.group(1)), rank, d.line)
                 rule = ('distribute', fm)
                 continue
             m2 = re.match(r'align\s*\(\s*((?:\[[^\]]*\]\s*)+)\s*with\s+([A-Za-z_]\w*)\s*((?:\[[^\]]*\]\s*)+)\)\s*$', cl)
@@ -696,9 +799,9 @@ class Converter:
                     lo, sep, hi = w.partition(':')
                     shadow.append((lo.strip(), (hi if sep else lo).strip()))
                 continue
-            self.err(d.line, f'array directive clause {cl!r} is not supported')
-        if rule is None:
-            self.err(d.line, f'array directive {d.rest!r} is not supported')
+            if cl:
+                self.err(d.line, f'array directive clause {cl!r} is not supported')
+        # no rule: distributed later by redistribute, or aligned by realign
         a = DvmArray(v, rank, sizes, bool(pr), rule)
         if shadow:
             a.shadow = shadow
@@ -728,13 +831,58 @@ class Converter:
         ind = self.indent_of(i)
         return f'DvmType {v.name}[64];\n' + ind + self.array_create(a, ind)
 
+    def dist_formats(self, items, rank, line):
+        """Per-axis distribution formats of distribute[...] / redistribute:
+        block, [*] or [] (not distributed), wgtblock(w, n), genblock(g),
+        multblock(m)."""
+        fm = []
+        for f in items:
+            f = f.strip()
+            if f in ('*', ''):
+                fm.append(('', None))
+                continue
+            if f == 'block':
+                fm.append(('block', None))
+                continue
+            m = re.match(r'(wgtblock|genblock|multblock)\s*\((.*)\)$', f, re.S)
+            if not m:
+                self.err(line, f'distribution format {f!r} is not supported')
+            fm.append((m.group(1), [x.strip() for x in split_top(m.group(2))]))
+        if len(fm) != rank:
+            self.err(line, f'{len(fm)} distribution formats for an array of rank {rank}')
+        return fm
+
+    def dist_args(self, fm, line):
+        """The distribution rule calls of `fm`; processor axes count the
+        distributed array axes only (cdvmh/src/converter_main.cpp:530-590)."""
+        out, mps = [], 1
+        for f, args in fm:
+            if f == '':
+                out.append('dvmh_distribution_replicated_()')
+                continue
+            if f == 'block':
+                out.append(f'dvmh_distribution_block_C({mps})')
+            elif f == 'multblock':
+                out.append(f'dvmh_distribution_multblock_C({mps}, {args[0]})')
+            else:
+                w = self.scope.get(args[0])
+                if not w or w.ctype not in RT_TYPE or w.dvm:
+                    self.err(line, f'{f}: {args[0]!r} is not an array of a known element type')
+                if f == 'wgtblock':
+                    if len(args) != 2:
+                        self.err(line, 'wgtblock(weights, count) expected')
+                    out.append(f'dvmh_distribution_wgtblock_C({mps}, {RT_TYPE[w.ctype]}, {args[0]}, {args[1]})')
+                else:
+                    out.append(f'dvmh_distribution_genblock_C({mps}, {RT_TYPE[w.ctype]}, {args[0]})')
+            mps += 1
+        return ', '.join(out)
+
     def distribution_call(self, a):
         nm = a.var.name
+        if a.rule is None:
+            return ''
         if a.rule[0] == 'distribute':
-            fm = a.rule[1]
-            args = ', '.join(f'dvmh_distribution_block_C({q + 1})' if f == 'block' else 'dvmh_distribution_replicated_C()'
-                             for q, f in enumerate(fm))
-            return f'dvmh_distribute_C({nm}, {a.rank}, {args});'
+            return f'dvmh_distribute_C({nm}, {a.rank}, {self.dist_args(a.rule[1], a.line)});'
         _, dummies, target, tsubs = a.rule
         tv = self.scope.get(target)
         if not tv or not tv.dvm:
@@ -745,15 +893,81 @@ class Converter:
         return f'dvmh_align_C({nm}, {target}, {tv.dvm.rank}, {", ".join(rules)});'
 
     def linear_rule(self, sub, names, line):
-        """'i', 'i + 1', 'j - 2', '3' over the dummy/loop names -> alignment rule call."""
+        """An alignment or mapping subscript over the dummy / loop names ->
+        its rule call (cdvmh/src/converter_main.cpp:598-622): a*v + b gives
+        (axis of v, a, b); an expression of none of the names a constant
+        (0, 0, expr); an empty subscript replication (-1, 0, 0)."""
         sub = sub.strip()
-        m = re.match(r'^([A-Za-z_]\w*)\s*(?:([+-])\s*(\w+))?$', sub)
-        if m and m.group(1) in names:
-            off = '0' if not m.group(2) else (m.group(3) if m.group(2) == '+' else '-' + m.group(3))
-            return f'dvmh_alignment_linear_C({names.index(m.group(1)) + 1}, 1, {off})'
-        if re.match(r'^-?\w+$', sub) and sub not in names:
-            return f'dvmh_alignment_linear_C(-1, 0, {sub})'
-        self.err(line, f'subscript {sub!r} is not of the form v, v + c, v - c or c')
+        if not sub:
+            return 'dvmh_alignment_linear_C(-1, 0, 0)'
+        used = [t.text for t in tokenize(sub) if t.kind == 'id' and t.text in names]
+        if not used:
+            return f'dvmh_alignment_linear_C(0, 0, {sub})'
+        m = re.match(r'^(?:(\w+)\s*\*\s*)?([A-Za-z_]\w*)\s*(?:([+-])\s*(\w+))?$', sub)
+        if m and m.group(2) in names and (m.group(1) is None or m.group(1) not in names):
+            mult = m.group(1) or '1'
+            off = '0' if not m.group(3) else (m.group(4) if m.group(3) == '+' else '-' + m.group(4))
+            return f'dvmh_alignment_linear_C({names.index(m.group(2)) + 1}, {mult}, {off})'
+        self.err(line, f'subscript {sub!r} is not of the form a * v + b, a constant or empty')
+
+    def realign(self, d, ind):
+        """realign(X[i]... with T[...]... [new]) (cdvmh/src/converter_main.cpp:98-118)."""
+        m = re.match(r'\(\s*([A-Za-z_]\w*)\s*((?:\[[^\]]*\]\s*)+)\s*with\s+([A-Za-z_]\w*)\s*((?:\[[^\]]*\]\s*)+)(\s*new)?\s*\)\s*$',
+                     d.rest, re.S)
+        if not m:
+            self.err(d.line, 'realign(A[i]... with B[...]...) expected')
+        a, b = self.scope.get(m.group(1)), self.scope.get(m.group(3))
+        if not a or not a.dvm or not b or not b.dvm:
+            self.err(d.line, 'realign relates two distributed arrays')
+        dummies = [x.strip() for x in brackets(m.group(2))]
+        if len(dummies) != a.dvm.rank:
+            self.err(d.line, f'{a.name}: {len(dummies)} of {a.dvm.rank} dummy subscripts')
+        tsubs = brackets(m.group(4))
+        if len(tsubs) != b.dvm.rank:
+            self.err(d.line, f'{b.name}: {len(tsubs)} of {b.dvm.rank} subscripts')
+        rules = ', '.join(self.linear_rule(x, dummies, d.line) for x in tsubs)
+        return self.line_call(d.line) + '\n' + ind + \
+            f'dvmh_realign_C({a.name}, {1 if m.group(5) else 0}, {b.name}, {b.dvm.rank}, {rules});'
+
+    def redistribute(self, d, ind):
+        """redistribute(A[format]...) (cdvmh/src/converter_main.cpp:72-92)."""
+        m = re.match(r'\(\s*([A-Za-z_]\w*)\s*((?:\[[^\]]*\]\s*)+)\)\s*$', d.rest, re.S)
+        if not m or not self.scope.get(m.group(1)) or not self.scope[m.group(1)].dvm:
+            self.err(d.line, 'redistribute(A[format]...) of a distributed array expected')
+        a = self.scope[m.group(1)].dvm
+        fm = self.dist_formats(brackets(m.group(2)), a.rank, d.line)
+        return self.line_call(d.line) + '\n' + ind + f'dvmh_redistribute_C({a.var.name}, {a.rank}, {self.dist_args(fm, d.line)});'
+
+    def rma_items(self, text, line):
+        """remote_access(A[e]..., ...) items -> [(array, subscripts)]."""
+        out = []
+        for item in split_top(text):
+            im = re.match(r'\s*([A-Za-z_]\w*)\s*((?:\[[^\]]*\]\s*)*)$', item)
+            v = self.scope.get(im.group(1)) if im else None
+            if not v or not v.dvm:
+                self.err(line, f'remote_access of {item.strip()!r}, which is not a distributed array')
+            subs = brackets(im.group(2))
+            if subs and len(subs) != v.dvm.rank:
+                self.err(line, f'{v.name}: {len(subs)} of {v.dvm.rank} subscripts')
+            out.append((v, subs or [''] * v.dvm.rank))
+        return out
+
+    def remote_block(self, d, i, j, ctx):
+        """A remote_access directive on a host block (converter_main.cpp:1250-1300):
+        the referenced elements are read on the host, so they are made current
+        there first (one process holds every element)."""
+        m = re.match(r'\((.*)\)\s*$', d.rest, re.S)
+        if not m or self.toks[i].text != '{':
+            self.err(d.line, 'remote_access(A[...]...) followed by a compound statement expected')
+        calls = [self.line_call(d.line)]
+        for v, subs in self.rma_items(m.group(1), d.line):
+            args = []
+            for x in subs:
+                x = x.strip()
+                args += [f'(DvmType)({x})', f'(DvmType)({x})'] if x else ['UNDEF_BOUND', 'UNDEF_BOUND']
+            calls.append(f'dvmh_get_actual_subarray_C({v.name}, {v.dvm.rank}, {", ".join(args)});')
+        ind = self.indent_of(i)
+        return ('\n' + ind).join(calls) + '\n' + ind + self.plain_statement(i, j, ctx)
 
     def actual(self, d, ind):
         fn_scalar = 'dvmh_actual_variable2_' if d.kind == 'actual' else 'dvmh_get_actual_variable2_'
@@ -795,10 +1009,15 @@ class Converter:
         if ctx['region'] is not None:
             self.err(d.line, 'nested regions')
         clauses = parse_clauses(d.rest)
-        intents, flags = {}, []
+        intents, flags, parts = {}, [], []
         for name, args in clauses:
             if name in ('in', 'out', 'inout', 'local'):
                 for a in split_top(args):
+                    am = re.match(r'\s*([A-Za-z_]\w*)\s*((?:\[[^\]]*\]\s*)+)$', a)
+                    if am:
+                        # a sub-array: [e] one index, [lo : hi] a range, [] the whole axis
+                        parts.append((am.group(1), 'INTENT_' + name.upper(), brackets(am.group(2))))
+                        continue
                     intents[a.strip()] = 'INTENT_' + name.upper()
             elif name == 'async':
                 flags.append('REGION_ASYNC')
@@ -817,15 +1036,27 @@ class Converter:
                f'DvmType {rname} = dvmh_region_create_C({" | ".join(flags) if flags else "0"});',
                f'DvmType {lname} = 0;']
         arrays = dict(inner_ctx['arrays'])
-        for a in intents:
+        for a in list(intents) + [p[0] for p in parts]:
             if a not in self.scope or not self.scope[a].dvm:
                 self.err(d.line, f'region clause names {a!r}, which is not a distributed array')
-        names = list(dict.fromkeys(list(intents) + list(arrays)))
+        part_names = {p[0] for p in parts}
+        names = list(dict.fromkeys(list(intents) + [a for a in arrays if a not in part_names]))
         for a in names:
             intent = intents.get(a) or ('INTENT_INOUT' if arrays[a] else 'INTENT_IN')
             reg.append(f'dvmh_region_register_array_C({rname}, {intent}, {a}, "{a}");')
+        for a, intent, subs in parts:
+            rank = self.scope[a].dvm.rank
+            if len(subs) != rank:
+                self.err(d.line, f'{a}: {len(subs)} of {rank} sub-array bounds')
+            bounds = []
+            for x in subs:
+                lo, sep, hi = x.partition(':')
+                lo, hi = lo.strip(), (hi.strip() if sep else lo.strip())
+                bounds += [f'(DvmType)({lo})' if lo else 'UNDEF_BOUND', f'(DvmType)({hi})' if hi else 'UNDEF_BOUND']
+            reg.append(f'dvmh_region_register_subarray_C({rname}, {intent}, {a}, "{a}", {rank}, {", ".join(bounds)});')
+        outs = inner_ctx.get('scalars_out', set())
         for s, ct in inner_ctx['scalars'].items():
-            reg.append(f'dvmh_region_register_scalar_C({rname}, INTENT_IN, &{s}, sizeof({s}), "{s}");')
+            reg.append(f'dvmh_region_register_scalar_C({rname}, {"INTENT_INOUT" if s in outs else "INTENT_IN"}, &{s}, sizeof({s}), "{s}");')
         reg.append(f'dvmh_region_execute_on_targets_C({rname}, DEVICE_TYPE_HOST | DEVICE_TYPE_CUDA);')
         (void_loop) = f'(void){lname};'
         return '{\n' + ''.join(ind2 + r + '\n' for r in reg) + ind2 + void_loop + inner + '\n' + ind2 + \
@@ -863,7 +1094,7 @@ class Converter:
         # from the start of the first body line, so every line keeps its indentation
         line0 = self.src.rfind('\n', 0, self.toks[first].start) + 1
         body = self.dedent(self.src[line0:self.toks[last - 1].end])
-        reds, privates, across, shadows, cuda_block = [], [], None, [], None
+        reds, privates, across, shadows, cuda_block, rmas = [], [], None, [], None, []
         red_lengths = {}
         for name, args in clauses:
             if name == 'reduction':
@@ -917,6 +1148,8 @@ class Converter:
                     shadows.append(x)
             elif name == 'cuda_block':
                 cuda_block = [x.strip() for x in split_top(args)]
+            elif name == 'remote_access':
+                rmas += self.rma_items(args, d.line)
             else:
                 self.err(d.line, f'parallel clause {name!r} is not supported')
         # parameters: distributed arrays and scalars the body uses
@@ -985,6 +1218,10 @@ class Converter:
         if across:
             ws = ''.join(f', (DvmType){lo}, (DvmType){hi}' for lo, hi in across['widths'])
             out.append(f'dvmh_loop_across_C({lp}, 0, {across["array"]}, {len(across["widths"])}{ws});')
+        for v, subs in rmas:
+            # with one process every element is local: the runtime only checks
+            rr = ', '.join(self.linear_rule(x, lvars, d.line) for x in subs)
+            out.append(f'dvmh_loop_remote_access_C({lp}, {v.name}, {v.dvm.rank}, {rr});')
         for r in reds:
             if r[2] not in RT_TYPE:
                 self.err(d.line, f'reduction type {r[2]!r} is not supported')

@@ -70,6 +70,9 @@ class Loop:
         self.params = d['params']                  # [('array', name, ctype, rank) | ('scalar', name, ctype)
                                                    #  | ('carray', name, ctype, [extents])]
         self.host_only = d.get('host_only', False)  # a loop outside regions: no GPU handler
+        # a region's statement outside parallel loops: a loop of rank 0 whose
+        # handlers run the statement once (cdvmh's sequential part)
+        self.sequential = d.get('sequential', False)
         self.privates = d.get('privates', [])      # [(decl text)]
         self.reds = d.get('reductions', [])        # [(func, var, ctype)] or [(func, var, ctype, loc, loctype)]
         # reduction arrays: variable -> element count (a C array of that many
@@ -214,7 +217,34 @@ class Loop:
 
 
 # ----------------------------------------------------------------- host handlers
+def seq_host_handler(L):
+    """Sequential part of a region (converter_main.cpp:1310-1330, a loop of
+    rank 0, host handler type 0): the statement runs once, by the calling
+    thread, against the arrays' host copies."""
+    formal = ['DvmType *pLoopRef']
+    for p in L.params:
+        formal.append(f'DvmType {p[1]}_hdr[]' if p[0] == 'array' else f'{p[2]} *{p[1]}_ptr')
+    b = ['DvmType loop_ref = *pLoopRef;', 'DvmType device_num = dvmh_loop_get_device_num_C(loop_ref);']
+    for p in L.params:
+        if p[0] == 'array':
+            _, nm, ct, ar = p
+            dims = ''
+            for j in range(2, ar + 1):
+                dims += f'[{nm}_hdr[{j - 1}]/{nm}_hdr[{j}]]' if j < ar else f'[{nm}_hdr[{j - 1}]]'
+            b.append(f'{ct} (*{nm}){dims} = ({ct} (*){dims})dvmh_get_natural_base_C(device_num, {nm}_hdr);')
+        else:
+            # the variable itself (a sequential part may assign the region's scalars)
+            b.append(f'#define {p[1]} (*{p[1]}_ptr)')
+    b.append('(void)loop_ref;')
+    b.append('{\n' + indent(L.body + '\n', 1) + '}')
+    for p in L.scalars:
+        b.append(f'#undef {p[1]}')
+    return f'void {L.name}_host({", ".join(formal)}) {{\n' + indent('\n'.join(b) + '\n', 1) + '}\n'
+
+
 def host_handler(L):
+    if L.sequential:
+        return seq_host_handler(L)
     r = L.rank
     out = []
     formal = ['DvmType *pLoopRef']
@@ -2241,8 +2271,48 @@ def across_handler(L):
     return '\n'.join(h) + '\n'
 
 
+def seq_kernel(L):
+    """The sequential part on the device: one lane runs the statement."""
+    fp, setup = kernel_array_params(L, unit=True)
+    k = [f'template <typename IndexT>',
+         f'__global__ __launch_bounds__(64) void {L.name}_kernel({", ".join(fp)}) {{',
+         IND + 'if (blockIdx.x != 0 || threadIdx.x != 0) return;']
+    k += [IND + x for x in setup]
+    k.append(indent('{\n' + indent(L.body + '\n', 1) + '}', 1))
+    k.append('}')
+    return '\n'.join(k) + '\n'
+
+
+def seq_handler(L):
+    h = [f'extern "C" void {L.name}_cuda(DvmType *pLoopRef{"".join(", DvmType " + p[1] + "_hdr[]" if p[0] == "array" else ", " + p[2] + " *" + p[1] + "_ptr" for p in L.params)}) {{']
+    b = ['DvmType tmpVar;',
+         'DvmType loop_ref = *pLoopRef;',
+         'DvmType device_num = dvmh_loop_get_device_num_C(loop_ref);']
+    for p in L.arrays:
+        _, nm, ct, ar = p
+        b.append(f'{ct} *{nm} = ({ct} *)dvmh_get_natural_base_C(device_num, {nm}_hdr);')
+        b.append(f'DvmType {nm}_devHdr[64 + 7];')
+        b.append(f'tmpVar = dvmh_fill_header_C(device_num, {nm}, {nm}_hdr, {nm}_devHdr, 0);')
+        b.append(f'if (tmpVar != 0) {{ fprintf(stderr, "{L.name}: array {nm} is not in natural layout\\n"); abort(); }}')
+    for p in L.scalars:
+        b.append(f'{p[2]} *{p[1]} = ({p[2]} *)dvmh_get_device_addr_C(device_num, {p[1]}_ptr);')
+    b.append('hipStream_t stream;')
+    b.append('dvmh_loop_cuda_get_config_C(loop_ref, 0, 0, 0, &stream, 0);')
+    b.append('int kernelIndexT = (int)dvmh_loop_guess_index_type_C(loop_ref);')
+    b.append('dvmh_amd_mark_kernel_C(loop_ref, 0);')
+    for it, cond in (('int', 'kernelIndexT == rt_INT'), ('long long', 'kernelIndexT != rt_INT')):
+        args = array_fact_args(L, it)
+        b.append(f'if ({cond}) hipLaunchKernelGGL(({L.name}_kernel<{it}>), dim3(1), dim3(64), 0, stream, {", ".join(args)});')
+    b.append('dvmh_amd_mark_kernel_C(loop_ref, 1);')
+    h.append(indent('\n'.join(b) + '\n', 1).rstrip('\n'))
+    h.append('}')
+    return '\n'.join(h) + '\n'
+
+
 def gpu_loop(L):
     text = []
+    if L.sequential:
+        return seq_kernel(L) + '\n' + seq_handler(L)
     if L.across:
         r = L.rank
         for dep in range(r):

@@ -89,9 +89,9 @@ BAD = [
     ('nested regions', '#pragma dvm region\n{\n#pragma dvm region\n{\n}\n}\n', 'nested regions'),
     ('imperfect nest', '#pragma dvm region\n{\n#pragma dvm parallel([i][j] on A[i][j])\nfor (i = 0; i < n; i++) {\n'
      'x = 1;\nfor (j = 0; j < n; j++) A[i][j] = 0;\n}\n}\n', 'perfect'),
-    ('unknown clause', '#pragma dvm region\n{\n#pragma dvm parallel([i][j] on A[i][j]) remote_access(A[i][j])\n'
-     'for (i = 0; i < n; i++) for (j = 0; j < n; j++) A[i][j] = 0;\n}\n', 'remote_access'),
-    ('non-unit subscript', '#pragma dvm region\n{\n#pragma dvm parallel([i][j] on A[2 * i][j])\n'
+    ('unknown clause', '#pragma dvm region\n{\n#pragma dvm parallel([i][j] on A[i][j]) shadow_compute\n'
+     'for (i = 0; i < n; i++) for (j = 0; j < n; j++) A[i][j] = 0;\n}\n', 'shadow_compute'),
+    ('non-linear subscript', '#pragma dvm region\n{\n#pragma dvm parallel([i][j] on A[i * j][j])\n'
      'for (i = 0; i < n; i++) for (j = 0; j < n; j++) A[i][j] = 0;\n}\n', 'subscript'),
     ('C array in a region', '#pragma dvm region\n{\n#pragma dvm parallel([i][j] on A[i][j])\n'
      'for (i = 0; i < n; i++) for (j = 0; j < n; j++) A[i][j] = C[i][j];\n}\n', 'inside regions'),

@@ -284,3 +284,71 @@ def test_reference_example_program(stem, ref, tmp_path):
     assert out == eps_lines(hist)
     if final is not None:
         assert data.shape == final.shape and np.array_equal(data.view(np.uint64), final.view(np.uint64))
+
+
+@pytest.mark.parametrize('stem', ['gauss_c', 'gauss_wb'])
+def test_gauss_examples(stem, tmp_path):
+    """examples/gauss_c.cdv and examples/gauss_wb.cdv: arrays declared without
+    a distribution, redistribute with WGTBLOCK weights (gauss_wb), realign,
+    remote_access in a loop clause and on a host block, a region over
+    sub-arrays and sequential parts of regions.  The system is the identity
+    with a right-hand side of ones, so every X[j] the program prints is 1."""
+    out, _ = build_and_run(stem, tmp_path)
+    assert out == ''.join('j=%4i   X[j]=%e\n' % (j, 1.0) for j in range(8, -1, -1))
+
+
+def test_mgrid_old_directive_form(tmp_path):
+    """examples/mgrid.cdv is written in the old C-DVM form (DVM(...) macros the
+    program defines as nothing, DO(...) loop-header macros), which cdvmh
+    leaves to the preprocessor: the program converts and runs as plain C.
+    Its output is the golden one of examples/democ_res.uni:323-324."""
+    out, _ = build_and_run('mgrid', tmp_path)
+    assert out == 'N1=8 M1=12 K1=16 \nN1=4 M1=6 K1=8 \n'
+
+
+def tsk_lp_reference():
+    """numpy restatement of examples/tsk_lp.cdv (FP32 grid blocks of two tasks,
+    boundary exchange between the blocks, then a 4-point relaxation and
+    MAX(|B - A|) per block; K = 8, Nt = {4, 4}, 10 iterations)."""
+    K, Nt, ntask = 8, [4, 4], 2
+    f32 = np.float32
+    At, Bt = [], []
+    for t in range(ntask):
+        N, N0 = Nt[t], sum(Nt[i] - 1 for i in range(t))
+        A = np.zeros((N + 1, K), dtype=f32)
+        for i in range(N + 1):
+            for j in range(K):
+                edge = j == 0 or j == K - 1 or (t == 0 and i == 0) or (t == ntask - 1 and i == N)
+                A[i, j] = f32(0) if edge else f32(1 + i + N0 + j)
+        At.append(A)
+        Bt.append(A.copy())
+    lines = ['Task %d mapped onto processors 0..0' % t for t in range(ntask)] + ['-' * 37]
+    for it in range(1, 11):
+        eps0 = 0.0
+        for t in range(ntask):
+            A = At[t]
+            if t != 0:
+                A[0, :] = Bt[t - 1][Nt[t - 1] - 1, :]
+            if t != ntask - 1:
+                A[Nt[t], :] = Bt[t + 1][1, :]
+        for t in range(ntask):
+            A, B, N = At[t], Bt[t], Nt[t]
+            eps = 0.0
+            for i in range(1, N):
+                for j in range(1, K - 1):
+                    B[i, j] = (((A[i - 1, j] + A[i, j - 1]) + A[i + 1, j]) + A[i, j + 1]) / f32(4)
+            for i in range(1, N):
+                for j in range(1, K - 1):
+                    eps = max(abs(float(B[i, j] - A[i, j])), eps)
+                    A[i, j] = B[i, j]
+            eps0 = max(eps0, eps)
+        lines.append('it=%d eps0=%f' % (it, eps0))
+    return ''.join(x + '\n' for x in lines)
+
+
+def test_tsk_lp_old_directive_form(tmp_path):
+    """examples/tsk_lp.cdv (task regions in the old DVM(...) form, left to the
+    preprocessor as cdvmh does): every line it prints equals the numpy
+    restatement of the program."""
+    out, _ = build_and_run('tsk_lp', tmp_path)
+    assert out == tsk_lp_reference()
