@@ -28,7 +28,8 @@ import pytest
 REF = '/root/reference/dvm_sys/examples'
 PKG = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), 'dvm-snapshot-r6603-fork_amd')
 REPO = os.path.dirname(PKG)
-sys.path.insert(0, os.path.join(PKG, 'emit'))
+sys.path.insert(0, os.path.join(REPO, 'oracle'))
+import build_ref_examples as BRE  # noqa: E402
 
 pytestmark = pytest.mark.skipif(not os.path.isdir(REF), reason='the reference tree is not present')
 
@@ -36,43 +37,17 @@ pytestmark = pytest.mark.skipif(not os.path.isdir(REF), reason='the reference tr
 def build_and_run(stem, tmp_path, sizes=None):
     """sizes: {macro: value} -- the program's `#define macro N` lines are
     rewritten to N in a copy under tmp_path before conversion."""
-    import cdvmh
-    import hipgen
     out = str(tmp_path)
-    src = os.path.join(REF, stem + '.cdv')
-    if sizes:
-        with open(src) as f:
-            text = f.read()
-        for name, val in sizes.items():
-            text, n = re.subn(r'(#define\s+%s\s+)\d+' % name, r'\g<1>%d' % val, text)
-            assert n == 1, name
-        src = os.path.join(out, stem + '.cdv')
-        with open(src, 'w') as f:
-            f.write(text)
-    cdvmh.convert_file(src, out, prog=stem)
-    hipgen.emit(os.path.join(out, stem + '.loops'), out)
-    inc = ['-I' + os.path.join(REPO, 'include'), '-I' + os.path.join(PKG, 'programs')]
-    cc = ['gcc', '-O2', '-fPIC', '-fopenmp', '-ffp-contract=off', '-D_DVMH=1'] + inc
-    objs = []
-    for src in (f'{stem}.c', f'{stem}_host.c'):
-        o = os.path.join(out, src + '.o')
-        subprocess.run(cc + ['-c', os.path.join(out, src), '-o', o], check=True, timeout=120)
-        objs.append(o)
-    o = os.path.join(out, stem + '_hip.o')
-    subprocess.run(['/opt/rocm/bin/hipcc', '--offload-arch=gfx950', '-O3', '-fPIC', '-std=c++17', '-ffp-contract=off',
-                    '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops', '-D__HIP_PLATFORM_AMD__',
-                    '-I' + os.path.join(PKG, 'csrc', 'rt'), '-I' + os.path.join(PKG, 'csrc', 'kernels')] + inc +
-                   ['-c', os.path.join(out, stem + '_hip.hip'), '-o', o], check=True, timeout=300,
-                   stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
-    objs.append(o)
     exe = os.path.join(out, stem)
-    lib = os.path.join(PKG, 'lib')
-    subprocess.run(['g++', '-fopenmp', '-o', exe] + objs + ['-L' + lib, '-ldvmh_amd', '-L/opt/rocm/lib', '-lamdhip64',
-                                                           '-Wl,-rpath,' + lib, '-Wl,-rpath,/opt/rocm/lib'],
-                   check=True, timeout=120)
-    r = subprocess.run([exe], cwd=out, capture_output=True, text=True, timeout=120)
+    BRE.build(stem, exe, out, sizes)
+    return run_example(exe, out)
+
+
+def run_example(exe, cwd):
+    """Runs a built example in `cwd`: its stdout and the jacobi.dat it writes (if any)."""
+    r = subprocess.run([exe], cwd=cwd, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr
-    dat = os.path.join(out, 'jacobi.dat')
+    dat = os.path.join(cwd, 'jacobi.dat')
     return r.stdout, (np.fromfile(dat) if os.path.exists(dat) else None)
 
 
