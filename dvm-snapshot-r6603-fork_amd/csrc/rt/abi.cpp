@@ -51,7 +51,8 @@ HandlerDesc loadHandler(DvmType idx) {
     h.func = (DvmHandlerFunc)saved[idx - 1].p;
     long n = saved[idx].i;
     DVMH_CHECK(idx + 1 + n <= (DvmType)saved.size(), "Corrupted handler helper");
-    for (long i = 0; i < n; i++) h.params.push_back(saved[idx + 1 + i].p);
+    h.params.resize((size_t)n);
+    for (long i = 0; i < n; i++) h.params[(size_t)i] = saved[idx + 1 + i].p;
     return h;
 }
 
@@ -1072,8 +1073,8 @@ extern "C" void dvmh_loop_register_handler_C(DvmType curLoop, DvmType deviceType
     Loop *l = asLoop(curLoop, "dvmh_loop_register_handler_C");
     HandlerDesc h = loadHandler(handlerHelper);
     h.type = handlerType;
-    if (deviceType == DEVICE_TYPE_HOST) { l->hostH = h; l->hasHost = true; }
-    else if (deviceType == DEVICE_TYPE_CUDA) { l->gpuH = h; l->hasGpu = true; }
+    if (deviceType == DEVICE_TYPE_HOST) { l->hostH = std::move(h); l->hasHost = true; }
+    else if (deviceType == DEVICE_TYPE_CUDA) { l->gpuH = std::move(h); l->hasGpu = true; }
     else fatal("error", "Unknown device type %ld in dvmh_loop_register_handler_C", deviceType);
 }
 
