@@ -907,14 +907,17 @@ extern "C" void dvmh_region_end_C(DvmType curRegion) {   // :3122
                 ra.data->hostPieces.remove(ra.box);
             }
         }
+        // a lazy region's loops run (or replay) now, before its results are read
+        if (!r->deferred.empty()) graphRunRegion(r);
         bool needSync = false;
         for (auto &rs : r->scalars)
             if (rs.intent & INTENT_OUT) needSync = true;
         if (needSync) {
+            // the region's loops still queued in an open batch write them first
+            if (r->pendingBatch > 0) batchDrain();
             for (auto &rs : r->scalars)
                 if (rs.intent & INTENT_OUT) gpu.d2h(rs.data->host, rs.data->dev.ptr, (size_t)rs.data->typeSize);
         }
-        if (!r->deferred.empty()) graphRunRegion(r);
         if (settings.syncMode == 1 && !r->async) flushDeferredPosts();
     }
     if (!r->onGpu) {

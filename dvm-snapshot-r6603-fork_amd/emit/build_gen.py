@@ -15,11 +15,11 @@ import cdvmh  # noqa: E402
 import hipgen  # noqa: E402
 
 # programs/*.cdv that are complete C-DVMH programs: the configurations, and
-# relax2d, redall, jacg, wide, redarr, pieces, asyncreg and gauss for constructs, reduction kinds,
+# relax2d, redall, jacg, wide, redarr, pieces, asyncreg, gauss and seqpart for constructs, reduction kinds,
 # handler sizes and data movements they do not use
 # (batched.cdv sketches the batched workload's structure and is not
 # compilable C)
-CDV_PROGRAMS = ['jac2d', 'jac3d', 'sor3d', 'adi3d', 'relax2d', 'redall', 'jacg', 'wide', 'redarr', 'pieces', 'asyncreg', 'gauss']
+CDV_PROGRAMS = ['jac2d', 'jac3d', 'sor3d', 'adi3d', 'relax2d', 'redall', 'jacg', 'wide', 'redarr', 'pieces', 'asyncreg', 'gauss', 'seqpart']
 
 for spec in sorted(glob.glob(os.path.join(PKG, 'programs', '*.loops'))):
     hipgen.emit(spec, os.path.join(PKG, 'gen'))

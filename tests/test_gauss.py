@@ -70,3 +70,46 @@ def test_gauss_gpu(n):
     before = D.stats()['loops_gpu']
     check(n)
     assert D.stats()['loops_gpu'] > before
+
+
+def seqpart_reference(n, steps):
+    i, j = np.meshgrid(np.arange(n, dtype=np.float64), np.arange(n, dtype=np.float64), indexing='ij')
+    A = i + 0.5 * j
+    ts = []
+    for it in range(steps):
+        t = A[it % n, (it * 3) % n] * 2.0 + float(it)
+        A = A * 0.5 + t
+        ts.append(t)
+    return np.array(ts), A.ravel()
+
+
+def run_seqpart(n, steps):
+    f = D.load('libcdv_seqpart.so').seqpart
+    f.argtypes = [ctypes.c_long, ctypes.c_long, _dp, _dp]
+    ts = np.full(steps, np.nan)
+    out = np.zeros(n * n)
+    f(n, steps, ts.ctypes.data_as(_dp), out.ctypes.data_as(_dp))
+    return ts, out
+
+
+def check_seqpart(n, steps):
+    ts, out = run_seqpart(n, steps)
+    rts, rout = seqpart_reference(n, steps)
+    assert np.array_equal(ts.view(np.uint64), rts.view(np.uint64))
+    assert np.array_equal(out.view(np.uint64), rout.view(np.uint64))
+
+
+def test_seqpart_host_path():
+    """programs/seqpart.cdv: a sequential part writes the region's scalar,
+    which the parallel loop reads and the host reads after the region."""
+    if D.stats()['gpu_present']:
+        pytest.skip('the host path runs where no GPU is present; the GPU tests cover this case')
+    check_seqpart(7, 4)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('n', [3, 64, 257])
+def test_seqpart_gpu(n):
+    before = D.stats()['loops_gpu']
+    check_seqpart(n, 5)
+    assert D.stats()['loops_gpu'] > before
