@@ -1518,6 +1518,7 @@ extern "C" void dvmh_amd_reset_stats_C(void) {
     rt.loopsOnHost = rt.loopsOnGpu = 0;
     rt.gpuLoopSeconds = 0;
     graphReplays = batchGroups = batchLoops = 0;
+    resetPairStats();
 }
 
 extern "C" const char *dvmh_amd_device_name_C(void) {

@@ -17,6 +17,13 @@
 // a few tile-planes apart).  A chain whose loop site the tuner is still
 // exploring runs loop by loop, so that every execution is timed.
 //
+// Two consecutive queued loops of one region whose handlers have a registered
+// pair handler (a stencil loop and the loop after it, fused by the emitter)
+// form a pair; pairs of the same pair handler group like loops of one batch
+// handler, and a group of pairs -- even a single pair -- runs through the
+// pair handler.  Lazy (graph) regions run their consecutive pairs the same
+// way.
+//
 // Anything that needs the queued loops' results or would run before them
 // (a loop without a batch handler, get_actual, actual, a host region) first
 // runs the queue (batchDrain); the batch stays open.  Scalars a queued loop
@@ -36,6 +43,7 @@
 namespace dvmh {
 
 long graphReplays = 0, batchGroups = 0, batchLoops = 0;
+static long pairCalls = 0, pairsRun = 0;
 static bool batching = false;
 static bool draining = false;
 static std::vector<Loop *> queued;
@@ -47,8 +55,43 @@ struct BatchCtx {
     uint32_t magic = BATCH_MAGIC;
     std::vector<Loop *> loops;
     std::vector<SpecLoop *> specs;
+    std::vector<Loop *> loops2;       // a pair handler's call: the second loop of each pair
+    std::vector<SpecLoop *> specs2;
+    DvmHandlerFunc func = nullptr;    // the pair handler (tags its persisted copies)
+    bool declined = false;            // the pair handler ran nothing: the loops run one by one
     std::vector<void *> scratch;
 };
+
+struct PairReg {
+    DvmHandlerFunc first, second, pair;
+};
+// filled by the emitted libraries' constructors, before the runtime starts
+static std::vector<PairReg> &pairRegs() {
+    static std::vector<PairReg> v;
+    return v;
+}
+
+static bool pairMember(const Loop *loop) {
+    if (!settings.fuse) return false;
+    for (const PairReg &p : pairRegs())
+        if (p.first == loop->gpuH.func || p.second == loop->gpuH.func) return true;
+    return false;
+}
+
+DvmHandlerFunc pairHandlerFor(const Loop *a, const Loop *b) {
+    if (!settings.fuse || !a || !b || !a->region || a->region != b->region || !a->region->onGpu) return nullptr;
+    if (!a->hasGpu || !b->hasGpu || a->rank != b->rank || a->userBlockFlag || b->userBlockFlag) return nullptr;
+    if (!a->acrosses.empty() || !b->acrosses.empty() || a->iterations() == 0) return nullptr;
+    for (int i = 0; i < a->rank; i++)
+        if (a->lo[i] != b->lo[i] || a->hi[i] != b->hi[i] || a->step[i] != 1 || b->step[i] != 1) return nullptr;
+    for (auto &r : a->reds)
+        if (r->elemCount != 1) return nullptr;
+    for (auto &r : b->reds)
+        if (r->elemCount != 1) return nullptr;
+    for (const PairReg &p : pairRegs())
+        if (p.first == a->gpuH.func && p.second == b->gpuH.func) return p.pair;
+    return nullptr;
+}
 
 static BatchCtx *asBatch(DvmType ref, const char *where) {
     BatchCtx *b = (BatchCtx *)ref;
@@ -56,8 +99,12 @@ static BatchCtx *asBatch(DvmType ref, const char *where) {
     return b;
 }
 
+static bool pairMember(const Loop *loop);
+
+// GPU loops with a batch handler, or that may be half of a fused pair, wait in
+// the queue of an open batch.
 bool batchQueue(Loop *loop) {
-    if (!batching || !loop->hasBatch) return false;
+    if (!batching || !(loop->hasBatch || pairMember(loop))) return false;
     loop->queued = true;
     if (loop->region) loop->region->pendingBatch++;
     queued.push_back(loop);
@@ -139,6 +186,67 @@ static void runBatchGroup(std::vector<Loop *> &group) {
     ctx.magic = 0;
 }
 
+// Runs pairs (firsts[i], seconds[i]) through their pair handler; a handler
+// that declines leaves them to run loop by loop, in program order.
+void runPairGroup(const std::vector<Loop *> &firsts, const std::vector<Loop *> &seconds, DvmHandlerFunc func) {
+    const size_t n = firsts.size();
+    DVMH_INTERNAL(n > 0 && seconds.size() == n, "pair group of %zu / %zu loops", n, seconds.size());
+    BatchCtx ctx;
+    ctx.loops = firsts;
+    ctx.loops2 = seconds;
+    ctx.func = func;
+    for (size_t i = 0; i < n; i++) {
+        for (Loop *l : {firsts[i], seconds[i]}) {
+            for (auto &r : l->reds) r->initValues(r->arrayAddr, r->locAddr);
+            prepareLoopLayouts(l);
+        }
+        SpecLoop *a = new SpecLoop(firsts[i], 1), *b = new SpecLoop(seconds[i], 1);
+        a->stream = b->stream = gpu.stream;
+        firsts[i]->spec = a;
+        seconds[i]->spec = b;
+        ctx.specs.push_back(a);
+        ctx.specs2.push_back(b);
+    }
+    HandlerDesc h;
+    h.func = func;
+    DvmType ref = (DvmType)&ctx;
+    callHandler(h, &ref);
+    if (!ctx.declined) {
+        for (SpecLoop *sl : ctx.specs) gpuRedFinishFlush(sl);
+        for (SpecLoop *sl : ctx.specs2) gpuRedFinishFlush(sl);
+        // an explored configuration of the pair site (picked through the first
+        // loop of the first pair) is timed from its pick to here, per pair
+        SpecLoop *s0 = ctx.specs[0];
+        if (s0->explore && s0->tuneStart && !graphCapturing()) {
+            hipEvent_t ev1 = tunerEvent(1);
+            HIP_CHECK(hipEventRecord(ev1, s0->stream));
+            HIP_CHECK(hipEventSynchronize(ev1));
+            float ms = 0;
+            HIP_CHECK(hipEventElapsedTime(&ms, s0->tuneStart, ev1));
+            if (s0->candidate >= 0) tunerRecord(s0, ms * 1e-3 / (double)n);
+        }
+        rt.loopsOnGpu += 2 * (long)n;
+        pairCalls++;
+        pairsRun += (long)n;
+    }
+    for (size_t i = 0; i < n; i++)
+        for (int m = 0; m < 2; m++) {
+            SpecLoop *sl = m ? ctx.specs2[i] : ctx.specs[i];
+            Loop *l = m ? seconds[i] : firsts[i];
+            for (void *p : sl->toFree) gpu.release(p);
+            if (!ctx.declined) finishLoopReductionsExt(l);
+            l->spec = nullptr;
+            delete sl;
+        }
+    for (void *p : ctx.scratch) gpu.release(p);
+    ctx.magic = 0;
+    if (ctx.declined)
+        for (size_t i = 0; i < n; i++) {
+            performLoopNow(firsts[i]);
+            performLoopNow(seconds[i]);
+        }
+}
+
 static void releaseLoop(Loop *l) {
     Region *r = l->region;
     delete l;
@@ -171,23 +279,48 @@ static void runQueue() {
         bool chain;
         std::vector<Loop *> loops;
         bool explore = false;
+        bool pair = false;
+        std::vector<Loop *> loops2;   // pair groups: the second loop of each pair
     };
     std::vector<Group> groups;
     batchEpoch++;
-    for (Loop *l : queued) {
+    for (size_t k = 0; k < queued.size(); k++) {
+        Loop *l = queued[k];
         int target = -1;
+        DvmHandlerFunc pf = !l->batchChain && k + 1 < queued.size() ? pairHandlerFor(l, queued[k + 1]) : nullptr;
+        if (pf) {
+            // a pair: joins the last group of its pair handler under the same
+            // rule as a loop (both loops belong to one region)
+            for (int g = (int)groups.size() - 1; g >= 0; g--)
+                if (groups[g].pair && groups[g].func == pf) { target = g; break; }
+            if (target >= 0 && lastGroupOf(l) >= target) target = -1;
+            if (target < 0) {
+                groups.push_back(Group{pf, false, {}, false, true, {}});
+                target = (int)groups.size() - 1;
+            }
+            groups[target].loops.push_back(l);
+            groups[target].loops2.push_back(queued[k + 1]);
+            markGroup(l, target);
+            k++;
+            continue;
+        }
+        if (!l->hasBatch) {
+            // queued as a possible pair member, unpaired: runs on its own
+            groups.push_back(Group{nullptr, false, {l}, false, false, {}});
+            markGroup(l, (int)groups.size() - 1);
+            continue;
+        }
         if (l->batchChain) {
-            // a chain only grows at the end of the queue: program order kept
             if (!groups.empty() && groups.back().chain && groups.back().func == l->batchH.func) target = (int)groups.size() - 1;
         } else {
             for (int g = (int)groups.size() - 1; g >= 0; g--)
-                if (!groups[g].chain && groups[g].func == l->batchH.func) { target = g; break; }
+                if (!groups[g].chain && !groups[g].pair && groups[g].func == l->batchH.func) { target = g; break; }
             // joining group `target` moves l before the groups after it: allowed
             // when none of target's or the later groups' loops shares an array
             if (target >= 0 && lastGroupOf(l) >= target) target = -1;
         }
         if (target < 0) {
-            groups.push_back(Group{l->batchH.func, l->batchChain, {}, false});
+            groups.push_back(Group{l->batchH.func, l->batchChain, {}, false, false, {}});
             target = (int)groups.size() - 1;
         }
         groups[target].loops.push_back(l);
@@ -200,6 +333,17 @@ static void runQueue() {
         g.explore = g.chain && tunerSitePendingExcept(g.loops[0]->file, g.loops[0]->line, DVMH_AMD_KIND_CHAIN) > 0;
     auto runGroups = [&]() {
         for (auto &g : groups) {
+            if (g.pair) {
+                for (Loop *l : g.loops) l->queued = false;
+                for (Loop *l : g.loops2) l->queued = false;
+                runPairGroup(g.loops, g.loops2, g.func);
+                bool sync = false;
+                for (Loop *l : g.loops) sync = sync || syncAtLoopEnd(l->region);
+                for (Loop *l : g.loops) releaseLoop(l);
+                for (Loop *l : g.loops2) releaseLoop(l);
+                if (sync && !graphCapturing()) flushDeferredPosts();
+                continue;
+            }
             if (g.loops.size() == 1 || g.explore) {
                 for (Loop *l : g.loops) {
                     l->queued = false;
@@ -250,6 +394,50 @@ extern "C" void dvmh_amd_batch_end_C(void) {
 
 extern "C" DvmType dvmh_amd_batch_count_C(DvmType batchRef) {
     return (DvmType)asBatch(batchRef, "dvmh_amd_batch_count_C")->loops.size();
+}
+
+extern "C" void dvmh_amd_register_pair_C(DvmHandlerFunc first, DvmHandlerFunc second, DvmHandlerFunc pair) {
+    DVMH_CHECK(first && second && pair, "dvmh_amd_register_pair_C: null handler");
+    pairRegs().push_back(PairReg{first, second, pair});
+}
+
+extern "C" void dvmh_amd_set_fuse_C(DvmType enable) {
+    runtimeInit();
+    settings.fuse = enable != 0;
+}
+
+extern "C" void dvmh_amd_pair_stats_C(DvmType out[2]) {
+    out[0] = pairCalls;
+    out[1] = pairsRun;
+}
+
+namespace dvmh {
+void resetPairStats() { pairCalls = pairsRun = 0; }
+}
+
+static BatchCtx *asPair(DvmType ref, DvmType member, DvmType i, const char *where) {
+    BatchCtx *b = asBatch(ref, where);
+    DVMH_CHECK(!b->loops2.empty(), "%s: not called from a pair handler", where);
+    DVMH_CHECK(member == 0 || member == 1, "%s: pair member %ld", where, (long)member);
+    DVMH_CHECK(i >= 0 && i < (DvmType)b->loops.size(), "%s: pair index %ld out of range", where, (long)i);
+    return b;
+}
+
+extern "C" DvmType dvmh_amd_pair_loop_C(DvmType batchRef, DvmType member, DvmType i) {
+    BatchCtx *b = asPair(batchRef, member, i, "dvmh_amd_pair_loop_C");
+    return (DvmType)(member ? b->specs2[i] : b->specs[i]);
+}
+
+extern "C" void *dvmh_amd_pair_param_C(DvmType batchRef, DvmType member, DvmType i, DvmType p) {
+    BatchCtx *b = asPair(batchRef, member, i, "dvmh_amd_pair_param_C");
+    const auto &params = (member ? b->loops2[i] : b->loops[i])->gpuH.params;
+    DVMH_CHECK(p >= 0 && p < (DvmType)params.size(), "Pair parameter index %ld out of range", (long)p);
+    return params[p];
+}
+
+extern "C" void dvmh_amd_pair_decline_C(DvmType batchRef) {
+    BatchCtx *b = asPair(batchRef, 0, 0, "dvmh_amd_pair_decline_C");
+    b->declined = true;
 }
 
 extern "C" DvmType dvmh_amd_batch_loop_C(DvmType batchRef, DvmType i) {
@@ -347,22 +535,21 @@ extern "C" void *dvmh_amd_batch_persist_C(DvmType batchRef, const void *tag, Dvm
 }
 
 // Folds each loop's partial segment and posts the results (one kernel, one D2H).
-extern "C" void dvmh_amd_batch_red_finish_C(DvmType batchRef, DvmType redIndex, const void *partials, const void *locPartials,
-        const DvmType *offsets) {
-    BatchCtx *b = asBatch(batchRef, "dvmh_amd_batch_red_finish_C");
-    long nl = (long)b->loops.size();
+static void redFinish(DvmType batchRef, BatchCtx *b, const std::vector<Loop *> &loops, const void *tag, long slot, DvmType redIndex,
+        const void *partials, const void *locPartials, const DvmType *offsets) {
+    long nl = (long)loops.size();
     DVMH_CHECK(nl > 0, "empty batch");
-    auto &r0 = b->loops[0]->reds.at(redIndex - 1);
+    DVMH_CHECK(redIndex >= 1 && redIndex <= (DvmType)loops[0]->reds.size(), "Batch reduction index %ld out of range", redIndex);
+    auto &r0 = loops[0]->reds[redIndex - 1];
     long es = r0->elemSize, ls = r0->isLoc() ? r0->locSize : 0;
-    for (Loop *l : b->loops) {
+    for (Loop *l : loops) {
         DVMH_CHECK(redIndex >= 1 && redIndex <= (DvmType)l->reds.size(), "Batch reduction index %ld out of range", redIndex);
         auto &r = l->reds[redIndex - 1];
         DVMH_CHECK(r->func == r0->func && r->rtType == r0->rtType && r->elemCount == 1 && (r->isLoc() ? r->locSize : 0) == ls,
                 "Batched loops disagree on reduction %ld", redIndex);
     }
     // the segment offsets usually repeat from one execution of the handler to the next
-    void *dOff = dvmh_amd_batch_persist_C(batchRef, (const void *)b->loops[0]->batchH.func, 1000 + redIndex, offsets,
-            (nl + 1) * (long)sizeof(long));
+    void *dOff = dvmh_amd_batch_persist_C(batchRef, tag, slot, offsets, (nl + 1) * (long)sizeof(long));
     void *dRes = dvmh_amd_batch_scratch_C(batchRef, nl * (es + ls) + 64);
     void *dLoc = ls ? (char *)dRes + ((nl * es + 15) & ~15L) : nullptr;
     dvmh_hip_reduce_segments(r0->func, r0->rtType, nl, (const long *)dOff, partials, locPartials, ls, dRes, dLoc, gpu.stream);
@@ -370,10 +557,26 @@ extern "C" void dvmh_amd_batch_red_finish_C(DvmType batchRef, DvmType redIndex, 
     char *host = graphCapturing() ? graphPinnedSlot(total) : pinnedTake(total);
     HIP_CHECK(hipMemcpyAsync(host, dRes, total - 64, hipMemcpyDeviceToHost, gpu.stream));
     for (long i = 0; i < nl; i++) {
-        auto &r = b->loops[i]->reds[redIndex - 1];
+        auto &r = loops[i]->reds[redIndex - 1];
         char *val = host + i * es;
         char *loc = ls ? host + ((nl * es + 15) & ~15L) + i * ls : nullptr;
         if (graphCapturing()) graphNotePost(r, val, loc);
         addPendingPost(r, val, loc, false);
     }
+    (void)b;
+}
+
+extern "C" void dvmh_amd_batch_red_finish_C(DvmType batchRef, DvmType redIndex, const void *partials, const void *locPartials,
+        const DvmType *offsets) {
+    BatchCtx *b = asBatch(batchRef, "dvmh_amd_batch_red_finish_C");
+    DVMH_CHECK(!b->loops.empty(), "empty batch");
+    redFinish(batchRef, b, b->loops, (const void *)b->loops[0]->batchH.func, 1000 + (long)redIndex, redIndex, partials, locPartials,
+            offsets);
+}
+
+extern "C" void dvmh_amd_pair_red_finish_C(DvmType batchRef, DvmType member, DvmType redIndex, const void *partials,
+        const void *locPartials, const DvmType *offsets) {
+    BatchCtx *b = asPair(batchRef, member, 0, "dvmh_amd_pair_red_finish_C");
+    redFinish(batchRef, b, member ? b->loops2 : b->loops, (const void *)b->func, 2000 + 100 * (long)member + (long)redIndex, redIndex,
+            partials, locPartials, offsets);
 }

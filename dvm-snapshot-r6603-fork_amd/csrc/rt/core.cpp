@@ -62,6 +62,7 @@ void Settings::load() {
     tuneRepeats = (int)envLong("DVMH_AMD_TUNE_REPEATS", 1);
     if (tuneRepeats < 1) tuneRepeats = 1;
     tuneTransforms = envLong("DVMH_AMD_TUNE_TRANSFORMS", 1) != 0;
+    fuse = envLong("DVMH_AMD_FUSE", 1) != 0;
     index64 = envLong("DVMH_AMD_INDEX64", 0) != 0;
     const char *db = getenv("DVMH_AMD_TUNEDB");
     tuneDb = db ? db : "";

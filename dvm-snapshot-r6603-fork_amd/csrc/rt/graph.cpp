@@ -72,6 +72,7 @@ static void hashBytes(std::ostringstream &o, const void *p, size_t n) {
 
 static std::string signature(Region *region) {
     std::ostringstream o;
+    o << "F" << settings.fuse << ";";   // pairs run fused or not
     for (auto &ra : region->arrays) {
         Data *d = ra.data;
         o << "A" << (const void *)d << ":" << d->dev.ptr << ":" << d->dev.diagState;
@@ -111,13 +112,27 @@ static bool tunerSettled(Region *region) {
     return true;
 }
 
-static void runEager(Region *region) {
-    for (Loop *l : region->deferred) {
-        performLoopNow(l);
-        delete l;
+// The region's queued loops in program order; consecutive loops with a
+// registered pair handler run fused.
+static void runDeferred(Region *region) {
+    std::vector<Loop *> &q = region->deferred;
+    for (size_t k = 0; k < q.size(); k++) {
+        DvmHandlerFunc pf = k + 1 < q.size() ? pairHandlerFor(q[k], q[k + 1]) : nullptr;
+        if (!pf) {
+            performLoopNow(q[k]);
+            delete q[k];
+            continue;
+        }
+        runPairGroup({q[k]}, {q[k + 1]}, pf);
+        if (syncAtLoopEnd(region) && !graphCapturing()) flushDeferredPosts();
+        delete q[k];
+        delete q[k + 1];
+        k++;
     }
-    region->deferred.clear();
+    q.clear();
 }
+
+static void runEager(Region *region) { runDeferred(region); }
 
 // Called by dvmh_region_end_C for a lazy region with queued loops.
 void graphRunRegion(Region *region) {
@@ -165,11 +180,7 @@ void graphRunRegion(Region *region) {
     long before = layoutChanges();
     HIP_CHECK(hipStreamBeginCapture(gpu.stream, hipStreamCaptureModeRelaxed));
     size_t nLoops = region->deferred.size();
-    for (Loop *l : region->deferred) {
-        performLoopNow(l);
-        delete l;
-    }
-    region->deferred.clear();
+    runDeferred(region);
     HIP_CHECK(hipStreamEndCapture(gpu.stream, &g.graph));
     capturing = false;
     settings.syncMode = savedSync;
@@ -215,6 +226,7 @@ static void destroyEntry(GraphEntry &g) {
 static std::vector<long> batchSignature(const std::vector<Loop *> &loops) {
     std::vector<long> w;
     w.reserve(loops.size() * 32);
+    w.push_back(settings.fuse);   // pairs run fused or not
     auto bytes = [&w](const std::vector<char> &v) {
         w.push_back((long)v.size());
         for (size_t i = 0; i < v.size(); i += sizeof(long)) {

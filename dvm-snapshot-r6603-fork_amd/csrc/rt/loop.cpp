@@ -327,6 +327,8 @@ static void prepareLayouts(Loop *loop) {
     }
 }
 
+void prepareLoopLayouts(Loop *loop) { prepareLayouts(loop); }
+
 static void restoreNaturalLayouts(Loop *loop) {
     if (!loop->region) return;
     for (auto &ra : loop->region->arrays) {

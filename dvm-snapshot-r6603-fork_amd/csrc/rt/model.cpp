@@ -54,7 +54,7 @@ double sitePoints(const SiteInfo &s) {
 void features(const SiteInfo &s, const Candidate &c, float x[F]) {
     for (int f = 0; f < F; f++) x[f] = 0.f;
     if (s.kind >= 0 && s.kind <= 2) x[s.kind] = 1.f;
-    if (s.kind == DVMH_AMD_KIND_STENCIL || s.kind == DVMH_AMD_KIND_BATCH)
+    if (s.kind == DVMH_AMD_KIND_STENCIL || s.kind == DVMH_AMD_KIND_BATCH || s.kind == DVMH_AMD_KIND_PAIR)
         x[DVMH_AMD_KIND_PARALLEL] = 1.f;   // a PARALLEL loop; the variant tells the kernel
     x[3] = (float)s.rank;
     x[4] = (float)s.depCount;
@@ -80,7 +80,7 @@ void features(const SiteInfo &s, const Candidate &c, float x[F]) {
     // iteration points one workgroup covers per launch, and the resulting
     // number of workgroups (occupancy of 256 CUs)
     double perWg;
-    if (c.tune[DVMH_AMD_T_VARIANT] == 1 && s.kind == DVMH_AMD_KIND_STENCIL) {
+    if (c.tune[DVMH_AMD_T_VARIANT] == 1 && (s.kind == DVMH_AMD_KIND_STENCIL || s.kind == DVMH_AMD_KIND_PAIR)) {
         perWg = threads * (double)std::max(1L, (long)c.tune[DVMH_AMD_T_TILE_A]) * (double)std::max(1L, (long)c.tune[DVMH_AMD_T_CHUNK]);
     } else if (s.kind == DVMH_AMD_KIND_PARALLEL || s.kind == DVMH_AMD_KIND_STENCIL || s.kind == DVMH_AMD_KIND_BATCH) {
         perWg = threads * (double)depth;

@@ -56,6 +56,7 @@ struct Settings {
     int tuneRepeats = 1;        // DVMH_AMD_TUNE_REPEATS: timed executions per candidate
     bool pollDone = true;          // DVMH_AMD_POLL: wait for reduction results on the completion word, not a stream sync
     bool tuneTransforms = true;    // DVMH_AMD_TUNE_TRANSFORMS (default 1): candidates may re-lay out arrays of ACROSS loops
+    bool fuse = true;              // DVMH_AMD_FUSE (default 1): registered loop pairs held back together run fused
     bool index64 = false;          // DVMH_AMD_INDEX64: every loop's kernels take 64-bit index arithmetic
     std::string tuneDb;         // DVMH_AMD_TUNEDB: file of recorded timings
     std::string modelFile;      // DVMH_AMD_MODEL: trained cost-model weights
@@ -373,6 +374,11 @@ void graphRunRegion(Region *region);
 // Runs a batch queue through runGroups, or replays the graph captured from an
 // identical earlier queue (returns true then); see graph.cpp.
 bool batchGraphRun(const std::vector<Loop *> &loops, bool eligible, const std::function<void()> &runGroups);
+// fused loop pairs (batch.cpp): the pair handler for a followed by b, or null
+DvmHandlerFunc pairHandlerFor(const Loop *a, const Loop *b);
+void runPairGroup(const std::vector<Loop *> &firsts, const std::vector<Loop *> &seconds, DvmHandlerFunc func);
+void prepareLoopLayouts(Loop *loop);
+void resetPairStats();
 void finishLoopReductionsExt(Loop *loop);
 void graphReleaseAll();
 void runtimeInit();   // settings, GPU and tuner set-up on first use

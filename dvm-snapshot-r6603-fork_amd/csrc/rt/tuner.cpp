@@ -143,6 +143,32 @@ std::vector<Candidate> makeCandidates(int kind, int loopRank, int depCount, cons
                         addCand(v, tl, 1, 1, {{DVMH_AMD_T_VARIANT, 1}, {DVMH_AMD_T_TILE_A, ck}, {DVMH_AMD_T_TILE_B, reg}});
                 }
         }
+    } else if (kind == DVMH_AMD_KIND_PAIR) {
+        // a fused pair runs the LDS march of its stencil loop, the second loop
+        // inside it off the tile edges and on the edges in a second kernel:
+        // taller tiles leave fewer edge points.  Default: 128 threads x 2
+        // points (rank 2) or 64 x 4 threads x 2 points (rank 3), 32 planes.
+        if (loopRank == 3) addCand(v, 64, 4, 1, {{DVMH_AMD_T_VARIANT, 1}, {DVMH_AMD_T_TILE_A, 2}, {DVMH_AMD_T_CHUNK, 32}});
+        else addCand(v, 128, 1, 1, {{DVMH_AMD_T_VARIANT, 1}, {DVMH_AMD_T_TILE_A, 2}, {DVMH_AMD_T_CHUNK, 32}});
+        const int mxs[] = {64, 128, 256, 512, 1024};
+        const int mys[] = {1, 2, 4, 8};
+        const int mcs[] = {1, 2, 4};
+        const int marches[] = {8, 16, 32, 64};
+        for (int bx : mxs)
+            for (int by : mys) {
+                if (loopRank == 2 && by > 1) continue;
+                if (loopRank == 3 && (by < 2 || bx > 256)) continue;   // an edge row per tile row otherwise
+                int n = bx * by;
+                if (n < 64 || n > 1024) continue;
+                for (int c : mcs) {
+                    long rows = loopRank == 3 ? by + 2 : 1;
+                    if (3 * rows * ((long)bx * c + 2) * 8 > 64512) continue;
+                    for (int m : marches)
+                        for (int xcd = 0; xcd <= 1; xcd++)
+                            addCand(v, bx, by, 1, {{DVMH_AMD_T_VARIANT, 1}, {DVMH_AMD_T_TILE_A, c}, {DVMH_AMD_T_CHUNK, m},
+                                                   {DVMH_AMD_T_XCD, xcd}});
+                }
+            }
     } else if (kind == DVMH_AMD_KIND_CHAIN) {
         // a chain of sweeps of a tiled-wavefront loop: tile shapes only (the
         // chain handler runs nothing else); times are per sweep of the chain
@@ -515,8 +541,9 @@ static DvmType tunerQuery(const char *file, DvmType line, int kind, DvmType out[
     std::string prefix = std::string(file) + ":" + std::to_string(line) + "|";
     for (auto &kv : sites) {
         if (kv.first.compare(0, prefix.size(), prefix) != 0) continue;
-        if (kv.second.kind == DVMH_AMD_KIND_CHAIN && kind != DVMH_AMD_KIND_CHAIN) continue;
-        if (kind == DVMH_AMD_KIND_CHAIN && kv.second.kind != DVMH_AMD_KIND_CHAIN) continue;
+        // a loop's own sites, or the chain or pair site asked for
+        const bool own = kv.second.kind != DVMH_AMD_KIND_CHAIN && kv.second.kind != DVMH_AMD_KIND_PAIR;
+        if (kind < 0 ? !own : kv.second.kind != kind) continue;
         int b = kv.second.bestMeasured();
         if (b < 0) return 0;
         const Candidate &c = kv.second.cands[b];
@@ -545,7 +572,8 @@ extern "C" DvmType dvmh_amd_tuner_transform_stats_C(const char *file, DvmType li
     out[2] = out[3] = -1;
     long seen = 0;
     for (auto &kv : sites) {
-        if (kv.first.compare(0, prefix.size(), prefix) != 0 || kv.second.kind == DVMH_AMD_KIND_CHAIN) continue;
+        if (kv.first.compare(0, prefix.size(), prefix) != 0 || kv.second.kind == DVMH_AMD_KIND_CHAIN ||
+                kv.second.kind == DVMH_AMD_KIND_PAIR) continue;
         seen++;
         const SiteInfo &si = kv.second;
         for (size_t c = 0; c < si.cands.size(); c++) {
@@ -565,4 +593,9 @@ extern "C" DvmType dvmh_amd_tuner_transform_stats_C(const char *file, DvmType li
 extern "C" DvmType dvmh_amd_tuner_query_chain_C(const char *file, DvmType line, DvmType out[13]) {
     runtimeInit();
     return tunerQuery(file, line, DVMH_AMD_KIND_CHAIN, out);
+}
+
+extern "C" DvmType dvmh_amd_tuner_query_pair_C(const char *file, DvmType line, DvmType out[13]) {
+    runtimeInit();
+    return tunerQuery(file, line, DVMH_AMD_KIND_PAIR, out);
 }

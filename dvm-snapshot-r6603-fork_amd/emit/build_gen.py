@@ -19,7 +19,7 @@ import hipgen  # noqa: E402
 # handler sizes and data movements they do not use
 # (batched.cdv sketches the batched workload's structure and is not
 # compilable C)
-CDV_PROGRAMS = ['jac2d', 'jac3d', 'sor3d', 'adi3d', 'relax2d', 'redall', 'jacg', 'wide', 'redarr', 'pieces', 'asyncreg', 'gauss', 'seqpart']
+CDV_PROGRAMS = ['jac2d', 'jac3d', 'sor3d', 'adi3d', 'relax2d', 'redall', 'jacg', 'wide', 'redarr', 'pieces', 'asyncreg', 'gauss', 'seqpart', 'pairs']
 
 for spec in sorted(glob.glob(os.path.join(PKG, 'programs', '*.loops'))):
     hipgen.emit(spec, os.path.join(PKG, 'gen'))

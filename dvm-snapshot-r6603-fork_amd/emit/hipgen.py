@@ -687,14 +687,19 @@ def parallel_kernel(L):
 MARCH_NPF = {1: [2, 3, 4, 5], 2: [3, 4, 5, 7, 8], 3: [4, 5, 7, 10, 11], 4: [5, 7, 10, 14]}
 
 
-def march_kernel(L):
+def march_kernel(L, tail=None):
     """LDS march variant of a PARALLEL loop with stencil arrays (rank 2 or 3,
     unit steps).  A workgroup owns a tile of the inner axes -- blockDim.x * COLS
     points of the innermost axis (x), blockDim.y of the middle one for rank 3 --
     and marches `march` planes along the outermost axis.  Each stencil array is
     staged in LDS as a rolling window of three planes (tile plus a halo of one),
     every element loaded from HBM once per tile; the body reads it through a
-    MarchView with the same A[i][j][k] syntax, other arrays stay in HBM."""
+    MarchView with the same A[i][j][k] syntax, other arrays stay in HBM.
+
+    With `tail` (a loop fused after L, see pair_eligible), only the tile
+    function is returned, named after the pair: after L's body, each point not
+    on an edge of its tile that faces another tile runs the tail loop's body at
+    the same point (pair_tail)."""
     r = L.rank
     ax_x, ax_y = r - 1, (r - 2 if r == 3 else None)
     fp, setup = [], []
@@ -721,8 +726,12 @@ def march_kernel(L):
         tp.append(f'{red[2]} (&{red[1]})[{n}]' if n > 1 else f'{red[2]} &{red[1]}')
         if len(red) > 3:
             tp.append(f'{red[4]} &{red[3]}')
+    fname = f'{L.name}_march_tile'
+    if tail is not None:
+        tp += pair_tail_params(L, tail)
+        fname = f'{pair_name(L, tail)}_pair_tile'
     k = ['template <typename IndexT, int COLS, int NPF>',
-         f'__device__ static inline void {L.name}_march_tile({", ".join(tp)}) {{']
+         f'__device__ static inline void {fname}({", ".join(tp)}) {{']
     b = ['extern __shared__ __align__(16) unsigned char dvmh_lds[];']
     b += setup
     for v, ct in L.vars:
@@ -852,6 +861,8 @@ def march_kernel(L):
     for i in range(r):
         pt.append(IND + var_assign(L, i, f't{i}', f'lo{i}', '1'))   # the variant runs unit steps only
     pt.append(indent('{\n' + indent(L.body + '\n', 1) + '}', 1))
+    if tail is not None:
+        pt += [IND + ln for ln in pair_tail(L, tail, ax_x, ax_y)]
     pt.append('}')
     inner += [IND + ln for ln in '\n'.join(pt).split('\n')]
     inner.append('}')
@@ -860,6 +871,8 @@ def march_kernel(L):
     b.append('}')
     k.append(indent('\n'.join(b) + '\n', 1).rstrip('\n'))
     k.append('}')
+    if tail is not None:
+        return '\n'.join(k) + '\n'
     # the per-loop kernel: one tile per workgroup
     names = [p_[1] for p_ in L.params]
     kp = list(fp[:-4]) + ['IndexT nbx', 'IndexT nby', 'IndexT nbz', 'long nblocks', 'int xcd', 'IndexT march'] + red_kernel_params(L)
@@ -1370,6 +1383,526 @@ def batch_handler(L):
     h.append(indent('\n'.join(b) + '\n', 1).rstrip('\n'))
     h.append('}')
     return '\n'.join(h) + '\n'
+
+
+# ----------------------------------------------------------------- GPU: fused loop pairs
+# A PARALLEL stencil loop L1 directly followed (same region, same bounds) by a
+# PARALLEL loop L2 that touches every array only at the loop point -- a time
+# step's update loop and its copy-back / residual loop -- can run as one
+# pass: each point runs L1's body and then L2's.  L1 reads its stencil arrays
+# through the LDS march, loaded before anything is written; L2 may overwrite
+# such an array at the point only where no other tile reads it, i.e. off the
+# tile edges that face another tile.  Points on those edges run L2 in a second
+# kernel after the first has finished (every read of the old values is done
+# then).  Sequential semantics are kept: L2 at point p sees L1's results at p
+# (same thread), and every read L1 makes of an array L2 writes sees the value
+# from before L2.
+
+WRITE_OP = r'(?:=(?!=)|\+=|-=|\*=|/=|%=|&=|\|=|\^=|<<=|>>=|\+\+|--)'
+
+
+def array_refs(L, name):
+    """Offsets (per loop axis) and write flag of every access name[...] in L's
+    body, or None when some access is not name[v0 + c0][v1 + c1]... with the
+    loop's own variables in order, or the name is used any other way."""
+    body = L.body
+    if re.search(r'\b' + name + r'\b(?!\s*\[)', body) or re.search(r'(\+\+|--)\s*' + name + r'\b', body):
+        return None
+    refs = []
+    for m in re.finditer(r'\b' + name + r'\s*((?:\[[^\[\]]*\]\s*)+)(\s*' + WRITE_OP + r')?', body):
+        idx = re.findall(r'\[([^\[\]]*)\]', m.group(1))
+        if len(idx) != L.rank:
+            return None
+        offs = []
+        for (v, _), e in zip(L.vars, idx):
+            mm = re.fullmatch(r'\s*' + re.escape(v) + r'\s*(?:([+-])\s*(\d+)\s*)?', e)
+            if not mm:
+                return None
+            offs.append(0 if not mm.group(1) else int(mm.group(2)) * (1 if mm.group(1) == '+' else -1))
+        refs.append((offs, bool(m.group(2))))
+    # every name[ must have been matched (nested brackets are not)
+    if len(re.findall(r'\b' + name + r'\s*\[', body)) != len(refs):
+        return None
+    return refs
+
+
+def pair_eligible(L1, L2):
+    """Whether L2 may run fused after L1 (see above): both PARALLEL on the GPU
+    with the same rank (2 or 3), L1 with stencil arrays it only reads, every
+    other array access of L1 and every array access of L2 at offset 0, scalar
+    reductions whose variables L1 does not name, and names that mean the same
+    kind of object in both loops.  The pair handler checks at run time that
+    equally named arrays are the same array and differently named ones are
+    not, and that the bounds agree."""
+    for L in (L1, L2):
+        if L.across or L.sequential or L.host_only or L.rank not in (2, 3):
+            return False
+        if any(p[0] not in ('array', 'scalar') for p in L.params):
+            return False
+        if any(L.rlen(red) > 1 for red in L.reds):
+            return False
+    if not L1.stencil or L1.rank != L2.rank:
+        return False
+    kinds1 = {p[1]: p for p in L1.params}
+    for p in L2.params:
+        q = kinds1.get(p[1])
+        if q is not None and tuple(q) != tuple(p):
+            return False
+    for L, stencil in ((L1, L1.stencil), (L2, [])):
+        for p in L.arrays:
+            if p[3] != L.rank:
+                return False
+            refs = array_refs(L, p[1])
+            if refs is None:
+                return False
+            if p[1] in stencil:
+                continue
+            if any(any(o != 0 for o in offs) for offs, _ in refs):
+                return False
+    names1 = set(kinds1) | {v for v, _ in L1.vars}
+    for red in L1.reds:
+        names1.add(red[1])
+        if len(red) > 3:
+            names1.add(red[3])
+    for red in L2.reds:
+        if red[1] in names1 or (len(red) > 3 and red[3] in names1):
+            return False
+    return True
+
+
+def pair_name(L1, L2):
+    return f'{L1.name}__{L2.name}'
+
+
+def pair_tail_params(L1, L2):
+    """Extra parameters of the pair's tile function: L2's arrays and scalars
+    that L1 does not have, and L2's reduction variables."""
+    n1 = {p[1] for p in L1.params}
+    tp = []
+    for p in L2.arrays:
+        _, nm, ct, ar = p
+        if nm in n1:
+            continue
+        tp.append(f'{ct} *__restrict__ p2_{nm}_base')
+        tp += [f'IndexT p2_{nm}_s{k}' for k in range(ar)]
+    for p in L2.scalars:
+        if p[1] not in n1:
+            tp.append(f'{p[2]} *__restrict__ p2_{p[1]}_ptr')
+    for red in L2.reds:
+        tp.append(f'{red[2]} &p2_{red[1]}')
+        if len(red) > 3:
+            tp.append(f'{red[4]} &p2_{red[3]}')
+    return tp
+
+
+def pair_tail_scope(L1, L2, tname):
+    """L2's names bound for one point (in a scope of their own, so they may
+    shadow L1's), its loop variables from the point's offsets tname(axis), then
+    its body.  Arrays L1 also has use L1's base (the same array)."""
+    n1 = {p[1] for p in L1.params}
+    s = []
+    for p in L2.arrays:
+        _, nm, ct, ar = p
+        pre = '' if nm in n1 else 'p2_'
+        s.append(f'const IndexT p2_{nm}_str[{ar}] = {{{", ".join(f"{pre}{nm}_s{k}" for k in range(ar))}}};')
+        s.append(f'const auto {nm} = dvmh_dev::makeView<{ct}, IndexT, {ar}, true>({pre}{nm}_base, p2_{nm}_str);')
+    for p in L2.scalars:
+        pre = '' if p[1] in n1 else 'p2_'
+        s.append(f'{p[2]} &{p[1]} = *{pre}{p[1]}_ptr;')
+    for red in L2.reds:
+        s.append(f'{red[2]} &{red[1]} = p2_{red[1]};')
+        if len(red) > 3:
+            s.append(f'{red[4]} &{red[3]} = p2_{red[3]};')
+    for v, ct in L2.vars:
+        s.append(f'{ct} {v};')
+    for pr in L2.privates:
+        s.append(pr + ';')
+    for i in range(L2.rank):
+        s.append(var_assign(L2, i, tname(i), f'lo{i}', '1'))
+    s.append('{\n' + indent(L2.body + '\n', 1) + '}')
+    return s
+
+
+def pair_tail(L1, L2, ax_x, ax_y):
+    """Inside the march's point loop: L2 at the point unless it lies on a tile
+    edge facing another tile (those run in the pair's border kernel)."""
+    edge = ['(d == 0 && bz > 0)', '(d == march - 1 && bz + 1 < nbz)',
+            f'(t{ax_x} == tx0 && bx > 0)', f'(t{ax_x} == tx0 + tileX - 1 && bx + 1 < nbx)']
+    if ax_y is not None:
+        edge += ['(threadIdx.y == 0 && by > 0)', '(threadIdx.y == blockDim.y - 1 && by + 1 < nby)']
+    s = ['// the fused loop at this point, unless another tile reads it',
+         f'if (!({" || ".join(edge)})) {{']
+    s += [IND + ln for ln in '\n'.join(pair_tail_scope(L1, L2, lambda i: f't{i}')).split('\n')]
+    s.append('}')
+    return s
+
+
+def pair_desc_layout(L1, L2):
+    """Per pair: L1's batch descriptor, then L2's, then the start of the pair's
+    segment of L2's reduction slots and its number of march tiles."""
+    lay1, n1 = batch_desc_layout(L1)
+    lay2, n2 = batch_desc_layout(L2)
+    lay2 = {k: v + n1 for k, v in lay2.items()}
+    ext = {'seg2': n1 + n2, 'ntiles': n1 + n2 + 1}
+    return lay1, lay2, ext, n1 + n2 + 2
+
+
+class _Renamed:
+    """A loop's reductions under prefixed names (for red_kernel_tail)."""
+    def __init__(self, L, pre):
+        self.reds = [tuple([r[0], pre + r[1], r[2]] + ([pre + r[3], r[4]] if len(r) > 3 else [])) for r in L.reds]
+
+    def rlen(self, red):
+        return 1
+
+
+def pair_kernels(L1, L2):
+    """The pair's march kernel (all pairs' tiles, L1 then L2 per interior
+    point) and its border kernel (L2 at the tile-edge points)."""
+    r = L1.rank
+    P = pair_name(L1, L2)
+    lay1, lay2, ext, dlen = pair_desc_layout(L1, L2)
+    n1 = {p[1] for p in L1.params}
+    R2 = _Renamed(L2, 'p2_')
+    grids = []
+    for red in L1.reds:
+        grids.append(f'{red[2]} *{red[1]}_grid')
+        if len(red) > 3:
+            grids.append(f'{red[4]} *{red[3]}_grid')
+    grids2 = []
+    for red in R2.reds:
+        grids2.append(f'{red[2]} *{red[1]}_grid')
+        if len(red) > 3:
+            grids2.append(f'{red[4]} *{red[3]}_grid')
+    fp = ['const long *__restrict__ desc', 'const long *__restrict__ tileStart', 'long npairs', 'long totalTiles', 'IndexT march',
+          'int xcd'] + grids + grids2
+    k = ['template <typename IndexT, int COLS, int NPF>',
+         f'__global__ __launch_bounds__(1024) void {P}_pair_kernel({", ".join(fp)}) {{']
+    b = ['for (long t_ = blockIdx.x; t_ < totalTiles; t_ += gridDim.x) {']
+    t = ['const long tile = dvmh_dev::xcdTile(t_, totalTiles, xcd);',
+         'long lo_ = 0, hi_ = npairs - 1;',
+         'while (lo_ < hi_) { long mid = (lo_ + hi_ + 1) >> 1; if (tileStart[mid] <= tile) lo_ = mid; else hi_ = mid - 1; }',
+         f'const long *d = desc + lo_ * {dlen};']
+    args = []
+    for p in L1.arrays:
+        _, nm, ct, ar = p
+        so = lay1[('str', nm)]
+        args.append(f'dvmh_dev::globalPtr<{ct}>(d[{lay1[("base", nm)]}])')
+        args += [f'(IndexT)d[{so + q}]' for q in range(ar)]
+    for p in L1.scalars:
+        args.append(f'dvmh_dev::globalPtr<{p[2]}>(d[{lay1[("sc", p[1])]}])')
+    bo = lay1['bounds']
+    for i in range(r):
+        args += [f'(IndexT)d[{bo + 3 * i}]', f'(IndexT)d[{bo + 3 * i + 1}]', f'(IndexT)d[{bo + 3 * i + 2}]']
+    nbo = lay1['nb']
+    args += [f'(IndexT)d[{nbo}]', f'(IndexT)d[{nbo + 1}]', f'(IndexT)d[{nbo + 2}]', 'march', 'tile - tileStart[lo_]']
+    for red in L1.reds:
+        t.append(f'{red[2]} {red[1]} = *(const {red[2]} *)&d[{lay1[("ri", red[1])]}];')
+        args.append(red[1])
+        if len(red) > 3:
+            t.append(f'{red[4]} {red[3]} = *(const {red[4]} *)&d[{lay1[("ri", red[3])]}];')
+            args.append(red[3])
+    for p in L2.arrays:
+        _, nm, ct, ar = p
+        if nm in n1:
+            continue
+        so = lay2[('str', nm)]
+        args.append(f'dvmh_dev::globalPtr<{ct}>(d[{lay2[("base", nm)]}])')
+        args += [f'(IndexT)d[{so + q}]' for q in range(ar)]
+    for p in L2.scalars:
+        if p[1] not in n1:
+            args.append(f'dvmh_dev::globalPtr<{p[2]}>(d[{lay2[("sc", p[1])]}])')
+    for red in L2.reds:
+        t.append(f'{red[2]} p2_{red[1]} = *(const {red[2]} *)&d[{lay2[("ri", red[1])]}];')
+        args.append(f'p2_{red[1]}')
+        if len(red) > 3:
+            t.append(f'{red[4]} p2_{red[3]} = *(const {red[4]} *)&d[{lay2[("ri", red[3])]}];')
+            args.append(f'p2_{red[3]}')
+    t.append(f'{P}_pair_tile<IndexT, COLS, NPF>({", ".join(args)});')
+    t += red_kernel_tail(L1, 'tile', accumulate=False)
+    t += red_kernel_tail(R2, f'd[{ext["seg2"]}] + (tile - tileStart[lo_])', accumulate=False)
+    b += [IND + ln for ln in '\n'.join(t).split('\n')]
+    b.append('}')
+    k.append(indent('\n'.join(b) + '\n', 1).rstrip('\n'))
+    k.append('}')
+    # ---- border kernel: L2 at the points on tile edges that face another tile.
+    # Per axis a (march axis, then y for rank 3, then x) with tile size T and n
+    # tiles, the edge indices are the 2 (n - 1) indices next to a boundary
+    # between tiles; slab a lists the points whose a-index is one of them and
+    # whose earlier axes' indices are not (each edge point once).
+    ax = [0, 1, 2] if r == 3 else [0, 1]
+    fpb = ['const long *__restrict__ desc', 'const long *__restrict__ blkStart', 'long npairs', 'long totalBlocks', 'IndexT march',
+           'IndexT tileY', 'IndexT tileX']
+    for red in L2.reds:
+        fpb.append(f'{red[2]} *{red[1]}_grid')
+        if len(red) > 3:
+            fpb.append(f'{red[4]} *{red[3]}_grid')
+    g = ['template <typename IndexT>',
+         f'__global__ __launch_bounds__(256) void {P}_pair_border_kernel({", ".join(fpb)}) {{']
+    gb = ['for (long blk = blockIdx.x; blk < totalBlocks; blk += gridDim.x) {']
+    u = ['long lo_ = 0, hi_ = npairs - 1;',
+         'while (lo_ < hi_) { long mid = (lo_ + hi_ + 1) >> 1; if (blkStart[mid] <= blk) lo_ = mid; else hi_ = mid - 1; }',
+         f'const long *d = desc + lo_ * {dlen};']
+    u += red_kernel_locals_from(L2, lay2)
+    bo2 = lay2['bounds']
+    for i in range(r):
+        u.append(f'const IndexT lo{i} = (IndexT)d[{bo2 + 3 * i}], cnt{i} = (IndexT)d[{bo2 + 3 * i + 1}];')
+    # tile sizes and counts per axis (the counts are L1's nbx, nby, nbz)
+    tsz = {0: 'march', ax[-1]: 'tileX'}
+    tcnt = {0: f'd[{nbo + 2}]', ax[-1]: f'd[{nbo}]'}
+    if r == 3:
+        tsz[1] = 'tileY'
+        tcnt[1] = f'd[{nbo + 1}]'
+    for a in ax:
+        u.append(f'const long T{a} = (long){tsz[a]}, n{a} = {tcnt[a]}, r{a} = n{a} > 1 ? 2 * (n{a} - 1) : 0;')
+    u.append('const long e = (blk - blkStart[lo_]) * (long)blockDim.x + (long)threadIdx.x;')
+    u.append('// the index next to boundary u / 2 between tiles: its last (u even) or first (u odd) index')
+    u.append('auto edgeAt = [](long uu, long T) -> long { return (uu & 1) ? ((uu >> 1) + 1) * T : ((uu >> 1) + 1) * T - 1; };')
+    u.append('auto onEdge = [](long tt, long T, long n) -> bool { const long q = tt / T, m = tt - q * T; return (m == 0 && q > 0) || (m == T - 1 && q + 1 < n); };')
+    u.append('bool valid = false;')
+    u.append(' '.join(f'long p{a} = 0;' for a in ax))
+    ext_cnt = {a: f'(long)cnt{a}' for a in ax}
+    start = '0'
+    for si, a in enumerate(ax):
+        dims = [(f'r{a}' if b == a else ext_cnt[b]) for b in ax]
+        size = ' * '.join(dims)
+        cond = 'if' if si == 0 else 'else if'
+        u.append(f'{cond} (e < {start} + {size}) {{')
+        u.append(IND + f'long f = e - ({start});')
+        # decode with the innermost axis fastest
+        for bi in reversed(range(len(ax))):
+            b_ = ax[bi]
+            dim = dims[bi]
+            if bi > 0:
+                u.append(IND + f'const long c{b_} = f % ({dim}); f /= ({dim});')
+            else:
+                u.append(IND + f'const long c{b_} = f;')
+        for b_ in ax:
+            u.append(IND + (f'p{b_} = edgeAt(c{b_}, T{b_});' if b_ == a else f'p{b_} = c{b_};'))
+        earlier = [f'!onEdge(p{b_}, T{b_}, n{b_})' for b_ in ax[:si]]
+        u.append(IND + f'valid = {" && ".join(earlier) if earlier else "true"};')
+        u.append('}')
+        start = f'{start} + {size}' if start != '0' else size
+    # L2's arrays and scalars from the descriptor, under names of the kernel's
+    # own, then bound to L2's names in the point's scope
+    for p in L2.arrays:
+        _, nm, ct, ar = p
+        so = lay2[('str', nm)]
+        u.append(f'{ct} *const dvmhp_{nm} = dvmh_dev::globalPtr<{ct}>(d[{lay2[("base", nm)]}]);')
+        u.append(f'const IndexT dvmhp_{nm}_str[{ar}] = {{{", ".join(f"(IndexT)d[{so + q}]" for q in range(ar))}}};')
+    for p in L2.scalars:
+        u.append(f'{p[2]} *const dvmhp_{p[1]} = dvmh_dev::globalPtr<{p[2]}>(d[{lay2[("sc", p[1])]}]);')
+    u.append('if (valid) {')
+    for a in ax:
+        u.append(IND + f'const IndexT t{a} = (IndexT)p{a};')
+    sc = []
+    for p in L2.arrays:
+        _, nm, ct, ar = p
+        sc.append(f'const auto {nm} = dvmh_dev::makeView<{ct}, IndexT, {ar}, true>(dvmhp_{nm}, dvmhp_{nm}_str);')
+    for p in L2.scalars:
+        sc.append(f'{p[2]} &{p[1]} = *dvmhp_{p[1]};')
+    for v, ct in L2.vars:
+        sc.append(f'{ct} {v};')
+    for pr in L2.privates:
+        sc.append(pr + ';')
+    for i in range(r):
+        sc.append(var_assign(L2, i, f't{i}', f'lo{i}', '1'))
+    sc.append('{\n' + indent(L2.body + '\n', 1) + '}')
+    u += [IND + ln for ln in '\n'.join(sc).split('\n')]
+    u.append('}')
+    u += red_kernel_tail(L2, f'd[{ext["seg2"]}] + d[{ext["ntiles"]}] + (blk - blkStart[lo_])', accumulate=False)
+    gb += [IND + ln for ln in '\n'.join(u).split('\n')]
+    gb.append('}')
+    g.append(indent('\n'.join(gb) + '\n', 1).rstrip('\n'))
+    g.append('}')
+    return '\n'.join(k) + '\n\n' + '\n'.join(g) + '\n'
+
+
+def red_kernel_locals_from(L, lay):
+    """Reduction variables of L initialised from a descriptor's init words."""
+    s = []
+    for red in L.reds:
+        s.append(f'{red[2]} {red[1]} = *(const {red[2]} *)&d[{lay[("ri", red[1])]}];')
+        if len(red) > 3:
+            s.append(f'{red[4]} {red[3]} = *(const {red[4]} *)&d[{lay[("ri", red[3])]}];')
+    return s
+
+
+def pair_border_count(r):
+    """Host text: edge points of one pair (see the border kernel), from cnt0..,
+    the tile sizes T0 (march), T1 (rows, rank 3), Tx and tile counts nb*."""
+    ax = [0, 1, 2] if r == 3 else [0, 1]
+    tcnt = {0: 'nbz', ax[-1]: 'nbx'}
+    if r == 3:
+        tcnt[1] = 'nby'
+    s = []
+    for a in ax:
+        s.append(f'long re{a} = {tcnt[a]} > 1 ? 2 * ({tcnt[a]} - 1) : 0;')
+    terms = []
+    for a in ax:
+        terms.append(' * '.join(f're{b}' if b == a else f'cnt{b}' for b in ax))
+    s.append(f'long edges = {" + ".join(terms)};')
+    return s
+
+
+def pair_handler(L1, L2):
+    r = L1.rank
+    P = pair_name(L1, L2)
+    lay1, lay2, ext, dlen = pair_desc_layout(L1, L2)
+    a1 = {p[1]: pi for pi, p in enumerate(L1.params) if p[0] == 'array'}
+    a2 = {p[1]: pi for pi, p in enumerate(L2.params) if p[0] == 'array'}
+    h = [f'extern "C" void {P}_cuda_pair(DvmType *pBatchRef) {{']
+    b = ['DvmType batchRef = *pBatchRef;',
+         'long np = (long)dvmh_amd_batch_count_C(batchRef);',
+         'DvmType lr0 = dvmh_amd_pair_loop_C(batchRef, 0, 0);',
+         f'long *desc = (long *)calloc((size_t)np * {dlen}, sizeof(long));',
+         'DvmType *tstart = (DvmType *)calloc((size_t)np + 1, sizeof(DvmType));',
+         'DvmType *bstart = (DvmType *)calloc((size_t)np + 1, sizeof(DvmType));',
+         'DvmType *seg2 = (DvmType *)calloc((size_t)np + 1, sizeof(DvmType));',
+         'bool fused = true;',
+         '// every pair: the same arrays under the same names, natural layouts,',
+         '// equal unit-step bounds and 32-bit indices',
+         'for (long li = 0; li < np && fused; li++) {']
+    lb = ['DvmType lr1 = dvmh_amd_pair_loop_C(batchRef, 0, li), lr2 = dvmh_amd_pair_loop_C(batchRef, 1, li);',
+          f'long *d = desc + li * {dlen};',
+          'long maxStride = 1;']
+    for n1, p1 in a1.items():
+        for n2, p2 in a2.items():
+            op = '!=' if n1 == n2 else '=='
+            lb.append(f'if (dvmh_amd_pair_param_C(batchRef, 0, li, {p1}) {op} dvmh_amd_pair_param_C(batchRef, 1, li, {p2})) fused = false;')
+    for mem, L, lay in ((0, L1, lay1), (1, L2, lay2)):
+        for pi, p in enumerate(L.params):
+            if p[0] == 'array':
+                _, nm, ct, ar = p
+                lb.append(f'{{ DvmType *hdr = (DvmType *)dvmh_amd_pair_param_C(batchRef, {mem}, li, {pi});')
+                lb.append(f'  void *base = dvmh_get_natural_base_C(1, hdr);')
+                lb.append('  DvmType devHdr[64 + 7];')
+                lb.append('  if (dvmh_fill_header_C(1, base, hdr, devHdr, 0) != 0) fused = false;')
+                lb.append(f'  d[{lay[("base", nm)]}] = (long)base;')
+                for q in range(ar):
+                    lb.append(f'  d[{lay[("str", nm)] + q}] = devHdr[{q + 1}];')
+                lb.append(f'  for (int q = 0; q < {ar}; q++) if (labs(devHdr[q + 1]) > maxStride) maxStride = labs(devHdr[q + 1]); }}')
+            else:
+                lb.append(f'd[{lay[("sc", p[1])]}] = (long)dvmh_get_device_addr_C(1, dvmh_amd_pair_param_C(batchRef, {mem}, li, {pi}));')
+    lb += [f'DvmType blo[{r}], bhi[{r}], bst[{r}], blo2[{r}], bhi2[{r}], bst2[{r}];',
+           'dvmh_loop_fill_bounds_C(lr1, blo, bhi, bst);',
+           'dvmh_loop_fill_bounds_C(lr2, blo2, bhi2, bst2);',
+           f'for (int a = 0; a < {r}; a++) if (bst[a] != 1 || bst2[a] != 1 || blo[a] != blo2[a] || bhi[a] != bhi2[a] || bhi[a] < blo[a]) fused = false;',
+           'if (dvmh_loop_guess_index_type_C(lr1) != rt_INT || dvmh_loop_guess_index_type_C(lr2) != rt_INT) fused = false;']
+    for i in range(r):
+        lb.append(f'long cnt{i} = bhi[{i}] - blo[{i}] + 1;')
+        for lay in (lay1, lay2):
+            lb.append(f'd[{lay["bounds"] + 3 * i}] = blo[{i}]; d[{lay["bounds"] + 3 * i + 1}] = cnt{i}; d[{lay["bounds"] + 3 * i + 2}] = 1;')
+        lb.append(f'if ((labs(blo[{i}]) + cnt{i} + 64) * maxStride > 2147483647L) fused = false;')
+    for mem, L, lay in ((0, L1, lay1), (1, L2, lay2)):
+        for ri, red in enumerate(L.reds):
+            lb.append(f'{{ {red[2]} v; ')
+            if len(red) > 3:
+                lb.append(f'  {red[4]} lv; dvmh_loop_red_init_C(lr{mem + 1}, {ri + 1}, &v, &lv);')
+                lb.append(f'  memcpy(&d[{lay[("ri", red[3])]}], &lv, sizeof(lv));')
+            else:
+                lb.append(f'  dvmh_loop_red_init_C(lr{mem + 1}, {ri + 1}, &v, 0);')
+            lb.append(f'  memcpy(&d[{lay[("ri", red[1])]}], &v, sizeof(v)); }}')
+    b += [IND + s for s in lb]
+    b.append('}')
+    # the pair's launch shape (its own tuner site)
+    shape_ok = 'threads.z == 1 && ' + ('threads.y == 1' if r == 2 else 'threads.y >= 2')
+    rows = '((size_t)threads.y + 2)' if r == 3 else '(size_t)1'
+    b += ['DvmType tune[DVMH_AMD_TUNE_N];',
+          'dim3 threads(0, 0, 0);',
+          'hipStream_t stream;',
+          'int mcols = 0, npf = 0;',
+          'long march = 0;',
+          'size_t ldsBytes = 0;',
+          'if (fused) {',
+          IND + 'dvmh_amd_loop_get_tuning_C(lr0, DVMH_AMD_KIND_PAIR, tune);',
+          *[IND + x for x in regs_query(f'{P}_pair_kernel<int, 2, 3>', 'lr0')],
+          IND + 'mcols = (int)tune[DVMH_AMD_T_TILE_A];',
+          IND + 'march = (long)tune[DVMH_AMD_T_CHUNK];',
+          IND + f'if (tune[DVMH_AMD_T_VARIANT] != 1 || !({shape_ok}) || mcols < 1 || mcols > 4 || march < 2) fused = false;',
+          '}',
+          'if (fused) {',
+          IND + f'size_t mplane = {rows} * ((size_t)threads.x * mcols + 2);']
+    for nm in L1.stencil:
+        ct = [p for p in L1.arrays if p[1] == nm][0][2]
+        b.append(IND + f'ldsBytes += (3 * mplane * sizeof({ct}) + 15) & ~(size_t)15;')
+    b += [IND + 'size_t mthreads = (size_t)threads.x * threads.y;',
+          IND + 'size_t need = (mplane + mthreads - 1) / mthreads;']
+    for cols, sets in MARCH_NPF.items():
+        b.append(IND + f'if (mcols == {cols}) {{ ' + ' '.join(f'if (npf == 0 && need <= {v}) npf = {v};' for v in sets) + ' }')
+    b += [IND + 'if (npf == 0 || ldsBytes > 64512) fused = false;',
+          '}']
+    # otherwise the runtime runs the loops one by one through their own handlers
+    b.append('if (!fused) {')
+    b.append(IND + 'free(desc); free(tstart); free(bstart); free(seg2);')
+    b.append(IND + 'dvmh_amd_pair_decline_C(batchRef);')
+    b.append(IND + 'return;')
+    b.append('}')
+    # tiles and edge blocks per pair
+    ax_x = r - 1
+    b += ['long tiles = 0, blocks = 0;',
+          'for (long li = 0; li < np; li++) {',
+          IND + f'long *d = desc + li * {dlen};']
+    for i in range(r):
+        b.append(IND + f'long cnt{i} = d[{lay1["bounds"] + 3 * i + 1}];')
+    b.append(IND + f'long nbx = (cnt{ax_x} + (long)threads.x * mcols - 1) / ((long)threads.x * mcols);')
+    b.append(IND + ('long nby = (cnt1 + threads.y - 1) / threads.y;' if r == 3 else 'long nby = 1;'))
+    b.append(IND + 'long nbz = (cnt0 + march - 1) / march;')
+    for lay in (lay1, lay2):
+        b.append(IND + f'd[{lay["nb"]}] = nbx; d[{lay["nb"] + 1}] = nby; d[{lay["nb"] + 2}] = nbz;')
+    b += [IND + s for s in pair_border_count(r)]
+    b += [IND + 'long ntl = nbx * nby * nbz, nblk = (edges + 255) / 256;',
+          IND + 'tstart[li] = tiles; bstart[li] = blocks; seg2[li] = tiles + blocks;',
+          IND + f'd[{ext["seg2"]}] = tiles + blocks; d[{ext["ntiles"]}] = ntl;',
+          IND + 'tiles += ntl; blocks += nblk;',
+          '}',
+          'tstart[np] = tiles; bstart[np] = blocks; seg2[np] = tiles + blocks;',
+          'static const char dvmh_tag = 0;',
+          f'long *dDesc = (long *)dvmh_amd_batch_persist_C(batchRef, &dvmh_tag, 0, desc, np * {dlen} * (long)sizeof(long));',
+          'long *dStart = (long *)dvmh_amd_batch_persist_C(batchRef, &dvmh_tag, 1, tstart, (np + 1) * (long)sizeof(long));',
+          'long *dBlk = (long *)dvmh_amd_batch_persist_C(batchRef, &dvmh_tag, 2, bstart, (np + 1) * (long)sizeof(long));']
+    ra1, ra2 = [], []
+    for red in L1.reds:
+        b.append(f'{red[2]} *{red[1]}_grid = ({red[2]} *)dvmh_amd_batch_scratch_C(batchRef, (tiles > 0 ? tiles : 1) * (long)sizeof({red[2]}));')
+        ra1.append(f'{red[1]}_grid')
+        if len(red) > 3:
+            b.append(f'{red[4]} *{red[3]}_grid = ({red[4]} *)dvmh_amd_batch_scratch_C(batchRef, (tiles > 0 ? tiles : 1) * (long)sizeof({red[4]}));')
+            ra1.append(f'{red[3]}_grid')
+    for red in L2.reds:
+        b.append(f'{red[2]} *p2_{red[1]}_grid = ({red[2]} *)dvmh_amd_batch_scratch_C(batchRef, (tiles + blocks > 0 ? tiles + blocks : 1) * (long)sizeof({red[2]}));')
+        ra2.append(f'p2_{red[1]}_grid')
+        if len(red) > 3:
+            b.append(f'{red[4]} *p2_{red[3]}_grid = ({red[4]} *)dvmh_amd_batch_scratch_C(batchRef, (tiles + blocks > 0 ? tiles + blocks : 1) * (long)sizeof({red[4]}));')
+            ra2.append(f'p2_{red[3]}_grid')
+    b += ['dvmh_amd_mark_kernel_C(lr0, 0);',
+          'if (tiles > 0) {',
+          IND + 'long wgs = tiles < 16384 ? tiles : 16384;   // persistent: each workgroup walks tiles']
+    margs = ['dDesc', 'dStart', 'np', 'tiles', '(int)march', '(int)tune[DVMH_AMD_T_XCD]'] + ra1 + ra2
+    for cols, sets in MARCH_NPF.items():
+        for v in sets:
+            b.append(IND + f'if (mcols == {cols} && npf == {v}) hipLaunchKernelGGL(({P}_pair_kernel<int, {cols}, {v}>), '
+                     f'dim3((unsigned)wgs), threads, ldsBytes, stream, {", ".join(margs)});')
+    b.append('}')
+    bargs = ['dDesc', 'dBlk', 'np', 'blocks', '(int)march', '(int)threads.y', '(int)(threads.x * mcols)'] + ra2
+    b += ['if (blocks > 0) {',
+          IND + 'long bwgs = blocks < 16384 ? blocks : 16384;',
+          IND + f'hipLaunchKernelGGL(({P}_pair_border_kernel<int>), dim3((unsigned)bwgs), dim3(256, 1, 1), 0, stream, {", ".join(bargs)});',
+          '}',
+          'dvmh_amd_mark_kernel_C(lr0, 1);']
+    for ri, red in enumerate(L1.reds):
+        loc = f'{red[3]}_grid' if len(red) > 3 else '0'
+        b.append(f'dvmh_amd_pair_red_finish_C(batchRef, 0, {ri + 1}, {red[1]}_grid, {loc}, tstart);')
+    for ri, red in enumerate(L2.reds):
+        loc = f'p2_{red[3]}_grid' if len(red) > 3 else '0'
+        b.append(f'dvmh_amd_pair_red_finish_C(batchRef, 1, {ri + 1}, p2_{red[1]}_grid, {loc}, seg2);')
+    b.append('free(desc); free(tstart); free(bstart); free(seg2);')
+    h.append(indent('\n'.join(b) + '\n', 1).rstrip('\n'))
+    h.append('}')
+    return '\n'.join(h) + '\n'
+
+
+def pair_code(L1, L2):
+    return '\n'.join([march_kernel(L1, tail=L2), pair_kernels(L1, L2), pair_handler(L1, L2)])
 
 
 # ----------------------------------------------------------------- GPU: LINE (one dependent axis)
@@ -2367,6 +2900,18 @@ def emit(spec_path, outdir):
     for L in loops:
         if not L.host_only:
             hip.append(gpu_loop(L))
+    # consecutive loops that may run fused: the runtime learns of each pair's
+    # handler when the library is loaded and uses it when the two loops run
+    # one after the other in a deferred region or a batch
+    pairs = [(L1, L2) for L1, L2 in zip(loops, loops[1:]) if not L1.host_only and not L2.host_only and pair_eligible(L1, L2)]
+    for L1, L2 in pairs:
+        hip.append(pair_code(L1, L2))
+    if pairs:
+        hip.append('__attribute__((constructor)) static void dvmh_register_pairs(void) {')
+        for L1, L2 in pairs:
+            hip.append(f'    dvmh_amd_register_pair_C((DvmHandlerFunc){L1.name}_cuda, (DvmHandlerFunc){L2.name}_cuda, '
+                       f'(DvmHandlerFunc){pair_name(L1, L2)}_cuda_pair);')
+        hip.append('}')
     os.makedirs(outdir, exist_ok=True)
     with open(os.path.join(outdir, f'{prog}_host.c'), 'w') as f:
         f.write('\n'.join(host))

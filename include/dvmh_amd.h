@@ -32,6 +32,8 @@ extern "C" {
                                       axis, CHUNK = planes per workgroup on the outermost) */
 #define DVMH_AMD_KIND_BATCH 4      /* PARALLEL loops of many instances run by one batch kernel */
 #define DVMH_AMD_KIND_CHAIN 5      /* consecutive sweeps of a tiled-wavefront ACROSS loop run by its chain handler */
+#define DVMH_AMD_KIND_PAIR 6       /* a stencil loop and the loop after it run fused by their pair handler
+                                      (VARIANT 1, TILE_A and CHUNK as for STENCIL) */
 /* tune[] slots */
 #define DVMH_AMD_T_DEPTH 0     /* iterations of the slowest mapped axis per thread */
 #define DVMH_AMD_T_XCD 1       /* 1: XCD-aware block-to-tile order */
@@ -185,6 +187,30 @@ void dvmh_amd_batch_red_finish_C(DvmType batchRef, DvmType redIndex, const void 
  * until the batch handler returns. */
 void *dvmh_amd_batch_scratch_C(DvmType batchRef, DvmType bytes);
 
+/* ---- fused loop pairs -----------------------------------------------------------
+ * An emitted HIP file registers, when its library is loaded, a pair handler
+ * for two consecutive loops of its program that may run as one pass (a
+ * stencil loop, then a loop touching every array only at the loop point).
+ * When the runtime holds the two loops back together -- the loops of a lazy
+ * (graph) region, or of an open batch -- and they follow each other in one
+ * region with equal bounds, it calls the pair handler as f(&batchRef) instead
+ * of the two loops' handlers; several such pairs of one batch go to one call.
+ * Inside, dvmh_amd_batch_count_C is the number of pairs, member 0 is the first
+ * loop of a pair and member 1 the second. */
+void dvmh_amd_register_pair_C(DvmHandlerFunc first, DvmHandlerFunc second, DvmHandlerFunc pair);
+DvmType dvmh_amd_pair_loop_C(DvmType batchRef, DvmType member, DvmType i);
+void *dvmh_amd_pair_param_C(DvmType batchRef, DvmType member, DvmType i, DvmType p);
+/* The pair handler runs nothing (the pairs do not meet its conditions): each
+ * loop then runs through its own handler, in program order, when it returns. */
+void dvmh_amd_pair_decline_C(DvmType batchRef);
+/* As dvmh_amd_batch_red_finish_C, for reduction redIndex of the member loops. */
+void dvmh_amd_pair_red_finish_C(DvmType batchRef, DvmType member, DvmType redIndex, const void *partials,
+        const void *locPartials, const DvmType *offsets);
+/* 1 (default, DVMH_AMD_FUSE): registered pairs run fused; 0: loop by loop. */
+void dvmh_amd_set_fuse_C(DvmType enable);
+/* out[0] pair-handler calls, out[1] loop pairs they ran. */
+void dvmh_amd_pair_stats_C(DvmType out[2]);
+
 /* ---- region graphs -----------------------------------------------------------
  * A region created with this flag bit is executed lazily: its loops are recorded
  * and launched at dvmh_region_end_C; the first repetition with identical
@@ -214,6 +240,9 @@ DvmType dvmh_amd_tuner_site_pending_C(const char *file, DvmType line);
 DvmType dvmh_amd_tuner_query_C(const char *file, DvmType line, DvmType out[13]);
 /* The same for the chain site of that loop (DVMH_AMD_KIND_CHAIN). */
 DvmType dvmh_amd_tuner_query_chain_C(const char *file, DvmType line, DvmType out[13]);
+/* The same for the pair site of a fused pair whose first loop is at (file, line)
+ * (DVMH_AMD_KIND_PAIR). */
+DvmType dvmh_amd_tuner_query_pair_C(const char *file, DvmType line, DvmType out[13]);
 /* How the layout-transform candidates of the loop's sites fared: out[0]
  * transform candidates, out[1] of them timed, out[2] best timed one in ns and
  * out[3] best timed natural-layout candidate in ns (-1 where none was timed);

@@ -2,8 +2,10 @@
 arrays, regions and loops.  The per-grid form runs every grid's loops through
 their ordinary handlers; the batched form queues one iteration's loops between
 dvmh_amd_batch_begin_C / dvmh_amd_batch_end_C and runs each loop site as one
-persistent kernel over all grids' tiles.  Both are compared bit for bit with
-the oracle (grid values) and exactly (per-grid MAX eps)."""
+persistent kernel over all grids' tiles -- or, with fused loop pairs on (the
+default), runs each grid's stencil loop and copy + MAX loop as one pair, all
+grids' pairs through one call of the pair handler.  Every form is compared bit
+for bit with the oracle (grid values) and exactly (per-grid MAX eps)."""
 import ctypes
 
 import numpy as np
@@ -38,30 +40,62 @@ def test_host_path_bit_exact(batched):
     _check(res, H.oracle_batched(sides, 4), 4)
 
 
+def _pair_stats():
+    out = (ctypes.c_long * 2)()
+    D.runtime().dvmh_amd_pair_stats_C(out)
+    return list(out)
+
+
 def _gpu_run(count, iters, batched, **kw):
     before = D.stats()
+    pb = _pair_stats()
     res = H.run_batched(count, iters, batched, **kw)
     after = D.stats()
+    pa = _pair_stats()
     assert after['gpu_present'] == 1
     assert after['loops_host'] == before['loops_host']
-    return res, {k: after[k] - before[k] for k in after}
+    d = {k: after[k] - before[k] for k in after}
+    d['pair_calls'], d['pairs'] = pa[0] - pb[0], pa[1] - pb[1]
+    return res, d
+
+
+class _Fuse:
+    """Fused loop pairs on or off for a block."""
+
+    def __init__(self, on):
+        self.on = on
+        self.f = D.runtime().dvmh_amd_set_fuse_C
+        self.f.argtypes = [ctypes.c_long]
+
+    def __enter__(self):
+        self.f(self.on)
+
+    def __exit__(self, *exc):
+        self.f(1)
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize('fuse', [0, 1])
 @pytest.mark.parametrize('batched', [0, 1])
-def test_gpu_ragged_bit_exact(batched):
+def test_gpu_ragged_bit_exact(batched, fuse):
     """Ragged sides including a grid with an empty interior (side 2), a single
     interior point (side 3) and sides that are not multiples of any tile."""
     sides = [2, 3, 67, 5, 130, 64, 1, 257, 33, 1024]
     sides = [max(s, 2) for s in sides]
     iters = 5
-    res, d = _gpu_run(len(sides), iters, batched, sides=sides)
+    with _Fuse(fuse):
+        res, d = _gpu_run(len(sides), iters, batched, sides=sides)
     _check(res, H.oracle_batched(sides, iters), iters)
-    if batched:
+    nonempty = sum(1 for s in sides if s > 2)
+    if batched and not fuse:
         # two fused launches per iteration (stencil site, copy+MAX site)
         assert d['batch_groups'] == 2 * iters
-        nonempty = sum(1 for s in sides if s > 2)
         assert d['batch_loops'] == 2 * iters * nonempty
+    elif batched:
+        # one pair-handler call per iteration for all grids' pairs
+        assert d['batch_groups'] == 0 and d['pair_calls'] == iters and d['pairs'] == iters * nonempty
+    else:
+        assert d['batch_groups'] == 0 and d['pair_calls'] == 0
 
 
 KIND_BATCH = 4
@@ -89,7 +123,8 @@ def test_gpu_batch_configs_bit_exact(cfg):
     r.dvmh_amd_tuner_force_C(KIND_BATCH, ctypes.cast(b, ctypes.c_void_p), ctypes.cast(t, ctypes.c_void_p))
     try:
         sides = [2, 3, 67, 5, 130, 64, 257, 33, 1024]
-        res, d = _gpu_run(len(sides), 4, 1, sides=sides)
+        with _Fuse(0):   # the batch kernels, not the pair handler
+            res, d = _gpu_run(len(sides), 4, 1, sides=sides)
         _check(res, H.oracle_batched(sides, 4), 4)
         assert d['batch_groups'] == 8
     finally:
@@ -97,13 +132,18 @@ def test_gpu_batch_configs_bit_exact(cfg):
 
 
 @pytest.mark.gpu
-def test_gpu_batched_matches_per_grid_seeded():
+@pytest.mark.parametrize('fuse', [0, 1])
+def test_gpu_batched_matches_per_grid_seeded(fuse):
     """64 seeded grids, 6 iterations: batched and per-grid forms agree bit for
     bit with each other and with the oracle on a sample of grids."""
     count, iters = 64, 6
     a, _ = _gpu_run(count, iters, 0)
-    b, d = _gpu_run(count, iters, 1)
-    assert d['batch_groups'] == 2 * iters and d['batch_loops'] == 2 * iters * count
+    with _Fuse(fuse):
+        b, d = _gpu_run(count, iters, 1)
+    if fuse:
+        assert d['pair_calls'] == iters and d['pairs'] == iters * count
+    else:
+        assert d['batch_groups'] == 2 * iters and d['batch_loops'] == 2 * iters * count
     assert np.array_equal(a['eps'], b['eps'])
     for g in range(count):
         assert H.bits_equal(a['grids'][g]['A'], b['grids'][g]['A'])
@@ -126,16 +166,21 @@ class _Graphs:
 
 
 @pytest.mark.gpu
-def test_gpu_batched_graph_replay_bit_exact():
+@pytest.mark.parametrize('fuse', [0, 1])
+def test_gpu_batched_graph_replay_bit_exact(fuse):
     """The batched time step captured as one graph: the first iteration runs
     eagerly, the second is captured (and launched), the rest replay the graph.
     Results equal the eager batched form and the oracle bit for bit."""
     count, iters = 64, 8
     a, _ = _gpu_run(count, iters, 1)
-    with _Graphs():
+    with _Graphs(), _Fuse(fuse):
         b, d = _gpu_run(count, iters, 1)
     assert d['graph_replays'] == iters - 2
-    assert d['batch_groups'] == 2 * 2   # only the eager and the captured iteration run the batch handlers
+    # only the eager and the captured iteration run the handlers
+    if fuse:
+        assert d['pair_calls'] == 2 and d['batch_groups'] == 0
+    else:
+        assert d['batch_groups'] == 2 * 2
     assert np.array_equal(a['eps'], b['eps'])
     for g in range(count):
         assert H.bits_equal(a['grids'][g]['A'], b['grids'][g]['A'])
@@ -172,7 +217,7 @@ def test_gpu_full_workload_properties(graph):
         assert d['graph_replays'] == iters - 2
     else:
         res, d = _gpu_run(count, iters, 1, fetch=True)
-        assert d['batch_loops'] == 2 * iters * count
+        assert d['pairs'] == iters * count
     assert np.all(np.diff(res['eps'], axis=0) <= 0)
     for g in range(count):
         n = int(res['sides'][g])
