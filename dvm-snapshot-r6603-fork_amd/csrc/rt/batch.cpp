@@ -499,8 +499,20 @@ std::map<std::pair<const void *, long>, Persisted> persisted;
 
 static void uploadAsync(void *dst, const void *src, size_t bytes) {
     if (bytes == 0) return;
-    // a captured copy reads a pinned copy the graph owns (the ring is reused)
-    char *stage = graphCapturing() ? graphPinnedSlot(bytes) : uploadRing.take(bytes);
+    if (graphCapturing()) {
+        // What a handler uploads while a graph is captured is fixed for the
+        // graph's life (its queue's signature covers the values), so it is
+        // copied now, outside the graph, instead of on every replay.  The
+        // destination is device memory the graph owns, allocated during the
+        // capture (nothing else uses it; the stream was idle when the capture
+        // began), and the copy ends before the capture goes on.
+        static hipStream_t side = nullptr;
+        if (!side) HIP_CHECK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+        HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, side));
+        HIP_CHECK(hipStreamSynchronize(side));
+        return;
+    }
+    char *stage = uploadRing.take(bytes);
     memcpy(stage, src, bytes);
     HIP_CHECK(hipMemcpyAsync(dst, stage, bytes, hipMemcpyHostToDevice, gpu.stream));
 }
@@ -550,12 +562,13 @@ static void redFinish(DvmType batchRef, BatchCtx *b, const std::vector<Loop *> &
     }
     // the segment offsets usually repeat from one execution of the handler to the next
     void *dOff = dvmh_amd_batch_persist_C(batchRef, tag, slot, offsets, (nl + 1) * (long)sizeof(long));
-    void *dRes = dvmh_amd_batch_scratch_C(batchRef, nl * (es + ls) + 64);
-    void *dLoc = ls ? (char *)dRes + ((nl * es + 15) & ~15L) : nullptr;
-    dvmh_hip_reduce_segments(r0->func, r0->rtType, nl, (const long *)dOff, partials, locPartials, ls, dRes, dLoc, gpu.stream);
+    // the folded records go straight into mapped pinned host memory (a graph's
+    // own slot when captured): no separate copy
     size_t total = (size_t)(((nl * es + 15) & ~15L) + nl * ls + 64);
     char *host = graphCapturing() ? graphPinnedSlot(total) : pinnedTake(total);
-    HIP_CHECK(hipMemcpyAsync(host, dRes, total - 64, hipMemcpyDeviceToHost, gpu.stream));
+    char *dRes = mappedDevicePtr(host);
+    char *dLoc = ls ? dRes + ((nl * es + 15) & ~15L) : nullptr;
+    dvmh_hip_reduce_segments(r0->func, r0->rtType, nl, (const long *)dOff, partials, locPartials, ls, dRes, dLoc, gpu.stream);
     for (long i = 0; i < nl; i++) {
         auto &r = loops[i]->reds[redIndex - 1];
         char *val = host + i * es;

@@ -178,6 +178,7 @@ void graphRunRegion(Region *region) {
     int savedSync = settings.syncMode;
     settings.syncMode = 1;
     long before = layoutChanges();
+    gpu.sync();   // uploads made while capturing go to idle memory (see uploadAsync)
     HIP_CHECK(hipStreamBeginCapture(gpu.stream, hipStreamCaptureModeRelaxed));
     size_t nLoops = region->deferred.size();
     runDeferred(region);
@@ -354,6 +355,7 @@ bool batchGraphRun(const std::vector<Loop *> &loops, bool eligible, const std::f
     int savedSync = settings.syncMode;
     settings.syncMode = 1;
     long before = layoutChanges();
+    gpu.sync();   // uploads made while capturing go to idle memory (see uploadAsync)
     HIP_CHECK(hipStreamBeginCapture(gpu.stream, hipStreamCaptureModeRelaxed));
     runGroups();
     HIP_CHECK(hipStreamEndCapture(gpu.stream, &g.graph));

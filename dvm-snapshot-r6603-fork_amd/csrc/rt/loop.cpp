@@ -142,7 +142,7 @@ static PinnedPool pinnedPool;
 char *pinnedTake(size_t bytes) { return pinnedPool.take(bytes); }
 
 // Device-side address of mapped pinned host memory (pool blocks or graph slots).
-static char *mappedDevicePtr(char *h) {
+char *mappedDevicePtr(char *h) {
     if (char *d = pinnedPool.device(h)) return d;
     void *d = nullptr;
     HIP_CHECK(hipHostGetDevicePointer(&d, h, 0));

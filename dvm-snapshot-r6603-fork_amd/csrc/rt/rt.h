@@ -361,6 +361,7 @@ bool loopIsQueued(Loop *loop);
 int hostSlotCount();
 bool batchKeepsRegion(Region *r);
 char *pinnedTake(size_t bytes);
+char *mappedDevicePtr(char *hostPinned);   // device-side alias of pinnedTake / graph slot memory
 void performLoopNow(Loop *loop);
 long layoutChanges();
 
