@@ -76,6 +76,12 @@ def kernel_stats(rt, cdv, lines):
     return launches, ns
 
 
+def pair_stats(rt):
+    out = (ctypes.c_long * 2)()
+    rt.dvmh_amd_pair_stats_C(out)
+    return list(out)
+
+
 def set_form(rt, mode, sync, graph=0):
     rt.dvmh_amd_tuner_set_mode_C(mode)
     rt.dvmh_amd_tuner_set_repeats_C(2)
@@ -186,6 +192,11 @@ def batched_leg(rt, iters):
                      'kernel_launches': launches,
                      'kernel_ms_per_iteration': ns * 1e-6 / iters,
                      'achieved_GBps': pts * 40 * iters / (ns * 1e-9) / 1e9 if ns else None}
+        if batched:
+            # each grid's two loops run as one fused pair: at least 24 B per
+            # point (A read, B and A written)
+            res[name]['achieved_GBps_fused_traffic'] = pts * 24 * iters / (ns * 1e-9) / 1e9 if ns else None
+            res[name]['frac_fused_traffic'] = pts * 24 * iters / (ns * 1e-9) / HBM_PEAK if ns else None
     P.fn('teardown', [])()
     # parity: a fresh batched run of 5 iterations in the timed form (eager,
     # captured, then 3 replays), smallest and largest grid
@@ -207,7 +218,8 @@ def batched_leg(rt, iters):
     P.fn('teardown', [])()
     set_form(rt, 0, 0, 0)
     return {'workload': '512 uneven 2-D Jacobi grids (sides 64..1024 by seed), FP64, 5-point + REDUCTION(MAX) per grid',
-            'form': 'batched: one persistent kernel per loop site per iteration, the time step replayed as one graph',
+            'form': 'batched: each grid\'s stencil and copy + MAX loops fused into one pass, all grids\' pairs in one '
+                    'persistent kernel (plus one over the tile-edge points) per iteration, the time step replayed as one graph',
             'parity_graph_replays': parity_replays,
             'points_per_iteration': pts, 'iterations': iters, 'bytes_per_point_update': 40,
             'per_grid': res['per_grid'], 'batched': res['batched'],
@@ -270,6 +282,14 @@ def chosen_layouts(rt, w, chain=False):
                 out[str(line)]['transform_candidates'] = {'count': t[0], 'timed': t[1],
                                                           'best_us': t[2] / 1e3 if t[2] >= 0 else None,
                                                           'best_natural_us': t[3] / 1e3 if t[3] >= 0 else None}
+    # a fused pair's own site (first loop's line)
+    fp = rt.dvmh_amd_tuner_query_pair_C
+    fp.argtypes = [ctypes.c_char_p, ctypes.c_long, ctypes.POINTER(ctypes.c_long)]
+    fp.restype = ctypes.c_long
+    for line in w.lines:
+        if not chain and fp(w.cdv, line, q) > 0:
+            out[f'{line} (fused pair)'] = {'block': [q[2], q[3], q[4]], 'tile_a': q[5 + 4], 'chunk': q[5 + 3], 'xcd': q[5 + 1],
+                                           'us': q[1] / 1e3}
     return out
 
 
@@ -313,10 +333,14 @@ def time_form(rt, name, n, iters, mode, sync, graph):
             break
     w.run(3 if name != 'sor3d_chain' else iters)   # settled: a repeating region is captured here, before timing
     sec = w.run(iters)
-    rt.dvmh_amd_set_graph_C(0)
+    # the same form without capture (regions still deferred, so loop pairs
+    # still run fused), every launch between timing marks
+    rt.dvmh_amd_set_graph_C(2 if graph else 0)
     rt.dvmh_amd_reset_kernel_stats_C()
     rt.dvmh_amd_set_kernel_timing_C(1)
+    p0 = pair_stats(rt)
     w.run(iters)
+    pairs = pair_stats(rt)[1] - p0[1]
     rt.dvmh_amd_set_kernel_timing_C(0)
     rt.dvmh_amd_set_graph_C(graph)
     launches, ns = kernel_stats(rt, w.cdv, w.lines)
@@ -328,6 +352,13 @@ def time_form(rt, name, n, iters, mode, sync, graph):
          'frac': byt * iters / (ns * 1e-9) / HBM_PEAK if ns else None,
          'frac_wall': byt * iters / sec / HBM_PEAK,
          'point_updates_per_iteration': upd, 'bytes_per_iteration': byt}
+    if pairs:
+        # fused loop pairs: the kernels move at least the fused step's bytes
+        # (each array read once and written once per point)
+        fb = w.fused_bytes_per_iter()
+        r['fused_pairs_per_iteration'] = pairs / iters
+        r['fused_bytes_per_iteration'] = fb
+        r['frac_fused_traffic'] = fb * iters / (ns * 1e-9) / HBM_PEAK if ns else None
     if layouts is not None:
         r['sites'] = layouts
     return r

@@ -137,6 +137,11 @@ static void runEager(Region *region) { runDeferred(region); }
 // Called by dvmh_region_end_C for a lazy region with queued loops.
 void graphRunRegion(Region *region) {
     if (region->deferred.empty()) return;
+    if (settings.graphNoCapture) {
+        // deferred, fused where pairs allow, but launched one by one (no replay)
+        runEager(region);
+        return;
+    }
     std::string key = rt.curFile + "#" + signature(region);
     GraphEntry &g = graphs[key];
     if (g.exec) {
@@ -295,7 +300,7 @@ static size_t hashWords(const std::vector<long> &w) {
 }
 
 bool batchGraphRun(const std::vector<Loop *> &loops, bool eligible, const std::function<void()> &runGroups) {
-    if (!eligible || loops.empty()) {
+    if (!eligible || loops.empty() || settings.graphNoCapture) {
         runGroups();
         return false;
     }

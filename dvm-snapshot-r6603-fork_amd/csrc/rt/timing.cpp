@@ -111,7 +111,10 @@ extern "C" void dvmh_amd_set_sync_mode_C(DvmType mode) {
     settings.syncMode = (int)mode;
 }
 
-extern "C" void dvmh_amd_set_graph_C(DvmType enable) { settings.graph = enable != 0; }
+extern "C" void dvmh_amd_set_graph_C(DvmType enable) {
+    settings.graph = enable != 0;
+    settings.graphNoCapture = enable == 2;
+}
 
 // Streaming kernels of known traffic, timed with HIP events on the runtime's
 // stream over `reps` back-to-back launches (after one untimed launch); the

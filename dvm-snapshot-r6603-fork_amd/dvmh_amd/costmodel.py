@@ -52,6 +52,7 @@ def rt():
         dpp = ctypes.POINTER(ctypes.c_double)
         sig = {
             'dvmh_amd_tuner_set_repeats_C': ([L], None),
+            'dvmh_amd_set_graph_C': ([L], None),
             'dvmh_amd_tuner_clear_C': ([], None),
             'dvmh_amd_tuner_site_pending_C': ([cp, L], L),
             'dvmh_amd_tuner_sites_C': ([ctypes.c_char_p, L], L),
@@ -97,6 +98,7 @@ def sweep_workload(name, size, repeats=2, iters_per_run=20, max_runs=400, log=No
     lib.dvmh_amd_tuner_set_mode_C(1)
     lib.dvmh_amd_tuner_set_repeats_C(repeats)
     w = workloads.ALL[name](size)
+    lib.dvmh_amd_set_graph_C(2 if w.lazy else 0)
     w.setup()
     runs = 0
     t0 = time.perf_counter()
@@ -110,6 +112,7 @@ def sweep_workload(name, size, repeats=2, iters_per_run=20, max_runs=400, log=No
             raise RuntimeError(f'{name} {size}: tuner still has {pending} candidates pending after {runs} runs')
     w.teardown()
     lib.dvmh_amd_tuner_set_mode_C(0)
+    lib.dvmh_amd_set_graph_C(0)
     if log:
         log(f'swept {name} {size}: {runs} runs, {time.perf_counter() - t0:.1f} s')
 
@@ -124,7 +127,8 @@ def sweep(specs=DEFAULT_SPECS, out=RECORDS, repeats=2, log=None, merge=False):
     for name, sizes in specs:
         for size in sizes:
             sweep_workload(name, size, repeats=repeats, log=log)
-            programs.add(workloads.ALL[name].cdv.decode() + ':')
+            W = workloads.ALL[name]
+            programs.update(W.record_prefixes or (W.cdv.decode() + ':',))
     os.makedirs(os.path.dirname(out) or '.', exist_ok=True)
     kept = []
     if merge and os.path.exists(out):

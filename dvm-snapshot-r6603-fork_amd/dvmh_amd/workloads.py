@@ -3,7 +3,7 @@ through their C entry points, plus the source positions (file, line) of the
 loops their iteration phase executes, which the tuner keys its sites by."""
 import ctypes
 
-from . import DEVICE_TYPE_CUDA, DEVICE_TYPE_HOST, SEED_BASE, Program
+from . import DEVICE_TYPE_CUDA, DEVICE_TYPE_HOST, REGION_AMD_GRAPH, SEED_BASE, Program
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _lp = ctypes.POINTER(ctypes.c_long)
@@ -14,6 +14,8 @@ class Workload:
     cdv = b''
     lines = ()
     seed_offset = 0
+    lazy = False            # regions deferred (lazy, never captured while swept)
+    record_prefixes = None  # tuning records a sweep of it replaces (default: its program's)
 
     def __init__(self, size, targets=DEVICE_TYPE_HOST | DEVICE_TYPE_CUDA, flags=0):
         self.size = size
@@ -56,6 +58,10 @@ class Jac2d(Workload):
     def bytes_per_iter(self):
         # stencil loop: read A, write B (16 B); copy loop with MAX: read A and B, write A (24 B)
         return 40 * self.points_per_iter()
+
+    def fused_bytes_per_iter(self):
+        # the two loops as one fused pass: read A, write B and A (24 B)
+        return 24 * self.points_per_iter()
 
 
 class Jac3d(Workload):
@@ -106,4 +112,17 @@ class Adi3d(Workload):
         return 16 * self.points_per_iter() + 8 * self.size ** 3
 
 
-ALL = {'jac2d': Jac2d, 'jac3d': Jac3d, 'sor3d': Sor3d, 'sor3d_chain': Sor3dChain, 'adi3d': Adi3d}
+class Jac2dPairs(Jac2d):
+    """2-D Jacobi with its region deferred to the region's end (the program's
+    graph flag), so the stencil loop and the copy + MAX loop run as one fused
+    pair: the tuner times the pair site (first loop's line, pair kind)."""
+    name = 'jac2d'
+    lines = (38,)
+    lazy = True
+    record_prefixes = ('jac2d.cdv:38|k6|',)
+
+    def __init__(self, size, targets=DEVICE_TYPE_HOST | DEVICE_TYPE_CUDA, flags=0):
+        super().__init__(size, targets, flags | REGION_AMD_GRAPH)
+
+
+ALL = {'jac2d': Jac2d, 'jac2d_pairs': Jac2dPairs, 'jac3d': Jac3d, 'sor3d': Sor3d, 'sor3d_chain': Sor3dChain, 'adi3d': Adi3d}

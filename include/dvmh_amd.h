@@ -63,7 +63,9 @@ void dvmh_amd_reset_kernel_stats_C(void);
  * 1: only when a result is needed (get_actual, region end, barrier). */
 void dvmh_amd_set_sync_mode_C(DvmType mode);
 /* 1: regions created with REGION_AMD_GRAPH are run lazily and replayed as graphs
- * (same as DVMH_AMD_GRAPH=1); 0: every region runs eagerly. */
+ * (same as DVMH_AMD_GRAPH=1); 0: every region runs eagerly; 2: such regions
+ * still run lazily (their loops at the region's end, consecutive pairs fused)
+ * but nothing is captured, so every launch can carry timing marks. */
 void dvmh_amd_set_graph_C(DvmType enable);
 
 /* ---- statistics ------------------------------------------------------------
