@@ -1399,6 +1399,9 @@ def batch_handler(L):
 # from before L2.
 
 WRITE_OP = r'(?:=(?!=)|\+=|-=|\*=|/=|%=|&=|\|=|\^=|<<=|>>=|\+\+|--)'
+# edge points per thread of the border kernel (a workgroup of 256 threads
+# takes 256 * PAIR_EDGE_PTS consecutive edge points, thread t every 256th)
+PAIR_EDGE_PTS = 8
 
 
 def array_refs(L, name):
@@ -1653,38 +1656,14 @@ def pair_kernels(L1, L2):
     if r == 3:
         tsz[1] = 'tileY'
         tcnt[1] = f'd[{nbo + 1}]'
+    # (32-bit arithmetic: the handler takes this path only for 32-bit indices)
     for a in ax:
-        u.append(f'const long T{a} = (long){tsz[a]}, n{a} = {tcnt[a]}, r{a} = n{a} > 1 ? 2 * (n{a} - 1) : 0;')
-    u.append('const long e = (blk - blkStart[lo_]) * (long)blockDim.x + (long)threadIdx.x;')
+        u.append(f'const IndexT T{a} = (IndexT){tsz[a]}, n{a} = (IndexT){tcnt[a]}, r{a} = n{a} > 1 ? 2 * (n{a} - 1) : 0;')
     u.append('// the index next to boundary u / 2 between tiles: its last (u even) or first (u odd) index')
-    u.append('auto edgeAt = [](long uu, long T) -> long { return (uu & 1) ? ((uu >> 1) + 1) * T : ((uu >> 1) + 1) * T - 1; };')
-    u.append('auto onEdge = [](long tt, long T, long n) -> bool { const long q = tt / T, m = tt - q * T; return (m == 0 && q > 0) || (m == T - 1 && q + 1 < n); };')
-    u.append('bool valid = false;')
-    u.append(' '.join(f'long p{a} = 0;' for a in ax))
-    ext_cnt = {a: f'(long)cnt{a}' for a in ax}
-    start = '0'
-    for si, a in enumerate(ax):
-        dims = [(f'r{a}' if b == a else ext_cnt[b]) for b in ax]
-        size = ' * '.join(dims)
-        cond = 'if' if si == 0 else 'else if'
-        u.append(f'{cond} (e < {start} + {size}) {{')
-        u.append(IND + f'long f = e - ({start});')
-        # decode with the innermost axis fastest
-        for bi in reversed(range(len(ax))):
-            b_ = ax[bi]
-            dim = dims[bi]
-            if bi > 0:
-                u.append(IND + f'const long c{b_} = f % ({dim}); f /= ({dim});')
-            else:
-                u.append(IND + f'const long c{b_} = f;')
-        for b_ in ax:
-            u.append(IND + (f'p{b_} = edgeAt(c{b_}, T{b_});' if b_ == a else f'p{b_} = c{b_};'))
-        earlier = [f'!onEdge(p{b_}, T{b_}, n{b_})' for b_ in ax[:si]]
-        u.append(IND + f'valid = {" && ".join(earlier) if earlier else "true"};')
-        u.append('}')
-        start = f'{start} + {size}' if start != '0' else size
+    u.append('auto edgeAt = [](IndexT uu, IndexT T) -> IndexT { return (uu & 1) ? ((uu >> 1) + 1) * T : ((uu >> 1) + 1) * T - 1; };')
+    u.append('auto onEdge = [](IndexT tt, IndexT T, IndexT n) -> bool { const IndexT q = tt / T, m = tt - q * T; return (m == 0 && q > 0) || (m == T - 1 && q + 1 < n); };')
     # L2's arrays and scalars from the descriptor, under names of the kernel's
-    # own, then bound to L2's names in the point's scope
+    # own (bound to L2's names in the point's scope)
     for p in L2.arrays:
         _, nm, ct, ar = p
         so = lay2[('str', nm)]
@@ -1692,9 +1671,35 @@ def pair_kernels(L1, L2):
         u.append(f'const IndexT dvmhp_{nm}_str[{ar}] = {{{", ".join(f"(IndexT)d[{so + q}]" for q in range(ar))}}};')
     for p in L2.scalars:
         u.append(f'{p[2]} *const dvmhp_{p[1]} = dvmh_dev::globalPtr<{p[2]}>(d[{lay2[("sc", p[1])]}]);')
-    u.append('if (valid) {')
+    u.append(f'for (int dvmh_k = 0; dvmh_k < {PAIR_EDGE_PTS}; dvmh_k++) {{')
+    pu = [f'const IndexT e = (IndexT)(blk - blkStart[lo_]) * (IndexT)blockDim.x * {PAIR_EDGE_PTS} + (IndexT)dvmh_k * (IndexT)blockDim.x + (IndexT)threadIdx.x;',
+          'bool valid = false;']
+    pu.append(' '.join(f'IndexT p{a} = 0;' for a in ax))
+    ext_cnt = {a: f'cnt{a}' for a in ax}
+    start = '0'
+    for si, a in enumerate(ax):
+        dims = [(f'r{a}' if b == a else ext_cnt[b]) for b in ax]
+        size = ' * '.join(dims)
+        cond = 'if' if si == 0 else 'else if'
+        pu.append(f'{cond} (e < {start} + {size}) {{')
+        pu.append(IND + f'IndexT f = e - ({start});')
+        # decode with the innermost axis fastest
+        for bi in reversed(range(len(ax))):
+            b_ = ax[bi]
+            dim = dims[bi]
+            if bi > 0:
+                pu.append(IND + f'const IndexT c{b_} = f % ({dim}); f /= ({dim});')
+            else:
+                pu.append(IND + f'const IndexT c{b_} = f;')
+        for b_ in ax:
+            pu.append(IND + (f'p{b_} = edgeAt(c{b_}, T{b_});' if b_ == a else f'p{b_} = c{b_};'))
+        earlier = [f'!onEdge(p{b_}, T{b_}, n{b_})' for b_ in ax[:si]]
+        pu.append(IND + f'valid = {" && ".join(earlier) if earlier else "true"};')
+        pu.append('}')
+        start = f'{start} + {size}' if start != '0' else size
+    pu.append('if (valid) {')
     for a in ax:
-        u.append(IND + f'const IndexT t{a} = (IndexT)p{a};')
+        pu.append(IND + f'const IndexT t{a} = p{a};')
     sc = []
     for p in L2.arrays:
         _, nm, ct, ar = p
@@ -1708,7 +1713,9 @@ def pair_kernels(L1, L2):
     for i in range(r):
         sc.append(var_assign(L2, i, f't{i}', f'lo{i}', '1'))
     sc.append('{\n' + indent(L2.body + '\n', 1) + '}')
-    u += [IND + ln for ln in '\n'.join(sc).split('\n')]
+    pu += [IND + ln for ln in '\n'.join(sc).split('\n')]
+    pu.append('}')
+    u += [IND + ln for ln in '\n'.join(pu).split('\n')]
     u.append('}')
     u += red_kernel_tail(L2, f'd[{ext["seg2"]}] + d[{ext["ntiles"]}] + (blk - blkStart[lo_])', accumulate=False)
     gb += [IND + ln for ln in '\n'.join(u).split('\n')]
@@ -1851,12 +1858,19 @@ def pair_handler(L1, L2):
     for lay in (lay1, lay2):
         b.append(IND + f'd[{lay["nb"]}] = nbx; d[{lay["nb"] + 1}] = nby; d[{lay["nb"] + 2}] = nbz;')
     b += [IND + s for s in pair_border_count(r)]
-    b += [IND + 'long ntl = nbx * nby * nbz, nblk = (edges + 255) / 256;',
+    b += [IND + '// edge points are numbered in 32-bit arithmetic in the border kernel',
+          IND + f'if (edges + {256 * PAIR_EDGE_PTS} > 2147483647L) fused = false;',
+          IND + f'long ntl = nbx * nby * nbz, nblk = (edges + {256 * PAIR_EDGE_PTS - 1}) / {256 * PAIR_EDGE_PTS};',
           IND + 'tstart[li] = tiles; bstart[li] = blocks; seg2[li] = tiles + blocks;',
           IND + f'd[{ext["seg2"]}] = tiles + blocks; d[{ext["ntiles"]}] = ntl;',
           IND + 'tiles += ntl; blocks += nblk;',
           '}',
           'tstart[np] = tiles; bstart[np] = blocks; seg2[np] = tiles + blocks;',
+          'if (!fused) {',
+          IND + 'free(desc); free(tstart); free(bstart); free(seg2);',
+          IND + 'dvmh_amd_pair_decline_C(batchRef);',
+          IND + 'return;',
+          '}',
           'static const char dvmh_tag = 0;',
           f'long *dDesc = (long *)dvmh_amd_batch_persist_C(batchRef, &dvmh_tag, 0, desc, np * {dlen} * (long)sizeof(long));',
           'long *dStart = (long *)dvmh_amd_batch_persist_C(batchRef, &dvmh_tag, 1, tstart, (np + 1) * (long)sizeof(long));',
