@@ -364,7 +364,7 @@ def time_form(rt, name, n, iters, mode, sync, graph):
     return r
 
 
-def other_configs(rt, parity=True):
+def other_configs(rt, parity=True, only=None):
     """The remaining configurations, plain (default configuration, per-loop
     launches) against the chosen form (cost model, results synchronised only
     when needed, repeating time steps replayed as graphs; SOR's sweeps chained
@@ -375,6 +375,8 @@ def other_configs(rt, parity=True):
              ('adi3d', 256, 20, 'C4 ADI 256^3 x 20 (six line sweeps + MAXLOC; 6 x 255 x 256^2 point-updates per step)')]
     out = {}
     for name, n, iters, desc in specs:
+        if only is not None and name not in only:
+            continue
         r = {'workload': desc}
         r['plain'] = time_form(rt, name, n, iters, 0, 0, 0)
         if name == 'sor3d':
@@ -499,6 +501,8 @@ def main():
                     help='run only the chosen form (for profiling runs); prints the JSON line without the other legs')
     ap.add_argument('--others-only', action='store_true',
                     help='run only the batched-grid and other-config legs (for profiling runs); prints their JSON')
+    ap.add_argument('--legs', default=None,
+                    help='with --others-only: comma-separated legs to run (batched, jac2d, sor3d, adi3d, transforms)')
     ap.add_argument('--calibrate-copy', action='store_true',
                     help='also run the streaming copy and read-only kernels on 1 GiB (scale reference for HBM counters)')
     args = ap.parse_args()
@@ -516,11 +520,12 @@ def main():
     if args.calibrate_copy:
         calibrate_copy()
     if args.others_only:
-        batched = batched_leg(rt, 20)
+        legs = set(args.legs.split(',')) if args.legs else {'batched', 'jac2d', 'sor3d', 'adi3d', 'transforms'}
+        batched = batched_leg(rt, 20) if 'batched' in legs else None
         log('batched', batched)
-        others = other_configs(rt)
+        others = other_configs(rt, only=legs)
         log('others', others)
-        transforms = layout_transforms()
+        transforms = layout_transforms() if 'transforms' in legs else None
         log('layout transforms', transforms)
         print(json.dumps({'batched_grids': batched, 'other_configs': others, 'layout_transforms': transforms}), flush=True)
         return
