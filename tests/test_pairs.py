@@ -187,3 +187,72 @@ def test_fuse_off_runs_loop_by_loop():
         assert pair_stats() == before
     finally:
         r.dvmh_amd_set_fuse_C(1)
+
+
+# ---- the emitter's pair analysis (CPU)
+
+def _loops(*descs):
+    import os
+    import sys
+    sys.path.insert(0, os.path.join(D.PKG_DIR, 'emit'))
+    import hipgen
+    return hipgen, [hipgen.Loop(d, 't') for d in descs]
+
+
+def _stencil(body='B[i][j] = 0.25 * (A[i - 1][j] + A[i + 1][j] + A[i][j - 1] + A[i][j + 1]);', params=None, **kw):
+    d = {'name': 'l1', 'vars': [('i', 'int'), ('j', 'int')], 'stencil': ['A'],
+         'params': params or [('array', 'A', 'double', 2), ('array', 'B', 'double', 2)], 'body': body}
+    d.update(kw)
+    return d
+
+
+def _pointwise(body='double d = fabs(B[i][j] - A[i][j]);\neps = Max(d, eps);\nA[i][j] = B[i][j];', params=None, **kw):
+    d = {'name': 'l2', 'vars': [('i', 'int'), ('j', 'int')], 'reductions': [('max', 'eps', 'double')],
+         'params': params or [('array', 'A', 'double', 2), ('array', 'B', 'double', 2)], 'body': body}
+    d.update(kw)
+    return d
+
+
+def test_pair_eligible_accepts_time_step():
+    hg, (a, b) = _loops(_stencil(), _pointwise())
+    assert hg.pair_eligible(a, b)
+    # loop variables of other names, an extra array and scalar in the second loop
+    hg, (a, b) = _loops(_stencil(), _pointwise(body='C[p][q] = C[p][q] + B[p][q] * w;\nA[p][q] = B[p][q];',
+                                              vars=[('p', 'int'), ('q', 'int')], reductions=[],
+                                              params=[('array', 'A', 'double', 2), ('array', 'B', 'double', 2),
+                                                      ('array', 'C', 'double', 2), ('scalar', 'w', 'double')]))
+    assert hg.pair_eligible(a, b)
+
+
+@pytest.mark.parametrize('what,l1,l2', [
+    ('second loop reads a neighbour', {}, {'body': 'A[i][j] = B[i - 1][j];'}),
+    ('second loop passes an array', {}, {'body': 'A[i][j] = g(B, i, j);'}),
+    ('nested subscript', {}, {'body': 'A[i][j] = B[k[i]][j];'}),
+    ('first loop reads a non-stencil array off the point', {'body': 'B[i][j] = A[i - 1][j] + B[i + 1][j];'}, {}),
+    ('a name is an array in one loop and a scalar in the other', {},
+     {'params': [('array', 'A', 'double', 2), ('scalar', 'B', 'double')], 'body': 'A[i][j] = B;'}),
+    ('reduction variable named like a first-loop array', {}, {'reductions': [('max', 'B', 'double')],
+                                                            'body': 'B = Max(B, A[i][j]);',
+                                                            'params': [('array', 'A', 'double', 2)]}),
+    ('across loop', {}, {'across': {'array': 'A', 'widths': [(1, 0), (0, 0)]}}),
+    ('first loop without a stencil array', {'stencil': []}, {}),
+])
+def test_pair_eligible_rejects(what, l1, l2):
+    hg, (a, b) = _loops(_stencil(**l1), _pointwise(**l2))
+    assert not hg.pair_eligible(a, b), what
+
+
+def test_pairs_registered_in_emitted_code(tmp_path):
+    """emit() writes a pair handler and one registration per eligible
+    consecutive pair, none for a pair that is not."""
+    import os
+    hg, _ = _loops(_stencil(), _pointwise())
+    spec = {'program': 't', 'include': 'dvmh_prog_common.h',
+            'loops': [_stencil(), _pointwise(), _pointwise(name='l3', body='A[i][j] = B[i + 1][j];', reductions=[])]}
+    path = os.path.join(str(tmp_path), 't.loops')
+    with open(path, 'w') as f:
+        f.write(repr(spec))
+    hg.emit(path, str(tmp_path))
+    text = open(os.path.join(str(tmp_path), 't_hip.hip')).read()
+    assert text.count('dvmh_amd_register_pair_C(') == 1
+    assert 'l1__l2_cuda_pair' in text and 'l2__l3' not in text
