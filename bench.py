@@ -252,8 +252,15 @@ def other_parity(rt, name, n, iters, chain=False):
               'loc_exact': bool(np.array_equal(res['loc'], ref['loc']))}
     st1 = D.stats()
     ok['iterations'] = iters
-    ok['form'] = 'chain of %d sweeps' % iters if chain else 'graph replay'
-    ok['graph_replays'] = st1['graph_replays'] - st0['graph_replays']
+    replays = st1['graph_replays'] - st0['graph_replays']
+    if chain:
+        ok['form'] = 'chain of %d sweeps' % iters
+    elif replays:
+        ok['form'] = 'deferred region replayed as a graph'
+    else:
+        # a region whose graph is below the replay threshold runs eagerly
+        ok['form'] = 'deferred region launched eagerly (its graph is too small to pay for a graph launch)'
+    ok['graph_replays'] = replays
     ok['batch_groups'] = st1['batch_groups'] - st0['batch_groups']
     set_form(rt, 0, 0, 0)
     return ok

@@ -59,6 +59,7 @@ void Settings::load() {
     allowAsync = envLong("DVMH_ALLOW_ASYNC", 0) != 0;   // rts/dvmh/settings.cpp:361
     pollDone = envLong("DVMH_AMD_POLL", 1) != 0;
     graph = envLong("DVMH_AMD_GRAPH", 0) != 0;
+    graphMinNodes = envLong("DVMH_AMD_GRAPH_MIN_NODES", 6);
     tuneRepeats = (int)envLong("DVMH_AMD_TUNE_REPEATS", 1);
     if (tuneRepeats < 1) tuneRepeats = 1;
     tuneTransforms = envLong("DVMH_AMD_TUNE_TRANSFORMS", 1) != 0;

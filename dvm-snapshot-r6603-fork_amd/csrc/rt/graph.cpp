@@ -144,7 +144,7 @@ void graphRunRegion(Region *region) {
     }
     std::string key = rt.curFile + "#" + signature(region);
     GraphEntry &g = graphs[key];
-    if (g.exec) {
+    if (g.exec && !g.noCapture) {
         HIP_CHECK(hipGraphLaunch(g.exec, gpu.stream));
         graphReplays++;
         rt.loopsOnGpu += (long)region->deferred.size();
@@ -194,6 +194,12 @@ void graphRunRegion(Region *region) {
     HIP_CHECK(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
     HIP_CHECK(hipGraphLaunch(g.exec, gpu.stream));
     (void)nLoops;
+    // A graph launch costs the host more than launching a few kernels one by
+    // one (measured: a three-kernel step 6 us slower replayed than eager):
+    // a region whose graph is that small runs eagerly from now on.
+    size_t nodes = 0;
+    HIP_CHECK(hipGraphGetNodes(g.graph, nullptr, &nodes));
+    if ((long)nodes < settings.graphMinNodes) g.noCapture = true;
     // the captured loops' records were queued as pending posts by the finishes
     if (syncAtLoopEnd(region)) flushDeferredPosts();
 }

@@ -54,6 +54,7 @@ struct Settings {
     bool allowAsync = false;    // DVMH_ALLOW_ASYNC: regions created with REGION_ASYNC synchronise only when results are needed
     bool graph = false;         // DVMH_AMD_GRAPH: capture repeating regions into HIP graphs
     bool graphNoCapture = false;   // lazy regions and batches run eagerly, never captured (kernel timing)
+    long graphMinNodes = 6;        // DVMH_AMD_GRAPH_MIN_NODES: a region's graph with fewer nodes is not replayed
     int tuneRepeats = 1;        // DVMH_AMD_TUNE_REPEATS: timed executions per candidate
     bool pollDone = true;          // DVMH_AMD_POLL: wait for reduction results on the completion word, not a stream sync
     bool tuneTransforms = true;    // DVMH_AMD_TUNE_TRANSFORMS (default 1): candidates may re-lay out arrays of ACROSS loops

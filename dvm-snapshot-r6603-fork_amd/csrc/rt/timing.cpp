@@ -111,6 +111,8 @@ extern "C" void dvmh_amd_set_sync_mode_C(DvmType mode) {
     settings.syncMode = (int)mode;
 }
 
+extern "C" void dvmh_amd_set_graph_min_nodes_C(DvmType nodes) { settings.graphMinNodes = nodes < 0 ? 6 : (long)nodes; }
+
 extern "C" void dvmh_amd_set_graph_C(DvmType enable) {
     settings.graph = enable != 0;
     settings.graphNoCapture = enable == 2;

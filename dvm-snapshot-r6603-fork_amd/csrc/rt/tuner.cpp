@@ -146,11 +146,13 @@ std::vector<Candidate> makeCandidates(int kind, int loopRank, int depCount, cons
     } else if (kind == DVMH_AMD_KIND_PAIR) {
         // a fused pair runs the LDS march of its stencil loop, the second loop
         // inside it off the tile edges and on the edges in a second kernel:
-        // taller tiles leave fewer edge points.  Default: 256 threads x 4
-        // points, 32 planes (rank 2: best of the sweep over 512 grids of sides
-        // 64..1024, 1.19 ms per time step) or 64 x 4 threads x 2 points (rank 3).
+        // taller tiles leave fewer edge points.  Default: rank 2 -- the
+        // handler's own shape (DVMH_AMD_T_OWN): the widest of 256 x 4 points x
+        // 32 planes (best of the sweep over 512 grids of sides 64..1024, 1.19
+        // ms per time step) down to 64 x 1 x 8 that still gives every CU four
+        // tiles; rank 3 -- 64 x 4 threads x 2 points, 32 planes.
         if (loopRank == 3) addCand(v, 64, 4, 1, {{DVMH_AMD_T_VARIANT, 1}, {DVMH_AMD_T_TILE_A, 2}, {DVMH_AMD_T_CHUNK, 32}});
-        else addCand(v, 256, 1, 1, {{DVMH_AMD_T_VARIANT, 1}, {DVMH_AMD_T_TILE_A, 4}, {DVMH_AMD_T_CHUNK, 32}});
+        else addCand(v, 256, 1, 1, {{DVMH_AMD_T_VARIANT, 1}, {DVMH_AMD_T_TILE_A, 4}, {DVMH_AMD_T_CHUNK, 32}, {DVMH_AMD_T_OWN, 1}});
         const int mxs[] = {64, 128, 256, 512, 1024};
         const int mys[] = {1, 2, 4, 8};
         const int mcs[] = {1, 2, 4};

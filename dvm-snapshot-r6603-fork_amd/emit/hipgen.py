@@ -1752,6 +1752,30 @@ def pair_border_count(r):
     return s
 
 
+def pair_own_shape(r, lay, dlen):
+    """Handler lines: with DVMH_AMD_T_OWN (the default configuration) a rank-2
+    pair takes the widest march shape that still gives every CU four tiles
+    over all pairs of the call (one 2048^2 grid and 512 grids of 64..1024 need
+    different shapes)."""
+    if r != 2:
+        return []
+    c0, c1 = lay['bounds'] + 1, lay['bounds'] + 4
+    return [IND + 'if (tune[DVMH_AMD_T_OWN] == 1) {',
+            IND * 2 + '// (threads, points per thread, planes), widest first',
+            IND * 2 + 'static const int ladder[5][3] = {{256, 4, 32}, {256, 2, 32}, {128, 2, 16}, {64, 2, 16}, {64, 1, 8}};',
+            IND * 2 + 'int q = 0;',
+            IND * 2 + 'for (; q < 4; q++) {',
+            IND * 3 + 'long t = 0;',
+            IND * 3 + f'for (long li = 0; li < np; li++) {{ const long *d = desc + li * {dlen}; '
+                      f'const long w = (long)ladder[q][0] * ladder[q][1]; t += ((d[{c1}] + w - 1) / w) * ((d[{c0}] + ladder[q][2] - 1) / ladder[q][2]); }}',
+            IND * 3 + 'if (t >= 1024) break;',
+            IND * 2 + '}',
+            IND * 2 + 'threads = dim3((unsigned)ladder[q][0], 1, 1);',
+            IND * 2 + 'mcols = ladder[q][1];',
+            IND * 2 + 'march = ladder[q][2];',
+            IND + '}']
+
+
 def pair_handler(L1, L2):
     r = L1.rank
     P = pair_name(L1, L2)
@@ -1826,6 +1850,7 @@ def pair_handler(L1, L2):
           *[IND + x for x in regs_query(f'{P}_pair_kernel<int, 2, 3>', 'lr0')],
           IND + 'mcols = (int)tune[DVMH_AMD_T_TILE_A];',
           IND + 'march = (long)tune[DVMH_AMD_T_CHUNK];',
+          *pair_own_shape(r, lay1, dlen),
           IND + f'if (tune[DVMH_AMD_T_VARIANT] != 1 || !({shape_ok}) || mcols < 1 || mcols > 4 || march < 2) fused = false;',
           '}',
           'if (fused) {',

@@ -42,7 +42,8 @@ extern "C" {
 #define DVMH_AMD_T_TILE_A 4    /* WAVEFRONT: tile extent on the first dependent axis; LINE: chunk steps */
 #define DVMH_AMD_T_TILE_B 5    /* WAVEFRONT: tile extent on the second dependent axis */
 #define DVMH_AMD_T_TRANSFORM 6 /* 1: arrays may be re-laid out for this loop */
-#define DVMH_AMD_T_OWN 7       /* BATCH: 1 = a batch of stencil loops runs its handler's own LDS march shape */
+#define DVMH_AMD_T_OWN 7       /* BATCH: 1 = a batch of stencil loops runs its handler's own LDS march shape;
+                                  PAIR: 1 = the pair handler picks its march shape from the pairs' sizes */
 void dvmh_amd_loop_get_tuning_C(DvmType curLoop, DvmType kind, DvmType tune[]);
 
 /* ---- kernel timing -----------------------------------------------------------
@@ -67,6 +68,10 @@ void dvmh_amd_set_sync_mode_C(DvmType mode);
  * still run lazily (their loops at the region's end, consecutive pairs fused)
  * but nothing is captured, so every launch can carry timing marks. */
 void dvmh_amd_set_graph_C(DvmType enable);
+/* A lazy region is captured once; its graph is replayed only when it has at
+ * least `nodes` nodes (kernels and copies; default 6, DVMH_AMD_GRAPH_MIN_NODES):
+ * smaller steps run faster launched one by one.  Negative: the default. */
+void dvmh_amd_set_graph_min_nodes_C(DvmType nodes);
 
 /* ---- statistics ------------------------------------------------------------
  * out[0] loops run on the host, out[1] loops run on the GPU,

@@ -17,6 +17,7 @@ def _rt():
     r = CM.rt()
     r.dvmh_amd_set_graph_C.argtypes = [ctypes.c_long]
     r.dvmh_amd_set_sync_mode_C.argtypes = [ctypes.c_long]
+    r.dvmh_amd_set_graph_min_nodes_C.argtypes = [ctypes.c_long]
     return r
 
 
@@ -27,6 +28,7 @@ class _Form:
     def __enter__(self):
         r = _rt()
         r.dvmh_amd_set_graph_C(self.graph)
+        r.dvmh_amd_set_graph_min_nodes_C(0)   # replay every captured region, however small
         r.dvmh_amd_set_sync_mode_C(self.sync)
         r.dvmh_amd_tuner_set_mode_C(self.mode)
         r.dvmh_amd_tuner_clear_C()
@@ -37,6 +39,7 @@ class _Form:
         r = _rt()
         self.after = D.stats()
         r.dvmh_amd_set_graph_C(0)
+        r.dvmh_amd_set_graph_min_nodes_C(-1)
         r.dvmh_amd_set_sync_mode_C(0)
         r.dvmh_amd_tuner_set_mode_C(0)
         r.dvmh_amd_tuner_clear_C()
