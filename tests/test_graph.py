@@ -170,3 +170,28 @@ def test_cost_model_form_at_full_size(prog, n):
     assert pending == 0
     assert same(second, oracle(n, final))
     assert replays > 0
+
+
+@pytest.mark.gpu
+def test_small_regions_run_eagerly_by_default():
+    """Under the default replay threshold a region whose captured graph has
+    few nodes (2-D Jacobi's fused pair: three kernels) runs eagerly after its
+    capture, while the ADI step (seven loops) is replayed; both bit-exact."""
+    r = _rt()
+    r.dvmh_amd_set_graph_C(1)
+    r.dvmh_amd_set_sync_mode_C(1)
+    try:
+        s0 = D.stats()
+        res = H.run_jac2d(130, 12, flags=D.REGION_AMD_GRAPH)
+        s1 = D.stats()
+        ra = H.run_adi3d(40, 6, flags=D.REGION_AMD_GRAPH)
+        s2 = D.stats()
+    finally:
+        r.dvmh_amd_set_graph_C(0)
+        r.dvmh_amd_set_sync_mode_C(0)
+    ref = H.oracle_jac2d(130, 12)
+    assert H.bits_equal(res['A'], ref['A']) and np.array_equal(res['eps'], ref['eps'])
+    refa = H.oracle_adi3d(40, 6)
+    assert H.bits_equal(ra['A'], refa['A']) and np.array_equal(ra['max'], refa['max'])
+    assert s1['graph_replays'] == s0['graph_replays']
+    assert s2['graph_replays'] > s1['graph_replays']
