@@ -1498,12 +1498,48 @@ def pair_tail_params(L1, L2):
     return tp
 
 
+def lds_reads(L1, L2):
+    """For each of L1's stencil arrays that L2 reads (at its point), the text
+    spans of those reads in L2's body when every one of them comes before the
+    first write of the array in the body text and the body has no loop: such a
+    read yields the value L1's LDS window holds at the point (the value before
+    L2, nothing having written it yet), so it need not go back to memory."""
+    if re.search(r'\b(for|while|do|goto)\b', L2.body):
+        return {}
+    out = {}
+    for p in L2.arrays:
+        nm = p[1]
+        if nm not in L1.stencil:
+            continue
+        reads, writes = [], []
+        for m in re.finditer(r'\b' + nm + r'\s*((?:\[[^\[\]]*\]\s*)+)(\s*' + WRITE_OP + r')?', L2.body):
+            if m.group(2):
+                writes.append(m.start())
+            else:
+                reads.append((m.start(), m.end(1)))
+        if not reads or (writes and max(r[0] for r in reads) > min(writes)):
+            continue
+        out[nm] = reads
+    return out
+
+
 def pair_tail_scope(L1, L2, tname):
     """L2's names bound for one point (in a scope of their own, so they may
     shadow L1's), its loop variables from the point's offsets tname(axis), then
-    its body.  Arrays L1 also has use L1's base (the same array)."""
+    its body.  Arrays L1 also has use L1's base (the same array); reads that
+    lds_reads allows take the point's value from L1's LDS window, captured
+    before the array's name is rebound."""
     n1 = {p[1] for p in L1.params}
     s = []
+    body = L2.body
+    spans = []
+    for nm, reads in lds_reads(L1, L2).items():
+        ct = [p for p in L2.arrays if p[1] == nm][0][2]
+        idx = ''.join(f'[(IndexT)(lo{i} + {tname(i)})]' for i in range(L2.rank))
+        s.append(f'const {ct} dvmhc_{nm} = {nm}{idx};   // the point\'s value in the LDS window')
+        spans += [(a, b, f'dvmhc_{nm} ') for a, b in reads]
+    for a, b, t in sorted(spans, reverse=True):
+        body = body[:a] + t + body[b:]
     for p in L2.arrays:
         _, nm, ct, ar = p
         pre = '' if nm in n1 else 'p2_'
@@ -1522,7 +1558,7 @@ def pair_tail_scope(L1, L2, tname):
         s.append(pr + ';')
     for i in range(L2.rank):
         s.append(var_assign(L2, i, tname(i), f'lo{i}', '1'))
-    s.append('{\n' + indent(L2.body + '\n', 1) + '}')
+    s.append('{\n' + indent(body + '\n', 1) + '}')
     return s
 
 

@@ -109,7 +109,7 @@ def test_pair_handlers_registered():
     """The converted library registers one pair handler per fusable pair and
     exports it (the runtime calls it through the registration)."""
     lib = D.load('libcdv_pairs.so')
-    for name in ('loop_pairs_cdv_51__loop_pairs_cdv_58_cuda_pair', 'loop_pairs_cdv_127__loop_pairs_cdv_135_cuda_pair'):
+    for name in ('loop_pairs_cdv_53__loop_pairs_cdv_60_cuda_pair', 'loop_pairs_cdv_130__loop_pairs_cdv_138_cuda_pair'):
         assert hasattr(lib, name)
 
 
@@ -256,3 +256,18 @@ def test_pairs_registered_in_emitted_code(tmp_path):
     text = open(os.path.join(str(tmp_path), 't_hip.hip')).read()
     assert text.count('dvmh_amd_register_pair_C(') == 1
     assert 'l1__l2_cuda_pair' in text and 'l2__l3' not in text
+
+
+def test_lds_reads_only_before_writes():
+    """The fused second loop takes an array's value at its point from the first
+    loop's LDS window only when every read of it comes before its first write
+    in the body text (and the body has no loop); otherwise it reads memory."""
+    hg, (a, b) = _loops(_stencil(), _pointwise())
+    assert list(hg.lds_reads(a, b)) == ['A']
+    hg, (a, b) = _loops(_stencil(), _pointwise(body='A[i][j] = A[i][j] * 0.5 + B[i][j];', reductions=[]))
+    assert hg.lds_reads(a, b) == {}
+    hg, (a, b) = _loops(_stencil(), _pointwise(body='double t = A[i][j];\nfor (int r = 0; r < 2; r++) t = t * 0.5;\nA[i][j] = t;',
+                                              reductions=[]))
+    assert hg.lds_reads(a, b) == {}
+    hg, (a, b) = _loops(_stencil(), _pointwise(body='B[i][j] = A[i][j];', reductions=[]))
+    assert list(hg.lds_reads(a, b)) == ['A']
