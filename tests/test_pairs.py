@@ -271,3 +271,20 @@ def test_lds_reads_only_before_writes():
     assert hg.lds_reads(a, b) == {}
     hg, (a, b) = _loops(_stencil(), _pointwise(body='B[i][j] = A[i][j];', reductions=[]))
     assert list(hg.lds_reads(a, b)) == ['A']
+
+
+@pytest.mark.gpu
+def test_pairs_declined_run_loop_by_loop():
+    """With 64-bit indices forced the pair handler declines (its kernels use
+    32-bit indices): the runtime then runs each queued pair's two loops through
+    their own handlers, in program order, with the same results."""
+    r = D.runtime()
+    r.dvmh_amd_set_index64_C.argtypes = [ctypes.c_long]
+    r.dvmh_amd_set_index64_C(1)
+    try:
+        before = pair_stats()
+        check((37, 300), 3, 1)
+        check((6, 5, 7), 2, 1)
+        assert pair_stats() == before
+    finally:
+        r.dvmh_amd_set_index64_C(0)
