@@ -720,7 +720,10 @@ def march_kernel(L, tail=None):
     fp += ['IndexT nbx', 'IndexT nby', 'IndexT nbz', 'IndexT march']
     # the march over one tile is a device function: the per-loop kernel calls it
     # for its workgroup's tile, the batch kernel for every tile it walks
-    tp = list(fp) + ['long tile']
+    # xshift: the x-tiling starts xshift points before the loop's first point
+    # (tile 0 is that much narrower), so that later tiles begin where the
+    # written array's rows are 128-byte aligned; batch and pair tiles pass 0
+    tp = list(fp) + ['long tile', 'IndexT xshift']
     for red in L.reds:
         n = L.rlen(red)
         tp.append(f'{red[2]} (&{red[1]})[{n}]' if n > 1 else f'{red[2]} &{red[1]}')
@@ -742,7 +745,7 @@ def march_kernel(L, tail=None):
     b.append('const IndexT by = (IndexT)(tile % nby); tile /= nby;')
     b.append('const IndexT bz = (IndexT)tile;')
     b.append('const IndexT tileX = (IndexT)blockDim.x * COLS;')
-    b.append(f'const IndexT tx0 = bx * tileX;')
+    b.append(f'const IndexT tx0 = bx * tileX - xshift;')
     if ax_y is not None:
         b.append('const IndexT tileY = (IndexT)blockDim.y;')
         b.append('const IndexT ty0 = by * tileY;')
@@ -856,7 +859,7 @@ def march_kernel(L, tail=None):
     inner.append('#pragma unroll')
     inner.append('for (int c = 0; c < COLS; c++) {')
     pt = [f'const IndexT t{ax_x} = tx0 + (IndexT)threadIdx.x + (IndexT)c * (IndexT)blockDim.x;']
-    cc = [f't{ax_x} < cnt{ax_x}'] + ([f't{ax_y} < cnt{ax_y}'] if ax_y is not None else [])
+    cc = [f't{ax_x} >= 0', f't{ax_x} < cnt{ax_x}'] + ([f't{ax_y} < cnt{ax_y}'] if ax_y is not None else [])
     pt.append(f'if ({" && ".join(cc)}) {{')
     for i in range(r):
         pt.append(IND + var_assign(L, i, f't{i}', f'lo{i}', '1'))   # the variant runs unit steps only
@@ -875,7 +878,7 @@ def march_kernel(L, tail=None):
         return '\n'.join(k) + '\n'
     # the per-loop kernel: one tile per workgroup
     names = [p_[1] for p_ in L.params]
-    kp = list(fp[:-4]) + ['IndexT nbx', 'IndexT nby', 'IndexT nbz', 'long nblocks', 'int xcd', 'IndexT march'] + red_kernel_params(L)
+    kp = list(fp[:-4]) + ['IndexT nbx', 'IndexT nby', 'IndexT nbz', 'long nblocks', 'int xcd', 'IndexT march', 'IndexT xshift'] + red_kernel_params(L)
     call = []
     for p_ in L.arrays:
         _, nm, ct, ar = p_
@@ -884,7 +887,7 @@ def march_kernel(L, tail=None):
         call.append(f'{p_[1]}_ptr')
     for i in range(r):
         call += [f'lo{i}', f'cnt{i}', f'st{i}']
-    call += ['nbx', 'nby', 'nbz', 'march', 'dvmh_dev::xcdTile((long)blockIdx.x, nblocks, xcd)']
+    call += ['nbx', 'nby', 'nbz', 'march', 'dvmh_dev::xcdTile((long)blockIdx.x, nblocks, xcd)', 'xshift']
     for red in L.reds:
         call.append(red[1])
         if len(red) > 3:
@@ -938,8 +941,18 @@ def parallel_handler(L):
         b.append(IND + 'variant = 0;')
         b.append('}')
         b.append('DvmType nbx, nby, nbz;')
+        b.append('DvmType xshift = 0;')
         b.append('if (variant == 1) {')
-        b.append(IND + f'nbx = (cnt{ax_x} + (DvmType)threads.x * mcols - 1) / ((DvmType)threads.x * mcols);')
+        # tiles after the first start where the rows of the array the body
+        # writes are 128-byte aligned (full cache lines stored)
+        wr = [p for p in L.arrays if p[1] not in L.stencil and p[3] == r and
+              re.search(r'\b' + p[1] + r'\s*(?:\[[^\]]*\]\s*)+(?:=(?!=)|\+=|-=|\*=|/=)', L.body)]
+        if wr:
+            _, wn, wct, _ = wr[0]
+            addr = ' + '.join([wn] + [f'boundsLow[{k}] * {wn}_devHdr[{k + 1}]' for k in range(r)])
+            b.append(IND + f'{{ const uintptr_t a = (uintptr_t)({addr});   /* the first point of {wn} */')
+            b.append(IND + f'  if ({wn}_devHdr[{r}] == 1 && a % sizeof({wct}) == 0) xshift = (DvmType)((a / sizeof({wct})) % (128 / sizeof({wct}))); }}')
+        b.append(IND + f'nbx = (cnt{ax_x} + xshift + (DvmType)threads.x * mcols - 1) / ((DvmType)threads.x * mcols);')
         b.append(IND + (f'nby = (cnt{ax_y} + threads.y - 1) / threads.y;' if r == 3 else 'nby = 1;'))
         b.append(IND + 'nbz = (cnt0 + march - 1) / march;')
         b.append('} else {')
@@ -975,7 +988,7 @@ def parallel_handler(L):
         args = array_fact_args(L, it)
         for i in range(r):
             args += [f'({it})boundsLow[{i}]', f'({it})cnt{i}', f'({it})loopSteps[{i}]']
-        args += [f'({it})nbx', f'({it})nby', f'({it})nbz', 'nblocks', '(int)tune[DVMH_AMD_T_XCD]', f'({it})march']
+        args += [f'({it})nbx', f'({it})nby', f'({it})nbz', 'nblocks', '(int)tune[DVMH_AMD_T_XCD]', f'({it})march', f'({it})xshift']
         args += red_fact_args(L)
         for cols, sets in MARCH_NPF.items():
             for v in sets:
@@ -1188,7 +1201,7 @@ def batch_march_kernel(L):
     for i in range(r):
         args += [f'(IndexT)d[{bo + 3 * i}]', f'(IndexT)d[{bo + 3 * i + 1}]', f'(IndexT)d[{bo + 3 * i + 2}]']
     nbo = lay['nb']
-    args += [f'(IndexT)d[{nbo}]', f'(IndexT)d[{nbo + 1}]', f'(IndexT)d[{nbo + 2}]', 'march', 'tile - tileStart[lo_]']
+    args += [f'(IndexT)d[{nbo}]', f'(IndexT)d[{nbo + 1}]', f'(IndexT)d[{nbo + 2}]', 'march', 'tile - tileStart[lo_]', '(IndexT)0']
     for red in L.reds:
         t.append(f'{red[2]} {red[1]} = *(const {red[2]} *)&d[{lay[("ri", red[1])]}];')
         args.append(red[1])
@@ -1634,7 +1647,7 @@ def pair_kernels(L1, L2):
     for i in range(r):
         args += [f'(IndexT)d[{bo + 3 * i}]', f'(IndexT)d[{bo + 3 * i + 1}]', f'(IndexT)d[{bo + 3 * i + 2}]']
     nbo = lay1['nb']
-    args += [f'(IndexT)d[{nbo}]', f'(IndexT)d[{nbo + 1}]', f'(IndexT)d[{nbo + 2}]', 'march', 'tile - tileStart[lo_]']
+    args += [f'(IndexT)d[{nbo}]', f'(IndexT)d[{nbo + 1}]', f'(IndexT)d[{nbo + 2}]', 'march', 'tile - tileStart[lo_]', '(IndexT)0']
     for red in L1.reds:
         t.append(f'{red[2]} {red[1]} = *(const {red[2]} *)&d[{lay1[("ri", red[1])]}];')
         args.append(red[1])
