@@ -1785,8 +1785,8 @@ def red_kernel_locals_from(L, lay):
 
 
 def pair_border_count(r):
-    """Host text: edge points of one pair (see the border kernel), from cnt0..,
-    the tile sizes T0 (march), T1 (rows, rank 3), Tx and tile counts nb*."""
+    """Host text: `edges`, the edge points of one pair (numbered as the border
+    kernel numbers them), from the extents cnt* and the tile counts nb*."""
     ax = [0, 1, 2] if r == 3 else [0, 1]
     tcnt = {0: 'nbz', ax[-1]: 'nbx'}
     if r == 3:
