@@ -2647,6 +2647,206 @@ def line_tile_launch(L):
     return b
 
 
+# Chunk lengths of the streamed line sweep (one aligned chunk of CK elements
+# per thread in flight while the previous one is walked).
+LINE_VEC_CK = [16, 32]
+
+
+def line_vec_kernel(L):
+    """Streamed line sweep for an ACROSS loop whose only dependent axis is the
+    innermost one, on a unit-stride line: one thread per line, no LDS.  The
+    line is cut into chunks of CK elements aligned to CK elements of memory.
+    A full chunk (every slot a step of the loop) is read and written with
+    whole 16-byte vector accesses; the partial first and last chunks load
+    element by element, clamped into the indices the body reads, and store
+    only their steps.  The walk runs in registers (RegLineView: slot 0 the
+    element behind in loop order, already updated; CK + 1 the one ahead, not
+    yet updated).  Full chunks run in a loop without branches, unrolled by two
+    so the two chunk buffers keep fixed roles: the loads of chunk c + 1 are
+    issued before chunk c is walked and stored, and the wait before the walk
+    of c + 1 then leaves the stores of c in flight -- one chunk of loads and
+    one of stores overlap every walk."""
+    r = L.rank
+    kx = r - 1
+    ind = list(range(r - 1))
+    X = L.across['array']
+    xct = [p for p in L.arrays if p[1] == X][0][2]
+    dmin, dmax = L.stencil_range(X)[kx]
+    fp, setup = kernel_array_params(L, unit=False)
+    for i in range(r):
+        fp += [f'IndexT lo{i}', f'IndexT cnt{i}', f'IndexT st{i}']
+    fp += ['IndexT nlines']
+    fp += red_kernel_params(L)
+    k = ['template <typename IndexT, int CK, int DIR>',
+         '// one or two waves per SIMD (a thread per line): registers go to chunks in flight',
+         f'__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) void {L.name}_lvec_kernel({", ".join(fp)}) {{']
+    b = list(setup)
+    for v, ct in L.vars:
+        b.append(f'{ct} {v};')
+    for pr in L.privates:
+        b.append(pr + ';')
+    b += red_kernel_locals(L)
+    b.append('const IndexT lv_line = (IndexT)blockIdx.x * (IndexT)blockDim.x + (IndexT)threadIdx.x;')
+    b.append(f'if (lv_line < nlines && cnt{kx} > 0) {{')
+    inner = []
+    if r == 3:
+        inner.append('const IndexT t1 = lv_line % cnt1, t0 = lv_line / cnt1;')
+    else:
+        inner.append('const IndexT t0 = lv_line;')
+    for a in ind:
+        inner.append(var_assign(L, a, f't{a}', f'lo{a}', f'st{a}'))
+    inner.append(f'{xct} *const lv_row = {X}_base' + ''.join(f' + (IndexT){L.vars[a][0]} * {X}_s{a}' for a in ind) +
+                 ';   // this thread\'s line, unit stride along it')
+    inner += [f'constexpr int LV_W = 16 / (int)sizeof({xct});   // elements per 16-byte access',
+              f'typedef {xct} lv_vec __attribute__((ext_vector_type(LV_W)));',
+              f'// the body reads the element ahead in loop order (offset {dmax} forward, {dmin} backward)',
+              f'constexpr bool LV_AHEAD = DIR > 0 ? ({dmax}) > 0 : ({dmin}) < 0;',
+              f'const IndexT lv_kA = lo{kx}, lv_kZ = lo{kx} + (cnt{kx} - 1) * (IndexT)DIR;',
+              f'// indices along the line the body reads (offsets {dmin} .. {dmax}): edge loads are clamped into them',
+              f'const IndexT lv_rlo = (DIR > 0 ? lv_kA : lv_kZ) + ({dmin}), lv_rhi = (DIR > 0 ? lv_kZ : lv_kA) + ({dmax});',
+              f'auto lv_ld = [&](IndexT x) -> {xct} {{ x = x < lv_rlo ? lv_rlo : (x > lv_rhi ? lv_rhi : x); return lv_row[x]; }};',
+              '// lv_v loop steps precede the first one in its aligned chunk; chunk c holds',
+              '// steps c * CK - lv_v .. + CK - 1 in slots 1 .. CK; chunks lv_cF .. lv_cL are full',
+              f'const IndexT lv_p = (IndexT)(((uintptr_t)(lv_row + lv_kA) / sizeof({xct})) & (uintptr_t)(CK - 1));',
+              'const IndexT lv_v = DIR > 0 ? lv_p : (IndexT)(CK - 1) - lv_p;',
+              f'const IndexT lv_nch = (lv_v + cnt{kx} + (IndexT)(CK - 1)) / (IndexT)CK;',
+              f'const IndexT lv_cF = lv_v > 0 ? 1 : 0, lv_cL = (lv_v + cnt{kx}) / (IndexT)CK - 1;',
+              'auto lv_x0 = [&](IndexT lv_cc) -> IndexT { return lv_kA + (lv_cc * (IndexT)CK - lv_v) * (IndexT)DIR; };   // index of slot 1',
+              f'{xct} lv_a[CK], lv_b[CK], lv_ahA = 0, lv_ahB = 0;',
+              f'{xct} lv_carry = lv_ld(lv_x0(0) - (IndexT)DIR);']
+
+    def vload(dst, ah, c):
+        # a full chunk: whole vectors, never outside the indices the body reads
+        return ['{',
+                IND + f'const IndexT x0 = lv_x0({c}), low = DIR > 0 ? x0 : x0 - (IndexT)(CK - 1);',
+                IND + 'const lv_vec *lv_q = (const lv_vec *)__builtin_assume_aligned(lv_row + low, CK * sizeof(' + xct + '));',
+                IND + '#pragma unroll',
+                IND + 'for (int m = 0; m < CK / LV_W; m++) {',
+                IND * 2 + 'const lv_vec u = lv_q[m];',
+                IND * 2 + '#pragma unroll',
+                IND * 2 + f'for (int e = 0; e < LV_W; e++) {dst}[DIR > 0 ? m * LV_W + e : CK - 1 - (m * LV_W + e)] = u[e];',
+                IND + '}',
+                IND + f'if constexpr (LV_AHEAD) {ah} = lv_ld(x0 + (IndexT)CK * (IndexT)DIR);',
+                '}']
+
+    def walk(c, guard):
+        # the loop variable is formed from the chunk's slot-1 index, the view's
+        # base: the slot arithmetic then folds to a constant in every step
+        s = ['const IndexT lv_xs = lv_x0(' + c + ');',
+             f'const dvmh_dev::RegLineView<{xct}, IndexT, {r}, {kx}, DIR> {X}{{lv_w, lv_xs, 0}};',
+             '#pragma unroll',
+             'for (int lv_s = 0; lv_s < CK; lv_s++) {']
+        if guard:
+            s.append(IND + f'const IndexT t{kx} = ({c}) * (IndexT)CK - lv_v + (IndexT)lv_s;')
+            s.append(IND + f'if (t{kx} >= 0 && t{kx} < cnt{kx}) {{')
+        d = 2 if guard else 1
+        s.append(IND * d + var_assign(L, kx, '(IndexT)lv_s', 'lv_xs', '(IndexT)DIR'))
+        s.append(indent('{\n' + indent(L.body + '\n', 1) + '}', d))
+        if guard:
+            s.append(IND + '}')
+        s.append('}')
+        return s
+
+    def full(buf, ah, c):
+        s = ['{',
+             IND + f'{xct} lv_w[CK + 2];',
+             IND + 'lv_w[0] = lv_carry;',
+             IND + '#pragma unroll',
+             IND + f'for (int m = 0; m < CK; m++) lv_w[1 + m] = {buf}[m];',
+             IND + f'lv_w[CK + 1] = {ah};']
+        s += [IND + ln for ln in walk(c, False)]
+        s += [IND + f'const IndexT low = DIR > 0 ? lv_x0({c}) : lv_x0({c}) - (IndexT)(CK - 1);',
+              IND + f'lv_vec *lv_q = (lv_vec *)__builtin_assume_aligned(lv_row + low, CK * sizeof({xct}));',
+              IND + '#pragma unroll',
+              IND + 'for (int m = 0; m < CK / LV_W; m++) {',
+              IND * 2 + 'lv_vec u;',
+              IND * 2 + '#pragma unroll',
+              IND * 2 + 'for (int e = 0; e < LV_W; e++) u[e] = lv_w[1 + (DIR > 0 ? m * LV_W + e : CK - 1 - (m * LV_W + e))];',
+              IND * 2 + 'lv_q[m] = u;',
+              IND + '}',
+              IND + 'lv_carry = lv_w[CK];',
+              '}']
+        return s
+
+    def partial(c):
+        s = ['{',
+             IND + f'const IndexT x0 = lv_x0({c});',
+             IND + f'{xct} lv_w[CK + 2];',
+             IND + 'lv_w[0] = lv_carry;',
+             IND + '#pragma unroll',
+             IND + 'for (int m = 0; m < CK + 1; m++) lv_w[1 + m] = lv_ld(x0 + (IndexT)m * (IndexT)DIR);']
+        s += [IND + ln for ln in walk(c, True)]
+        s += [IND + '#pragma unroll',
+              IND + 'for (int m = 0; m < CK; m++) {',
+              IND * 2 + f'const IndexT t = ({c}) * (IndexT)CK - lv_v + (IndexT)m;',
+              IND * 2 + f'if (t >= 0 && t < cnt{kx}) lv_row[x0 + (IndexT)m * (IndexT)DIR] = lv_w[1 + m];',
+              IND + '}',
+              IND + 'lv_carry = lv_w[CK];',
+              '}']
+        return s
+
+    inner.append('// the partial first chunk')
+    inner.append('if (lv_v > 0)')
+    inner += [IND + ln for ln in partial('(IndexT)0')]
+    inner.append('// full chunks: two or more run pipelined')
+    inner.append('if (lv_cL - lv_cF >= 1) {')
+    body = []
+    body += vload('lv_a', 'lv_ahA', 'lv_cF')
+    body += vload('lv_b', 'lv_ahB', 'lv_cF + 1')
+    body += full('lv_a', 'lv_ahA', 'lv_cF')
+    body.append('IndexT lv_c = lv_cF + 1;   // lv_b holds chunk lv_c while lv_c <= lv_cL')
+    body.append('while (lv_c + 1 <= lv_cL) {')
+    loop = []
+    loop += vload('lv_a', 'lv_ahA', 'lv_c + 1')
+    loop += full('lv_b', 'lv_ahB', 'lv_c')
+    loop.append('// the last pair reloads chunk lv_cL: no branch, its values are not used')
+    loop += vload('lv_b', 'lv_ahB', '(lv_c + 2 <= lv_cL ? lv_c + 2 : lv_cL)')
+    loop += full('lv_a', 'lv_ahA', 'lv_c + 1')
+    loop.append('lv_c += 2;')
+    body += [IND + ln for ln in loop]
+    body.append('}')
+    body.append('if (lv_c == lv_cL)')
+    body += [IND + ln for ln in full('lv_b', 'lv_ahB', 'lv_c')]
+    inner += [IND + ln for ln in body]
+    inner.append('} else if (lv_cL == lv_cF) {')
+    one = vload('lv_a', 'lv_ahA', 'lv_cF') + full('lv_a', 'lv_ahA', 'lv_cF')
+    inner += [IND + ln for ln in one]
+    inner.append('}')
+    inner.append('// the partial last chunk (unless it is the first one, done above)')
+    inner.append('if (lv_nch - 1 > lv_cL && !(lv_nch == 1 && lv_v > 0)) {')
+    inner.append(IND + 'const IndexT lv_c = lv_nch - 1;')
+    inner += [IND + ln for ln in partial('lv_c')]
+    inner.append('}')
+    b += [IND + ln for ln in '\n'.join(inner).split('\n')]
+    b.append('}')
+    b += red_kernel_tail(L, 'blockIdx.x', accumulate=False)
+    k.append(indent('\n'.join(b) + '\n', 1).rstrip('\n'))
+    k.append('}')
+    return '\n'.join(k) + '\n'
+
+
+def line_vec_launch(L):
+    r = L.rank
+    b = ['DvmType nlines = 1' + ''.join(f' * cnt{a}' for a in range(r - 1)) + ';',
+         'DvmType nblocks = (nlines + threads.x - 1) / threads.x;']
+    b += handler_reds_prepare(L, 'nblocks', 0)
+    b.append('if (nblocks > 0) {')
+    b.append(IND + 'dvmh_amd_mark_kernel_C(loop_ref, 0);')
+    args = array_fact_args(L, 'int')
+    for i in range(r):
+        args += [f'(int)boundsLow[{i}]', f'(int)cnt{i}', f'(int)loopSteps[{i}]']
+    args += ['(int)nlines']
+    args += red_fact_args(L)
+    for ck in LINE_VEC_CK:
+        for d in (1, -1):
+            b.append(IND + f'if (CK == {ck} && loopSteps[{r - 1}] == {d}) hipLaunchKernelGGL(({L.name}_lvec_kernel<int, {ck}, {d}>), '
+                     f'dim3((unsigned)nblocks), threads, 0, stream, {", ".join(args)});')
+    b.append(IND + 'dvmh_amd_mark_kernel_C(loop_ref, 1);')
+    b.append('}')
+    b += handler_reds_finish(L)
+    return b
+
+
 def tile_launch(L):
     X = L.across['array']
     xct = [p for p in L.arrays if p[1] == X][0][2]
@@ -2844,7 +3044,13 @@ def across_handler(L):
             cks = ' || '.join(f'CK == {ck}' for ck in LINE_TILE_CK)
             b.append(IND + f'bool ltile = tune[DVMH_AMD_T_VARIANT] == 1 && (loopSteps[{r - 1}] == 1 || loopSteps[{r - 1}] == -1) && '
                      f'{X}_tfm == 0 && threads.x <= 256 && threads.y == 1 && threads.z == 1 && ({cks}) && ldsBytes <= 64512;')
-            b.append(IND + 'if (ltile) {')
+            vcks = ' || '.join(f'CK == {ck}' for ck in LINE_VEC_CK)
+            b.append(IND + '// streamed line sweep (variant 3): unit stride along the line, 32-bit indices, no LDS')
+            b.append(IND + f'bool lvec = tune[DVMH_AMD_T_VARIANT] == 3 && (loopSteps[{r - 1}] == 1 || loopSteps[{r - 1}] == -1) && '
+                     f'{X}_tfm == 0 && {X}_devHdr[{r}] == 1 && kernelIndexT == rt_INT && threads.x <= 256 && threads.y == 1 && threads.z == 1 && ({vcks});')
+            b.append(IND + 'if (lvec) {')
+            b += [IND * 2 + s for s in line_vec_launch(L)]
+            b.append(IND + '} else if (ltile) {')
             b += [IND * 2 + s for s in line_tile_launch(L)]
             b.append(IND + '} else {')
             b += [IND * 2 + s for s in line_launch(L, dep)]
@@ -2945,6 +3151,7 @@ def gpu_loop(L):
             text.append(tile_kernel(L))
         if L.line_tile_eligible():
             text.append(line_tile_kernel(L))
+            text.append(line_vec_kernel(L))
         for dep in range(r - 1):
             if L.line_stage_eligible(dep):
                 text.append(line_reg_kernel(L, dep))

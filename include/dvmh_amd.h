@@ -25,7 +25,10 @@ extern "C" {
 #define DVMH_AMD_KIND_LINE 1       /* ACROSS with one dependent axis: line sweeps
                                       (VARIANT 1, dependent axis innermost: lines staged in
                                       LDS chunks of TILE_A steps; VARIANT 2, other axes: each
-                                      thread's line in register chunks of TILE_A steps) */
+                                      thread's line in register chunks of TILE_A steps;
+                                      VARIANT 3, dependent axis innermost and unit stride:
+                                      each thread streams its line in aligned chunks of
+                                      TILE_A elements, 16-byte vector accesses, no LDS) */
 #define DVMH_AMD_KIND_WAVEFRONT 2  /* ACROSS with >= 2 dependent axes: hyperplane wavefront */
 #define DVMH_AMD_KIND_STENCIL 3    /* PARALLEL whose handler also has the LDS march variant
                                       (VARIANT 1: TILE_A = points per thread on the innermost

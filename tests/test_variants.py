@@ -140,14 +140,17 @@ KIND_LINE = 1
 
 
 def _ltile(tl, ck, reg=0):
+    """reg 0: LDS chunks; 1: the walk on a register copy of the LDS chunk;
+    2: the streamed sweep (variant 3: aligned vector chunks, no LDS)."""
     tune = [0] * 8
-    tune[T_VARIANT] = 1
+    tune[T_VARIANT] = 3 if reg == 2 else 1
     tune[T_TILE_A] = ck
-    tune[5] = reg   # TILE_B: the walk on a register copy of the chunk
+    tune[5] = 1 if reg == 1 else 0   # TILE_B
     return (tl, 1, 1), tune
 
 
-LTILES = [(64, 8, 0), (128, 16, 0), (256, 8, 0), (64, 32, 0), (128, 8, 0), (64, 8, 1), (128, 16, 1), (256, 32, 1)]
+LTILES = [(64, 8, 0), (128, 16, 0), (256, 8, 0), (64, 32, 0), (128, 8, 0), (64, 8, 1), (128, 16, 1), (256, 32, 1),
+          (64, 16, 2), (128, 32, 2), (256, 16, 2)]
 
 
 @pytest.mark.gpu
@@ -213,9 +216,11 @@ def test_adi3d_register_line_sweep_full_size():
 
 
 @pytest.mark.gpu
-def test_adi3d_tiled_line_sweep_full_size():
-    """256^3 (the ADI workload's size), two steps, the walk on registers."""
-    block, tune = _ltile(128, 16, 1)
+@pytest.mark.parametrize('cfg', [(128, 16, 1), (64, 32, 2)])
+def test_adi3d_tiled_line_sweep_full_size(cfg):
+    """256^3 (the ADI workload's size), two steps: the walk on a register copy
+    of LDS chunks, and the streamed sweep (aligned vector chunks, no LDS)."""
+    block, tune = _ltile(*cfg)
     _force(KIND_LINE, block, tune)
     try:
         res = H.run_adi3d(256, 2)
