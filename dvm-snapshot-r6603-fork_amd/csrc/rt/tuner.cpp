@@ -142,12 +142,10 @@ std::vector<Candidate> makeCandidates(int kind, int loopRank, int depCount, cons
                     for (int reg = 0; reg <= 1; reg++)   // TILE_B 1: the walk runs on a register copy of the line's chunk
                         addCand(v, tl, 1, 1, {{DVMH_AMD_T_VARIANT, 1}, {DVMH_AMD_T_TILE_A, ck}, {DVMH_AMD_T_TILE_B, reg}});
                 }
-            // streamed line sweep (variant 3): each thread reads and writes its
-            // own line in aligned chunks of TILE_A elements, no LDS
-            const int vcks[] = {16, 32};
+            // streamed line sweep (variant 3): a wave's 64 lines move in aligned
+            // 128-byte chunks, coalesced, transposed through the wave's LDS rows
             for (int tl : tls)
-                for (int ck : vcks)
-                    addCand(v, tl, 1, 1, {{DVMH_AMD_T_VARIANT, 3}, {DVMH_AMD_T_TILE_A, ck}});
+                addCand(v, tl, 1, 1, {{DVMH_AMD_T_VARIANT, 3}});
         }
     } else if (kind == DVMH_AMD_KIND_PAIR) {
         // a fused pair runs the LDS march of its stencil loop, the second loop

@@ -2647,25 +2647,22 @@ def line_tile_launch(L):
     return b
 
 
-# Chunk lengths of the streamed line sweep (one aligned chunk of CK elements
-# per thread in flight while the previous one is walked).
-LINE_VEC_CK = [16, 32]
-
-
 def line_vec_kernel(L):
     """Streamed line sweep for an ACROSS loop whose only dependent axis is the
-    innermost one, on a unit-stride line: one thread per line, no LDS.  The
-    line is cut into chunks of CK elements aligned to CK elements of memory.
-    A full chunk (every slot a step of the loop) is read and written with
-    whole 16-byte vector accesses; the partial first and last chunks load
-    element by element, clamped into the indices the body reads, and store
-    only their steps.  The walk runs in registers (RegLineView: slot 0 the
-    element behind in loop order, already updated; CK + 1 the one ahead, not
-    yet updated).  Full chunks run in a loop without branches, unrolled by two
-    so the two chunk buffers keep fixed roles: the loads of chunk c + 1 are
-    issued before chunk c is walked and stored, and the wait before the walk
-    of c + 1 then leaves the stores of c in flight -- one chunk of loads and
-    one of stores overlap every walk."""
+    innermost one, on a unit-stride line: a wave owns 64 lines, a lane walks
+    one of them.  Each line is cut into chunks of 128 bytes (CK elements)
+    aligned in memory.  A chunk that is whole on every line of the wave moves
+    between HBM and registers coalesced -- 8 lanes read one line's chunk as
+    16-byte pieces, 8 lines per load -- and is transposed through the wave's
+    LDS rows (stride CK + 1), so each lane walks its own line in registers
+    (RegLineView: slot 0 the element behind in loop order, already updated;
+    CK + 1 the one ahead, not yet updated) and the results go back the same
+    way.  The loads of chunk c + 1 are in flight while chunk c is transposed,
+    walked and stored (two piece buffers with fixed roles, a loop unrolled by
+    two, the first chunk peeled so every wait leaves the previous stores in
+    flight).  The chunks before and after that range (partial, or not whole
+    on some line of the wave) are walked lane by lane with loads clamped into
+    the indices the body reads and stores of the loop's own steps only."""
     r = L.rank
     kx = r - 1
     ind = list(range(r - 1))
@@ -2677,57 +2674,59 @@ def line_vec_kernel(L):
         fp += [f'IndexT lo{i}', f'IndexT cnt{i}', f'IndexT st{i}']
     fp += ['IndexT nlines']
     fp += red_kernel_params(L)
-    k = ['template <typename IndexT, int CK, int DIR>',
-         '// one or two waves per SIMD (a thread per line): registers go to chunks in flight',
+    k = ['template <typename IndexT, int DIR>',
+         '// a thread per line leaves one or two waves per SIMD: registers go to chunks in flight',
          f'__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) void {L.name}_lvec_kernel({", ".join(fp)}) {{']
-    b = list(setup)
+    b = [f'constexpr int CK = 128 / (int)sizeof({xct});   // elements per chunk: one 128-byte line',
+         f'constexpr int LV_W = 16 / (int)sizeof({xct});   // elements per 16-byte piece',
+         'constexpr int LV_RS = CK + 1;   // LDS row stride of a line',
+         f'typedef {xct} lv_vec __attribute__((ext_vector_type(LV_W)));',
+         f'__shared__ __align__(16) {xct} lv_lds[4 * 64 * LV_RS];']
+    b += setup
     for v, ct in L.vars:
         b.append(f'{ct} {v};')
     for pr in L.privates:
         b.append(pr + ';')
     b += red_kernel_locals(L)
-    b.append('const IndexT lv_line = (IndexT)blockIdx.x * (IndexT)blockDim.x + (IndexT)threadIdx.x;')
-    b.append(f'if (lv_line < nlines && cnt{kx} > 0) {{')
-    inner = []
+    b += ['const int lv_lane = (int)(threadIdx.x & 63), lv_sub = lv_lane >> 3, lv_pc = lv_lane & 7;',
+          f'{xct} *const lv_buf = lv_lds + (threadIdx.x >> 6) * 64 * LV_RS;   // this wave\'s rows',
+          'const IndexT lv_l0 = (IndexT)blockIdx.x * (IndexT)blockDim.x + (IndexT)(threadIdx.x & ~63u);   // the wave\'s first line',
+          f'if (lv_l0 < nlines && cnt{kx} > 0) {{   // uniform over the wave']
+    inner = ['const bool lv_mine = lv_l0 + (IndexT)lv_lane < nlines;',
+             '// row of a line (lines past the end read the last one; nothing of theirs is stored)']
+    if r == 3:
+        rowof = (f'auto lv_rowOf = [&](IndexT ln) -> {xct} * {{ ln = ln < nlines ? ln : nlines - 1; '
+                 f'return {X}_base + (lo0 + (ln / cnt1) * st0) * {X}_s0 + (lo1 + (ln % cnt1) * st1) * {X}_s1; }};')
+    else:
+        rowof = f'auto lv_rowOf = [&](IndexT ln) -> {xct} * {{ ln = ln < nlines ? ln : nlines - 1; return {X}_base + (lo0 + ln * st0) * {X}_s0; }};'
+    inner.append(rowof)
+    inner.append('const IndexT lv_line = lv_mine ? lv_l0 + (IndexT)lv_lane : nlines - 1;   // the line this lane walks')
     if r == 3:
         inner.append('const IndexT t1 = lv_line % cnt1, t0 = lv_line / cnt1;')
     else:
         inner.append('const IndexT t0 = lv_line;')
     for a in ind:
         inner.append(var_assign(L, a, f't{a}', f'lo{a}', f'st{a}'))
-    inner.append(f'{xct} *const lv_row = {X}_base' + ''.join(f' + (IndexT){L.vars[a][0]} * {X}_s{a}' for a in ind) +
-                 ';   // this thread\'s line, unit stride along it')
-    inner += [f'constexpr int LV_W = 16 / (int)sizeof({xct});   // elements per 16-byte access',
-              f'typedef {xct} lv_vec __attribute__((ext_vector_type(LV_W)));',
-              f'// the body reads the element ahead in loop order (offset {dmax} forward, {dmin} backward)',
+    inner.append(f'{xct} *const lv_row = lv_rowOf(lv_line);')
+    inner += [f'// the body reads the element ahead in loop order (offset {dmax} forward, {dmin} backward)',
               f'constexpr bool LV_AHEAD = DIR > 0 ? ({dmax}) > 0 : ({dmin}) < 0;',
               f'const IndexT lv_kA = lo{kx}, lv_kZ = lo{kx} + (cnt{kx} - 1) * (IndexT)DIR;',
-              f'// indices along the line the body reads (offsets {dmin} .. {dmax}): edge loads are clamped into them',
+              f'// indices along the line the body reads (offsets {dmin} .. {dmax}): lane loads are clamped into them',
               f'const IndexT lv_rlo = (DIR > 0 ? lv_kA : lv_kZ) + ({dmin}), lv_rhi = (DIR > 0 ? lv_kZ : lv_kA) + ({dmax});',
               f'auto lv_ld = [&](IndexT x) -> {xct} {{ x = x < lv_rlo ? lv_rlo : (x > lv_rhi ? lv_rhi : x); return lv_row[x]; }};',
+              f'auto lv_pos = [&]({xct} *row) -> IndexT {{ return (IndexT)(((uintptr_t)(row + lv_kA) / sizeof({xct})) & (uintptr_t)(CK - 1)); }};',
               '// lv_v loop steps precede the first one in its aligned chunk; chunk c holds',
-              '// steps c * CK - lv_v .. + CK - 1 in slots 1 .. CK; chunks lv_cF .. lv_cL are full',
-              f'const IndexT lv_p = (IndexT)(((uintptr_t)(lv_row + lv_kA) / sizeof({xct})) & (uintptr_t)(CK - 1));',
+              '// steps c * CK - lv_v .. + CK - 1 in slots 1 .. CK; chunks up to lv_cL are whole from chunk 1 on',
+              'const IndexT lv_p = lv_pos(lv_row);',
               'const IndexT lv_v = DIR > 0 ? lv_p : (IndexT)(CK - 1) - lv_p;',
               f'const IndexT lv_nch = (lv_v + cnt{kx} + (IndexT)(CK - 1)) / (IndexT)CK;',
-              f'const IndexT lv_cF = lv_v > 0 ? 1 : 0, lv_cL = (lv_v + cnt{kx}) / (IndexT)CK - 1;',
+              f'const IndexT lv_cL = (lv_v + cnt{kx}) / (IndexT)CK - 1;',
               'auto lv_x0 = [&](IndexT lv_cc) -> IndexT { return lv_kA + (lv_cc * (IndexT)CK - lv_v) * (IndexT)DIR; };   // index of slot 1',
-              f'{xct} lv_a[CK], lv_b[CK], lv_ahA = 0, lv_ahB = 0;',
-              f'{xct} lv_carry = lv_ld(lv_x0(0) - (IndexT)DIR);']
-
-    def vload(dst, ah, c):
-        # a full chunk: whole vectors, never outside the indices the body reads
-        return ['{',
-                IND + f'const IndexT x0 = lv_x0({c}), low = DIR > 0 ? x0 : x0 - (IndexT)(CK - 1);',
-                IND + 'const lv_vec *lv_q = (const lv_vec *)__builtin_assume_aligned(lv_row + low, CK * sizeof(' + xct + '));',
-                IND + '#pragma unroll',
-                IND + 'for (int m = 0; m < CK / LV_W; m++) {',
-                IND * 2 + 'const lv_vec u = lv_q[m];',
-                IND * 2 + '#pragma unroll',
-                IND * 2 + f'for (int e = 0; e < LV_W; e++) {dst}[DIR > 0 ? m * LV_W + e : CK - 1 - (m * LV_W + e)] = u[e];',
-                IND + '}',
-                IND + f'if constexpr (LV_AHEAD) {ah} = lv_ld(x0 + (IndexT)CK * (IndexT)DIR);',
-                '}']
+              '// chunks 1 .. lv_w1 are whole on every line of the wave',
+              'IndexT lv_w1 = lv_mine ? lv_cL : (IndexT)0x7fffffff;',
+              'for (int o = 32; o > 0; o >>= 1) { const IndexT q = __shfl_xor(lv_w1, o); lv_w1 = q < lv_w1 ? q : lv_w1; }',
+              'lv_w1 = __builtin_amdgcn_readfirstlane(lv_w1);',
+              f'{xct} lv_carry = lv_ld(lv_x0(0) - (IndexT)DIR), lv_ah = 0;']
 
     def walk(c, guard):
         # the loop variable is formed from the chunk's slot-1 index, the view's
@@ -2747,27 +2746,6 @@ def line_vec_kernel(L):
         s.append('}')
         return s
 
-    def full(buf, ah, c):
-        s = ['{',
-             IND + f'{xct} lv_w[CK + 2];',
-             IND + 'lv_w[0] = lv_carry;',
-             IND + '#pragma unroll',
-             IND + f'for (int m = 0; m < CK; m++) lv_w[1 + m] = {buf}[m];',
-             IND + f'lv_w[CK + 1] = {ah};']
-        s += [IND + ln for ln in walk(c, False)]
-        s += [IND + f'const IndexT low = DIR > 0 ? lv_x0({c}) : lv_x0({c}) - (IndexT)(CK - 1);',
-              IND + f'lv_vec *lv_q = (lv_vec *)__builtin_assume_aligned(lv_row + low, CK * sizeof({xct}));',
-              IND + '#pragma unroll',
-              IND + 'for (int m = 0; m < CK / LV_W; m++) {',
-              IND * 2 + 'lv_vec u;',
-              IND * 2 + '#pragma unroll',
-              IND * 2 + 'for (int e = 0; e < LV_W; e++) u[e] = lv_w[1 + (DIR > 0 ? m * LV_W + e : CK - 1 - (m * LV_W + e))];',
-              IND * 2 + 'lv_q[m] = u;',
-              IND + '}',
-              IND + 'lv_carry = lv_w[CK];',
-              '}']
-        return s
-
     def partial(c):
         s = ['{',
              IND + f'const IndexT x0 = lv_x0({c});',
@@ -2785,38 +2763,90 @@ def line_vec_kernel(L):
               '}']
         return s
 
-    inner.append('// the partial first chunk')
-    inner.append('if (lv_v > 0)')
-    inner += [IND + ln for ln in partial('(IndexT)0')]
-    inner.append('// full chunks: two or more run pipelined')
-    inner.append('if (lv_cL - lv_cF >= 1) {')
-    body = []
-    body += vload('lv_a', 'lv_ahA', 'lv_cF')
-    body += vload('lv_b', 'lv_ahB', 'lv_cF + 1')
-    body += full('lv_a', 'lv_ahA', 'lv_cF')
-    body.append('IndexT lv_c = lv_cF + 1;   // lv_b holds chunk lv_c while lv_c <= lv_cL')
-    body.append('while (lv_c + 1 <= lv_cL) {')
-    loop = []
-    loop += vload('lv_a', 'lv_ahA', 'lv_c + 1')
-    loop += full('lv_b', 'lv_ahB', 'lv_c')
-    loop.append('// the last pair reloads chunk lv_cL: no branch, its values are not used')
-    loop += vload('lv_b', 'lv_ahB', '(lv_c + 2 <= lv_cL ? lv_c + 2 : lv_cL)')
-    loop += full('lv_a', 'lv_ahA', 'lv_c + 1')
-    loop.append('lv_c += 2;')
-    body += [IND + ln for ln in loop]
-    body.append('}')
-    body.append('if (lv_c == lv_cL)')
-    body += [IND + ln for ln in full('lv_b', 'lv_ahB', 'lv_c')]
-    inner += [IND + ln for ln in body]
-    inner.append('} else if (lv_cL == lv_cF) {')
-    one = vload('lv_a', 'lv_ahA', 'lv_cF') + full('lv_a', 'lv_ahA', 'lv_cF')
-    inner += [IND + ln for ln in one]
+    sync = '__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront"); __builtin_amdgcn_wave_barrier(); __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");'
+
+    def step(c, cur, nxt, full):
+        # chunk c (pieces in `cur`) through LDS and the walk; chunk c + 1 into
+        # `nxt`.  full: every line of the wave exists, so the walk and the
+        # stores need no guard (a guarded store may be skipped, and the wait
+        # for the next chunk's loads would then also wait for the stores)
+        walk_lines = walk('lv_c', False)
+        if full:
+            walk_lines = [IND + ln for ln in walk_lines]
+        else:
+            walk_lines = [IND + 'if (lv_mine) {'] + [IND * 2 + ln for ln in walk_lines] + [IND + '}']
+        store = '*(lv_vec *)(lv_pb[m] + (IndexT)DIR * lv_c * (IndexT)CK + lv_pc * LV_W) = u;'
+        if not full:
+            store = 'if (lv_l0 + (IndexT)(m * 8 + lv_sub) < nlines) ' + store
+        return (['{',
+                 IND + f'const IndexT lv_c = {c};',
+                 IND + f'lv_gload(lv_c + 1 <= lv_w1 ? lv_c + 1 : lv_w1, {nxt});   // the last chunk reloads itself: no branch',
+                 IND + 'if constexpr (LV_AHEAD) lv_ah = lv_ld(lv_x0(lv_c) + (IndexT)CK * (IndexT)DIR);',
+                 IND + '__builtin_amdgcn_sched_barrier(0);   // those loads issue before this chunk\'s wait',
+                 IND + '#pragma unroll',
+                 IND + 'for (int m = 0; m < 8; m++)',
+                 IND * 2 + '#pragma unroll',
+                 IND * 2 + f'for (int e = 0; e < LV_W; e++) lv_buf[(m * 8 + lv_sub) * LV_RS + lv_pc * LV_W + e] = {cur}[m][e];',
+                 IND + sync,
+                 IND + f'{xct} lv_w[CK + 2];',
+                 IND + 'lv_w[0] = lv_carry;',
+                 IND + '#pragma unroll',
+                 IND + 'for (int q = 0; q < CK; q++) lv_w[1 + q] = lv_buf[lv_lane * LV_RS + (DIR > 0 ? q : CK - 1 - q)];',
+                 IND + 'lv_w[CK + 1] = lv_ah;'] +
+                walk_lines + [
+                 IND + 'lv_carry = lv_w[CK];',
+                 IND + '#pragma unroll',
+                 IND + 'for (int q = 0; q < CK; q++) lv_buf[lv_lane * LV_RS + (DIR > 0 ? q : CK - 1 - q)] = lv_w[1 + q];',
+                 IND + sync,
+                 IND + '#pragma unroll',
+                 IND + 'for (int m = 0; m < 8; m++) {',
+                 IND * 2 + 'lv_vec u;',
+                 IND * 2 + '#pragma unroll',
+                 IND * 2 + 'for (int e = 0; e < LV_W; e++) u[e] = lv_buf[(m * 8 + lv_sub) * LV_RS + lv_pc * LV_W + e];',
+                 IND * 2 + store,
+                 IND + '}',
+                 IND + sync,
+                 '}'])
+
+    def pipeline(full):
+        p = ['lv_vec lv_g[8], lv_h[8];',
+             'lv_gload(1, lv_h);']
+        p += step('(IndexT)1', 'lv_h', 'lv_g', full)
+        p += ['IndexT lv_c2 = 2;   // lv_g holds chunk lv_c2 while lv_c2 <= lv_w1',
+              'while (lv_c2 + 1 <= lv_w1) {']
+        p += [IND + ln for ln in step('lv_c2', 'lv_g', 'lv_h', full)]
+        p += [IND + ln for ln in step('lv_c2 + 1', 'lv_h', 'lv_g', full)]
+        p += [IND + 'lv_c2 += 2;', '}', 'if (lv_c2 == lv_w1)']
+        p += [IND + ln for ln in step('lv_c2', 'lv_g', 'lv_h', full)]
+        return p
+
+    inner.append('if (lv_w1 >= 1) {')
+    pipe = ['// this lane\'s pieces: line lv_l0 + m * 8 + lv_sub, 16-byte piece lv_pc of every chunk;',
+            '// lv_pb[m] is that line\'s chunk 0 (aligned), chunk c at lv_pb[m] + DIR * c * CK',
+            f'{xct} *lv_pb[8];',
+            '#pragma unroll',
+            'for (int m = 0; m < 8; m++) {',
+            IND + f'{xct} *const row = lv_rowOf(lv_l0 + (IndexT)(m * 8 + lv_sub));',
+            IND + 'lv_pb[m] = row + lv_kA - lv_pos(row);',
+            '}',
+            'auto lv_gload = [&](IndexT c, lv_vec (&g)[8]) {',
+            IND + '#pragma unroll',
+            IND + 'for (int m = 0; m < 8; m++) g[m] = *(const lv_vec *)(lv_pb[m] + (IndexT)DIR * c * (IndexT)CK + lv_pc * LV_W);',
+            '};',
+            '// the first chunk is partial somewhere (or wholly before the loop\'s first step): lane by lane',
+            'if (lv_mine)']
+    pipe += [IND + ln for ln in partial('(IndexT)0')]
+    pipe.append('if (lv_l0 + 64 <= nlines) {   // every line of the wave exists')
+    pipe += [IND + ln for ln in pipeline(True)]
+    pipe.append('} else {')
+    pipe += [IND + ln for ln in pipeline(False)]
+    pipe.append('}')
+    inner += [IND + ln for ln in pipe]
     inner.append('}')
-    inner.append('// the partial last chunk (unless it is the first one, done above)')
-    inner.append('if (lv_nch - 1 > lv_cL && !(lv_nch == 1 && lv_v > 0)) {')
-    inner.append(IND + 'const IndexT lv_c = lv_nch - 1;')
-    inner += [IND + ln for ln in partial('lv_c')]
-    inner.append('}')
+    inner.append('// chunks not whole on every line of the wave: lane by lane')
+    inner.append('if (lv_mine)')
+    inner.append(IND + 'for (IndexT lv_c = lv_w1 >= 1 ? lv_w1 + 1 : 0; lv_c < lv_nch; lv_c++)')
+    inner += [IND * 2 + ln for ln in partial('lv_c')]
     b += [IND + ln for ln in '\n'.join(inner).split('\n')]
     b.append('}')
     b += red_kernel_tail(L, 'blockIdx.x', accumulate=False)
@@ -2837,10 +2867,9 @@ def line_vec_launch(L):
         args += [f'(int)boundsLow[{i}]', f'(int)cnt{i}', f'(int)loopSteps[{i}]']
     args += ['(int)nlines']
     args += red_fact_args(L)
-    for ck in LINE_VEC_CK:
-        for d in (1, -1):
-            b.append(IND + f'if (CK == {ck} && loopSteps[{r - 1}] == {d}) hipLaunchKernelGGL(({L.name}_lvec_kernel<int, {ck}, {d}>), '
-                     f'dim3((unsigned)nblocks), threads, 0, stream, {", ".join(args)});')
+    for d in (1, -1):
+        b.append(IND + f'if (loopSteps[{r - 1}] == {d}) hipLaunchKernelGGL(({L.name}_lvec_kernel<int, {d}>), '
+                 f'dim3((unsigned)nblocks), threads, 0, stream, {", ".join(args)});')
     b.append(IND + 'dvmh_amd_mark_kernel_C(loop_ref, 1);')
     b.append('}')
     b += handler_reds_finish(L)
@@ -3044,10 +3073,10 @@ def across_handler(L):
             cks = ' || '.join(f'CK == {ck}' for ck in LINE_TILE_CK)
             b.append(IND + f'bool ltile = tune[DVMH_AMD_T_VARIANT] == 1 && (loopSteps[{r - 1}] == 1 || loopSteps[{r - 1}] == -1) && '
                      f'{X}_tfm == 0 && threads.x <= 256 && threads.y == 1 && threads.z == 1 && ({cks}) && ldsBytes <= 64512;')
-            vcks = ' || '.join(f'CK == {ck}' for ck in LINE_VEC_CK)
-            b.append(IND + '// streamed line sweep (variant 3): unit stride along the line, 32-bit indices, no LDS')
+            b.append(IND + '// streamed line sweep (variant 3): unit stride along the line, 32-bit indices, whole waves, no dynamic LDS')
             b.append(IND + f'bool lvec = tune[DVMH_AMD_T_VARIANT] == 3 && (loopSteps[{r - 1}] == 1 || loopSteps[{r - 1}] == -1) && '
-                     f'{X}_tfm == 0 && {X}_devHdr[{r}] == 1 && kernelIndexT == rt_INT && threads.x <= 256 && threads.y == 1 && threads.z == 1 && ({vcks});')
+                     f'{X}_tfm == 0 && {X}_devHdr[{r}] == 1 && kernelIndexT == rt_INT && threads.x <= 256 && threads.x % 64 == 0 && '
+                     f'threads.y == 1 && threads.z == 1 && sizeof({xct}) >= 4 && sizeof({xct}) <= 8;')
             b.append(IND + 'if (lvec) {')
             b += [IND * 2 + s for s in line_vec_launch(L)]
             b.append(IND + '} else if (ltile) {')

@@ -27,8 +27,8 @@ extern "C" {
                                       LDS chunks of TILE_A steps; VARIANT 2, other axes: each
                                       thread's line in register chunks of TILE_A steps;
                                       VARIANT 3, dependent axis innermost and unit stride:
-                                      each thread streams its line in aligned chunks of
-                                      TILE_A elements, 16-byte vector accesses, no LDS) */
+                                      a wave streams 64 lines in aligned 128-byte chunks,
+                                      coalesced, transposed through LDS, a lane per line) */
 #define DVMH_AMD_KIND_WAVEFRONT 2  /* ACROSS with >= 2 dependent axes: hyperplane wavefront */
 #define DVMH_AMD_KIND_STENCIL 3    /* PARALLEL whose handler also has the LDS march variant
                                       (VARIANT 1: TILE_A = points per thread on the innermost

@@ -141,7 +141,8 @@ KIND_LINE = 1
 
 def _ltile(tl, ck, reg=0):
     """reg 0: LDS chunks; 1: the walk on a register copy of the LDS chunk;
-    2: the streamed sweep (variant 3: aligned vector chunks, no LDS)."""
+    2: the streamed sweep (variant 3: a wave's lines in aligned 128-byte
+    chunks through LDS; ck is not used)."""
     tune = [0] * 8
     tune[T_VARIANT] = 3 if reg == 2 else 1
     tune[T_TILE_A] = ck
@@ -155,7 +156,7 @@ LTILES = [(64, 8, 0), (128, 16, 0), (256, 8, 0), (64, 32, 0), (128, 8, 0), (64, 
 
 @pytest.mark.gpu
 @pytest.mark.parametrize('cfg', LTILES)
-@pytest.mark.parametrize('n,steps', [(6, 2), (33, 2), (64, 3)])
+@pytest.mark.parametrize('n,steps', [(6, 2), (33, 2), (64, 3), (100, 2)])
 def test_adi3d_tiled_line_sweep_bit_exact(cfg, n, steps):
     """The two sweeps along k use the tiled variant (the other four fall back
     to the direct line kernel with the same block); n = 6 gives one partial
@@ -169,6 +170,25 @@ def test_adi3d_tiled_line_sweep_bit_exact(cfg, n, steps):
     ref = H.oracle_adi3d(n, steps)
     assert H.bits_equal(res['A'], ref['A'])
     assert np.array_equal(res['max'], ref['max']) and np.array_equal(res['loc'], ref['loc'])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('tl', [64, 256])
+@pytest.mark.parametrize('n,its', [(45, 4), (130, 3), (384, 2)])
+def test_adikat_streamed_line_sweep_bit_exact(tl, n, its):
+    """The reference ADI's k-sweep reads both neighbours (the one ahead not yet
+    updated) and folds a MAX: the streamed sweep with rows of every alignment
+    (n = 45, 130) and whole 128-byte chunks (384), eps of every iteration equal
+    to the oracle's."""
+    block, tune = _ltile(tl, 16, 2)
+    _force(KIND_LINE, block, tune)
+    try:
+        eps = H.run_adikat(n, its, maxeps=0.0)
+    finally:
+        _force(KIND_LINE, None, None)
+    ref = np.zeros(its)
+    H.oracle().orc_adi_kat(n, its, 0, H.vp(ref))
+    assert np.array_equal(eps, ref), (eps, ref)
 
 
 def _lreg(bx, by, ck, xcd=0):
@@ -216,10 +236,10 @@ def test_adi3d_register_line_sweep_full_size():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize('cfg', [(128, 16, 1), (64, 32, 2)])
+@pytest.mark.parametrize('cfg', [(128, 16, 1), (64, 16, 2)])
 def test_adi3d_tiled_line_sweep_full_size(cfg):
     """256^3 (the ADI workload's size), two steps: the walk on a register copy
-    of LDS chunks, and the streamed sweep (aligned vector chunks, no LDS)."""
+    of LDS chunks, and the streamed sweep."""
     block, tune = _ltile(*cfg)
     _force(KIND_LINE, block, tune)
     try:
