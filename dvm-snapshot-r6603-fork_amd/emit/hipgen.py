@@ -2343,18 +2343,22 @@ def tile_kernel(L):
           '// outside those the body reads is clamped into them (its slot is never read)']
     for a, v in enumerate(('dt_ii', 'dt_jj', 'dt_kk')):
         b.append(f'const IndexT {v}_lo = lo{a} + ({rng[a][0]}), {v}_hi = lo{a} + cnt{a} - 1 + ({rng[a][1]});')
-    b += ['for (IndexT dt_e0 = dt_tid; dt_e0 < dt_elems; dt_e0 += 4 * dt_nthr) {',
+    b += ['// slot (p, q, r) of element e = dt_tid + m * dt_nthr, advanced by dt_nthr',
+          '// in the radix (R1, R2) of the tile: two divisions per thread, not per element',
+          'IndexT dt_sp = dt_nthr / dt_R12, dt_sq = (dt_nthr - dt_sp * dt_R12) / dt_R2, dt_sr = dt_nthr - dt_sp * dt_R12 - dt_sq * dt_R2;',
+          'IndexT dt_p = dt_tid / dt_R12, dt_q = (dt_tid - dt_p * dt_R12) / dt_R2, dt_r = dt_tid - dt_p * dt_R12 - dt_q * dt_R2;',
+          'for (IndexT dt_e0 = dt_tid; dt_e0 < dt_elems; dt_e0 += 4 * dt_nthr) {',
           IND + f'{xct} dt_v[4];',
           IND + '#pragma unroll',
           IND + 'for (int u = 0; u < 4; u++) {',
-          IND * 2 + 'IndexT dt_e = dt_e0 + (IndexT)u * dt_nthr;',
-          IND * 2 + 'dt_e = dt_e < dt_elems ? dt_e : dt_elems - 1;',
-          IND * 2 + 'const IndexT dt_p = dt_e / dt_R12, dt_rem = dt_e - (dt_e / dt_R12) * dt_R12;',
-          IND * 2 + 'const IndexT dt_q = dt_rem / dt_R2, dt_r = dt_rem - (dt_rem / dt_R2) * dt_R2;',
+          IND * 2 + '// (past the last element p runs beyond the tile: the clamp keeps the load in bounds)',
           IND * 2 + 'IndexT dt_ii = dt_oi + dt_p, dt_jj = dt_oj + dt_q, dt_kk = dt_ok + dt_r;']
     for v in ('dt_ii', 'dt_jj', 'dt_kk'):
         b.append(IND * 2 + f'{v} = {v} < {v}_lo ? {v}_lo : ({v} > {v}_hi ? {v}_hi : {v});')
     b += [IND * 2 + f'dt_v[u] = {X}_g[dt_ii][dt_jj][dt_kk];',
+          IND * 2 + 'dt_r += dt_sr; dt_q += dt_sq; dt_p += dt_sp;',
+          IND * 2 + 'if (dt_r >= dt_R2) { dt_r -= dt_R2; dt_q++; }',
+          IND * 2 + 'if (dt_q >= dt_R1) { dt_q -= dt_R1; dt_p++; }',
           IND + '}',
           IND + '#pragma unroll',
           IND + 'for (int u = 0; u < 4; u++)',
@@ -2403,12 +2407,17 @@ def tile_kernel(L):
           IND + '}',
           IND + '__syncthreads();',
           '}',
-          '// write back the tile (not the halo)',
+          '// write back the tile (not the halo); (p, q, r) advanced as in the staging',
           'const IndexT dt_nbc = dt_nb * dt_nc;',
-          'for (IndexT dt_e = dt_tid; dt_e < dt_na * dt_nbc; dt_e += dt_nthr) {',
-          IND + 'const IndexT dt_p = dt_e / dt_nbc, dt_rem = dt_e - (dt_e / dt_nbc) * dt_nbc;',
-          IND + 'const IndexT dt_q = dt_rem / dt_nc, dt_r = dt_rem - (dt_rem / dt_nc) * dt_nc;',
-          IND + f'{X}_g[dt_oi + 1 + dt_p][dt_oj + 1 + dt_q][dt_ok + 1 + dt_r] = dt_tile[((dt_p + 1) * dt_R1 + (dt_q + 1)) * dt_R2 + (dt_r + 1)];',
+          '{',
+          IND + 'const IndexT wp = dt_nthr / dt_nbc, wq = (dt_nthr - wp * dt_nbc) / dt_nc, wr = dt_nthr - wp * dt_nbc - wq * dt_nc;',
+          IND + 'IndexT p = dt_tid / dt_nbc, q = (dt_tid - p * dt_nbc) / dt_nc, r = dt_tid - p * dt_nbc - q * dt_nc;',
+          IND + 'for (IndexT dt_e = dt_tid; dt_e < dt_na * dt_nbc; dt_e += dt_nthr) {',
+          IND * 2 + f'{X}_g[dt_oi + 1 + p][dt_oj + 1 + q][dt_ok + 1 + r] = dt_tile[((p + 1) * dt_R1 + (q + 1)) * dt_R2 + (r + 1)];',
+          IND * 2 + 'r += wr; q += wq; p += wp;',
+          IND * 2 + 'if (r >= dt_nc) { r -= dt_nc; q++; }',
+          IND * 2 + 'if (q >= dt_nb) { q -= dt_nb; p++; }',
+          IND + '}',
           '}',
           '__syncthreads();   // the LDS tile is free for the next one']
     k.append(indent('\n'.join(b) + '\n', 1).rstrip('\n'))
