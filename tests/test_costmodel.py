@@ -116,13 +116,21 @@ def test_model_gain_on_held_out_site():
     its best candidate of any kernel variant those miss: a handful of timed
     executions instead of every candidate) is within 5% of the exhaustive
     optimum, beats the default configuration, and beats the same measuring
-    budget spent on random candidates (the method without the model)."""
-    CM.load_records(CM.RECORDS)
-    n, loss = CM.train(epochs=3000, lr=3e-3, seed=1, holdout=CM.HOLDOUT)
-    assert n > 500 and loss < 0.5
-    for site in [s for s in CM.sites() if CM.HOLDOUT in s and s.startswith('jac3d.cdv:')]:
-        ev = CM.evaluate(site, k=4)
-        print(ev)
+    budget spent on random candidates (the method without the model).  A
+    training run is one draw (seeds 1-5 on these records land between 1.000
+    and 1.057 of the optimum): the model of median quality over three seeds
+    is the one checked."""
+    evals = {}
+    for seed in (1, 2, 3):
+        CM.load_records(CM.RECORDS)
+        n, loss = CM.train(epochs=3000, lr=3e-3, seed=seed, holdout=CM.HOLDOUT)
+        assert n > 500 and loss < 0.5
+        for site in [s for s in CM.sites() if CM.HOLDOUT in s and s.startswith('jac3d.cdv:')]:
+            evals.setdefault(site, []).append(CM.evaluate(site, k=4))
+    assert evals
+    for site, evs in evals.items():
+        ev = sorted(evs, key=lambda e: e['model_vs_exhaustive'])[1]
+        print(site, [round(e['model_vs_exhaustive'], 4) for e in evs])
         assert ev['model_vs_exhaustive'] <= 1.05, ev
         assert ev['model_vs_exhaustive'] < ev['random_k_vs_exhaustive'], ev
         assert ev['model_speedup_vs_default'] > 1.0, ev
@@ -131,9 +139,10 @@ def test_model_gain_on_held_out_site():
 @pytest.mark.gpu
 @needs_records
 def test_trial_set_reaches_rare_variant():
-    """The ADI sweeps along the innermost axis at 256^3 have a kernel variant
-    (tiled line sweep) that few records show, so the model ranks it low; the
-    trial set still times its best-ranked candidate, and the site's choice is
+    """The ADI sweeps along the innermost axis at 256^3 have kernel variants
+    (the LDS-chunk and the streamed line sweeps) that few records show, so the
+    model ranks them low; the trial set still times the best-ranked candidate
+    of each, and the site's choice is
     within 10% of its exhaustive optimum (in-sample: these sites are trained on)."""
     CM.load_records(CM.RECORDS)
     CM.train(epochs=3000, lr=3e-3, seed=1, holdout=CM.HOLDOUT)
