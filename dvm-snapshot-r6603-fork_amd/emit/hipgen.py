@@ -386,6 +386,19 @@ def host_handler(L):
 
 
 # ----------------------------------------------------------------- GPU: common pieces
+def scalar_bind(body, p, pre='', ptr=None):
+    """A scalar parameter inside a kernel: a const copy when the body never
+    assigns it or takes its address (loads and what is computed from it are
+    then loop-invariant to the compiler), else a reference to the device copy."""
+    nm = p[1]
+    written = re.search(r'(?<![\w.>])' + nm + r'\s*(?:=(?!=)|\+=|-=|\*=|/=|%=|&=|\|=|\^=|<<=|>>=|\+\+|--)'
+                        r'|(?:\+\+|--)\s*' + nm + r'\b|&\s*' + nm + r'\b', body)
+    src = ptr if ptr else f'{pre}{nm}_ptr'
+    if written:
+        return f'{p[2]} &{nm} = *{src};'
+    return f'const {p[2]} {nm} = *{src};'
+
+
 def kernel_array_params(L, unit):
     fp, setup = [], []
     for p in L.arrays:
@@ -397,7 +410,7 @@ def kernel_array_params(L, unit):
         setup.append(f'const auto {nm} = dvmh_dev::makeView<{ct}, IndexT, {ar}, {"true" if unit else "false"}>({nm}_base, {nm}_str);')
     for p in L.scalars:
         fp.append(f'{p[2]} *__restrict__ {p[1]}_ptr')
-        setup.append(f'{p[2]} &{p[1]} = *{p[1]}_ptr;')
+        setup.append(scalar_bind(L.body, p))
     return fp, setup
 
 
@@ -413,7 +426,7 @@ def kernel_diag_array_params(L):
         setup.append(f'const auto {nm} = dvmh_dev::makeDiagView<{ct}, IndexT, {ar}>({nm}_base, {nm}_str, {nm}_di);')
     for p in L.scalars:
         fp.append(f'{p[2]} *__restrict__ {p[1]}_ptr')
-        setup.append(f'{p[2]} &{p[1]} = *{p[1]}_ptr;')
+        setup.append(scalar_bind(L.body, p))
     return fp, setup
 
 
@@ -714,7 +727,7 @@ def march_kernel(L, tail=None):
         setup.append(f'const auto {nm} = dvmh_dev::makeView<{ct}, IndexT, {ar}, true>({nm}_base, {nm}_str);')
     for p in L.scalars:
         fp.append(f'{p[2]} *__restrict__ {p[1]}_ptr')
-        setup.append(f'{p[2]} &{p[1]} = *{p[1]}_ptr;')
+        setup.append(scalar_bind(L.body, p))
     for i in range(r):
         fp += [f'IndexT lo{i}', f'IndexT cnt{i}', f'IndexT st{i}']
     fp += ['IndexT nbx', 'IndexT nby', 'IndexT nbz', 'IndexT march']
@@ -1560,7 +1573,7 @@ def pair_tail_scope(L1, L2, tname):
         s.append(f'const auto {nm} = dvmh_dev::makeView<{ct}, IndexT, {ar}, true>({pre}{nm}_base, p2_{nm}_str);')
     for p in L2.scalars:
         pre = '' if p[1] in n1 else 'p2_'
-        s.append(f'{p[2]} &{p[1]} = *{pre}{p[1]}_ptr;')
+        s.append(scalar_bind(L2.body, p, pre))
     for red in L2.reds:
         s.append(f'{red[2]} &{red[1]} = p2_{red[1]};')
         if len(red) > 3:
@@ -1754,7 +1767,7 @@ def pair_kernels(L1, L2):
         _, nm, ct, ar = p
         sc.append(f'const auto {nm} = dvmh_dev::makeView<{ct}, IndexT, {ar}, true>(dvmhp_{nm}, dvmhp_{nm}_str);')
     for p in L2.scalars:
-        sc.append(f'{p[2]} &{p[1]} = *dvmhp_{p[1]};')
+        sc.append(scalar_bind(L2.body, p, ptr=f'dvmhp_{p[1]}'))
     for v, ct in L2.vars:
         sc.append(f'{ct} {v};')
     for pr in L2.privates:
@@ -2306,7 +2319,7 @@ def tile_kernel(L):
         setup.append(f'const auto {vn} = dvmh_dev::makeView<{ct}, IndexT, {ar}, true>({nm}_base, {nm}_str);')
     for p in L.scalars:
         fp.append(f'{p[2]} *__restrict__ {p[1]}_ptr')
-        setup.append(f'{p[2]} &{p[1]} = *{p[1]}_ptr;')
+        setup.append(scalar_bind(L.body, p))
     for i in range(3):
         fp += [f'IndexT lo{i}', f'IndexT cnt{i}', f'IndexT st{i}']
     call = []
@@ -2512,7 +2525,7 @@ def line_tile_kernel(L):
         setup.append(f'const auto {nm} = dvmh_dev::makeView<{ct}, IndexT, {ar}, false>({nm}_base, {nm}_str);')
     for p in L.scalars:
         fp.append(f'{p[2]} *__restrict__ {p[1]}_ptr')
-        setup.append(f'{p[2]} &{p[1]} = *{p[1]}_ptr;')
+        setup.append(scalar_bind(L.body, p))
     for i in range(r):
         fp += [f'IndexT lo{i}', f'IndexT cnt{i}', f'IndexT st{i}']
     fp += ['IndexT dl_nlines']
