@@ -2402,7 +2402,7 @@ def tile_kernel(L):
           IND * 3 + 'if (dt_has[u] && dt_r >= 0 && dt_r < dt_nc) {',
           IND * 4 + 'const IndexT t0 = dt_a0 + dt_p, t1 = dt_b0 + dt_q, t2 = dt_c0 + dt_r;']
     for i in range(3):
-        b.append(IND * 4 + var_assign(L, i, f't{i}', f'lo{i}', f'st{i}'))
+        b.append(IND * 4 + var_assign(L, i, f't{i}', f'lo{i}', '(IndexT)1'))   # the handlers run unit forward steps only
     b.append(indent('{\n' + indent(L.body + '\n', 1) + '}', 4))
     b += [IND * 3 + '}',
           IND * 2 + '}',
@@ -2413,7 +2413,7 @@ def tile_kernel(L):
           IND * 3 + 'if (dt_r >= 0 && dt_r < dt_nc) {',
           IND * 4 + 'const IndexT t0 = dt_a0 + dt_p, t1 = dt_b0 + dt_q, t2 = dt_c0 + dt_r;']
     for i in range(3):
-        b.append(IND * 4 + var_assign(L, i, f't{i}', f'lo{i}', f'st{i}'))
+        b.append(IND * 4 + var_assign(L, i, f't{i}', f'lo{i}', '(IndexT)1'))   # the handlers run unit forward steps only
     b.append(indent('{\n' + indent(L.body + '\n', 1) + '}', 4))
     b += [IND * 3 + '}',
           IND * 2 + '}',
