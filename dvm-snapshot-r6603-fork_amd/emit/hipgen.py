@@ -2352,7 +2352,7 @@ def tile_kernel(L):
           f'{xct} *dt_tile = ({xct} *)dvmh_lds;',
           'const IndexT dt_oi = lo0 + dt_a0 - 1, dt_oj = lo1 + dt_b0 - 1, dt_ok = lo2 + dt_c0 - 1;   // array index of tile slot (0, 0, 0)',
           'const IndexT dt_tid = (IndexT)threadIdx.x, dt_nthr = (IndexT)blockDim.x;',
-          '// stage the tile and its halo, four loads in flight per thread; an index',
+          '// stage the tile and its halo, sixteen loads in flight per thread; an index',
           '// outside those the body reads is clamped into them (its slot is never read)']
     for a, v in enumerate(('dt_ii', 'dt_jj', 'dt_kk')):
         b.append(f'const IndexT {v}_lo = lo{a} + ({rng[a][0]}), {v}_hi = lo{a} + cnt{a} - 1 + ({rng[a][1]});')
@@ -2360,10 +2360,12 @@ def tile_kernel(L):
           '// in the radix (R1, R2) of the tile: two divisions per thread, not per element',
           'IndexT dt_sp = dt_nthr / dt_R12, dt_sq = (dt_nthr - dt_sp * dt_R12) / dt_R2, dt_sr = dt_nthr - dt_sp * dt_R12 - dt_sq * dt_R2;',
           'IndexT dt_p = dt_tid / dt_R12, dt_q = (dt_tid - dt_p * dt_R12) / dt_R2, dt_r = dt_tid - dt_p * dt_R12 - dt_q * dt_R2;',
-          'for (IndexT dt_e0 = dt_tid; dt_e0 < dt_elems; dt_e0 += 4 * dt_nthr) {',
-          IND + f'{xct} dt_v[4];',
+          'for (IndexT dt_b0 = 0; dt_b0 < dt_elems; dt_b0 += 16 * dt_nthr) {',
+          IND + 'const IndexT dt_e0 = dt_b0 + dt_tid;',
+          IND + f'{xct} dt_v[16];',
           IND + '#pragma unroll',
-          IND + 'for (int u = 0; u < 4; u++) {',
+          IND + 'for (int u = 0; u < 16; u++) {',
+          IND * 2 + 'if (dt_b0 + (IndexT)u * dt_nthr >= dt_elems) break;   // (uniform) no element of the workgroup left',
           IND * 2 + '// (past the last element p runs beyond the tile: the clamp keeps the load in bounds)',
           IND * 2 + 'IndexT dt_ii = dt_oi + dt_p, dt_jj = dt_oj + dt_q, dt_kk = dt_ok + dt_r;']
     for v in ('dt_ii', 'dt_jj', 'dt_kk'):
@@ -2374,7 +2376,7 @@ def tile_kernel(L):
           IND * 2 + 'if (dt_q >= dt_R1) { dt_q -= dt_R1; dt_p++; }',
           IND + '}',
           IND + '#pragma unroll',
-          IND + 'for (int u = 0; u < 4; u++)',
+          IND + 'for (int u = 0; u < 16; u++)',
           IND * 2 + 'if (dt_e0 + (IndexT)u * dt_nthr < dt_elems) dt_tile[dt_e0 + (IndexT)u * dt_nthr] = dt_v[u];',
           '}',
           '__syncthreads();',
