@@ -282,56 +282,61 @@ static void runQueue() {
         bool pair = false;
         std::vector<Loop *> loops2;   // pair groups: the second loop of each pair
     };
-    std::vector<Group> groups;
-    batchEpoch++;
-    for (size_t k = 0; k < queued.size(); k++) {
-        Loop *l = queued[k];
-        int target = -1;
-        DvmHandlerFunc pf = !l->batchChain && k + 1 < queued.size() ? pairHandlerFor(l, queued[k + 1]) : nullptr;
-        if (pf) {
-            // a pair: joins the last group of its pair handler under the same
-            // rule as a loop (both loops belong to one region)
-            for (int g = (int)groups.size() - 1; g >= 0; g--)
-                if (groups[g].pair && groups[g].func == pf) { target = g; break; }
-            if (target >= 0 && lastGroupOf(l) >= target) target = -1;
-            if (target < 0) {
-                groups.push_back(Group{pf, false, {}, false, true, {}});
-                target = (int)groups.size() - 1;
-            }
-            groups[target].loops.push_back(l);
-            groups[target].loops2.push_back(queued[k + 1]);
-            markGroup(l, target);
-            k++;
-            continue;
-        }
-        if (!l->hasBatch) {
-            // queued as a possible pair member, unpaired: runs on its own
-            groups.push_back(Group{nullptr, false, {l}, false, false, {}});
-            markGroup(l, (int)groups.size() - 1);
-            continue;
-        }
-        if (l->batchChain) {
-            if (!groups.empty() && groups.back().chain && groups.back().func == l->batchH.func) target = (int)groups.size() - 1;
-        } else {
-            for (int g = (int)groups.size() - 1; g >= 0; g--)
-                if (!groups[g].chain && !groups[g].pair && groups[g].func == l->batchH.func) { target = g; break; }
-            // joining group `target` moves l before the groups after it: allowed
-            // when none of target's or the later groups' loops shares an array
-            if (target >= 0 && lastGroupOf(l) >= target) target = -1;
-        }
-        if (target < 0) {
-            groups.push_back(Group{l->batchH.func, l->batchChain, {}, false, false, {}});
-            target = (int)groups.size() - 1;
-        }
-        groups[target].loops.push_back(l);
-        markGroup(l, target);
-    }
     std::vector<Loop *> all;
     all.swap(queued);
     bool settled = settings.graph && queueSettled(all);
-    for (auto &g : groups)
-        g.explore = g.chain && tunerSitePendingExcept(g.loops[0]->file, g.loops[0]->line, DVMH_AMD_KIND_CHAIN) > 0;
+    // the groups are formed only when the queue runs loop by loop or is
+    // captured: a replay of its graph needs none of them
+    std::vector<Group> groups;
+    auto buildGroups = [&]() {
+        batchEpoch++;
+        for (size_t k = 0; k < all.size(); k++) {
+            Loop *l = all[k];
+            int target = -1;
+            DvmHandlerFunc pf = !l->batchChain && k + 1 < all.size() ? pairHandlerFor(l, all[k + 1]) : nullptr;
+            if (pf) {
+                // a pair: joins the last group of its pair handler under the same
+                // rule as a loop (both loops belong to one region)
+                for (int g = (int)groups.size() - 1; g >= 0; g--)
+                    if (groups[g].pair && groups[g].func == pf) { target = g; break; }
+                if (target >= 0 && lastGroupOf(l) >= target) target = -1;
+                if (target < 0) {
+                    groups.push_back(Group{pf, false, {}, false, true, {}});
+                    target = (int)groups.size() - 1;
+                }
+                groups[target].loops.push_back(l);
+                groups[target].loops2.push_back(all[k + 1]);
+                markGroup(l, target);
+                k++;
+                continue;
+            }
+            if (!l->hasBatch) {
+                // queued as a possible pair member, unpaired: runs on its own
+                groups.push_back(Group{nullptr, false, {l}, false, false, {}});
+                markGroup(l, (int)groups.size() - 1);
+                continue;
+            }
+            if (l->batchChain) {
+                if (!groups.empty() && groups.back().chain && groups.back().func == l->batchH.func) target = (int)groups.size() - 1;
+            } else {
+                for (int g = (int)groups.size() - 1; g >= 0; g--)
+                    if (!groups[g].chain && !groups[g].pair && groups[g].func == l->batchH.func) { target = g; break; }
+                // joining group `target` moves l before the groups after it: allowed
+                // when none of target's or the later groups' loops shares an array
+                if (target >= 0 && lastGroupOf(l) >= target) target = -1;
+            }
+            if (target < 0) {
+                groups.push_back(Group{l->batchH.func, l->batchChain, {}, false, false, {}});
+                target = (int)groups.size() - 1;
+            }
+            groups[target].loops.push_back(l);
+            markGroup(l, target);
+        }
+        for (auto &g : groups)
+            g.explore = g.chain && tunerSitePendingExcept(g.loops[0]->file, g.loops[0]->line, DVMH_AMD_KIND_CHAIN) > 0;
+    };
     auto runGroups = [&]() {
+        buildGroups();
         for (auto &g : groups) {
             if (g.pair) {
                 for (Loop *l : g.loops) l->queued = false;

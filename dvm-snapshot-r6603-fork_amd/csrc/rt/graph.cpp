@@ -220,6 +220,7 @@ struct BatchGraph {
     std::vector<std::pair<long, long>> postAt;   // (queue position, reduction index) of each captured post
 };
 static std::unordered_map<size_t, BatchGraph> batchGraphs;
+static BatchGraph *lastBatchGraph = nullptr;   // the entry of the last batch queue (element addresses are stable)
 constexpr size_t MAX_BATCH_GRAPHS = 16;
 
 static void destroyEntry(GraphEntry &g) {
@@ -235,65 +236,82 @@ static void destroyEntry(GraphEntry &g) {
 // device address, layout), reductions (variables, initial bytes) and ACROSS
 // widths.  Scalars are read by the kernels from device memory, so only their
 // addresses enter.
-static std::vector<long> batchSignature(const std::vector<Loop *> &loops) {
-    std::vector<long> w;
-    w.reserve(loops.size() * 32);
-    w.push_back(settings.fuse);   // pairs run fused or not
-    auto bytes = [&w](const std::vector<char> &v) {
-        w.push_back((long)v.size());
+template <class Emit>
+static void signatureWords(const std::vector<Loop *> &loops, Emit &&emit) {
+    emit(settings.fuse);   // pairs run fused or not
+    auto bytes = [&emit](const std::vector<char> &v) {
+        emit((long)v.size());
         for (size_t i = 0; i < v.size(); i += sizeof(long)) {
             long x = 0;
             memcpy(&x, v.data() + i, std::min(sizeof(long), v.size() - i));
-            w.push_back(x);
+            emit(x);
         }
     };
     for (Loop *l : loops) {
-        w.push_back((long)std::hash<std::string>()(l->file));
-        w.push_back(l->line);
-        w.push_back(l->rank);
+        emit((long)std::hash<std::string>()(l->file));
+        emit(l->line);
+        emit(l->rank);
         for (int i = 0; i < l->rank; i++) {
-            w.push_back(l->lo[i]);
-            w.push_back(l->hi[i]);
-            w.push_back(l->step[i]);
+            emit(l->lo[i]);
+            emit(l->hi[i]);
+            emit(l->step[i]);
         }
-        w.push_back((long)l->batchH.func);
-        w.push_back((long)l->gpuH.func);
-        w.push_back(l->batchChain);
-        for (void *p : l->gpuH.params) w.push_back((long)p);
+        emit((long)l->batchH.func);
+        emit((long)l->gpuH.func);
+        emit(l->batchChain);
+        for (void *p : l->gpuH.params) emit((long)p);
         if (l->region) {
             for (auto &ra : l->region->arrays) {
                 const Data *d = ra.data;
-                w.push_back((long)d);
-                w.push_back((long)d->dev.ptr);
-                w.push_back(d->dev.diagState);
-                for (int i = 0; i < d->rank; i++) w.push_back(d->dev.axisPerm[i]);
+                emit((long)d);
+                emit((long)d->dev.ptr);
+                emit(d->dev.diagState);
+                for (int i = 0; i < d->rank; i++) emit(d->dev.axisPerm[i]);
             }
             for (auto &rs : l->region->scalars) {
-                w.push_back((long)rs.data);
-                w.push_back((long)rs.data->dev.ptr);
+                emit((long)rs.data);
+                emit((long)rs.data->dev.ptr);
             }
         }
         for (auto &r : l->reds) {
-            w.push_back(r->func);
-            w.push_back(r->rtType);
-            w.push_back((long)r->arrayAddr);
-            w.push_back((long)r->locAddr);
+            emit(r->func);
+            emit(r->rtType);
+            emit((long)r->arrayAddr);
+            emit((long)r->locAddr);
             bytes(r->initArr);
             bytes(r->initLoc);
         }
         for (auto &a : l->acrosses) {
-            w.push_back((long)a.data);
+            emit((long)a.data);
             for (int i = 0; i < a.data->rank; i++) {
-                w.push_back(a.low[i]);
-                w.push_back(a.high[i]);
+                emit(a.low[i]);
+                emit(a.high[i]);
             }
         }
-        w.push_back(l->userBlock[0]);
-        w.push_back(l->userBlock[1]);
-        w.push_back(l->userBlock[2]);
-        w.push_back(-1);
+        emit(l->userBlock[0]);
+        emit(l->userBlock[1]);
+        emit(l->userBlock[2]);
+        emit(-1);
     }
+}
+
+static std::vector<long> batchSignature(const std::vector<Loop *> &loops) {
+    std::vector<long> w;
+    w.reserve(loops.size() * 32);
+    signatureWords(loops, [&w](long x) { w.push_back(x); });
     return w;
+}
+
+// The queue's signature equals `sig`, compared word by word as it is formed
+// (no vector built, no hash).
+static bool signatureEquals(const std::vector<long> &sig, const std::vector<Loop *> &loops) {
+    size_t k = 0;
+    bool eq = true;
+    signatureWords(loops, [&](long x) {
+        eq = eq && k < sig.size() && sig[k] == x;
+        k++;
+    });
+    return eq && k == sig.size();
 }
 
 static size_t hashWords(const std::vector<long> &w) {
@@ -310,25 +328,34 @@ bool batchGraphRun(const std::vector<Loop *> &loops, bool eligible, const std::f
         runGroups();
         return false;
     }
-    std::vector<long> sig = batchSignature(loops);
-    const size_t h = hashWords(sig);
-    auto it = batchGraphs.find(h);
-    if (it != batchGraphs.end() && it->second.sig != sig) {
-        gpu.sync();
-        destroyEntry(it->second.g);
-        batchGraphs.erase(it);
-        it = batchGraphs.end();
-    }
-    if (it == batchGraphs.end()) {
-        if (batchGraphs.size() >= MAX_BATCH_GRAPHS) {
+    // a program repeats one time step: the entry used last is tried first,
+    // compared word by word as the signature is formed
+    BatchGraph *found = lastBatchGraph && lastBatchGraph->g.exec && signatureEquals(lastBatchGraph->sig, loops) ? lastBatchGraph : nullptr;
+    if (!found) {
+        std::vector<long> sig = batchSignature(loops);
+        const size_t h = hashWords(sig);
+        auto it = batchGraphs.find(h);
+        if (it != batchGraphs.end() && it->second.sig != sig) {
             gpu.sync();
-            for (auto &kv : batchGraphs) destroyEntry(kv.second.g);
-            batchGraphs.clear();
+            if (lastBatchGraph == &it->second) lastBatchGraph = nullptr;
+            destroyEntry(it->second.g);
+            batchGraphs.erase(it);
+            it = batchGraphs.end();
         }
-        it = batchGraphs.emplace(h, BatchGraph()).first;
-        it->second.sig = std::move(sig);
+        if (it == batchGraphs.end()) {
+            if (batchGraphs.size() >= MAX_BATCH_GRAPHS) {
+                gpu.sync();
+                for (auto &kv : batchGraphs) destroyEntry(kv.second.g);
+                batchGraphs.clear();
+                lastBatchGraph = nullptr;
+            }
+            it = batchGraphs.emplace(h, BatchGraph()).first;
+            it->second.sig = std::move(sig);
+        }
+        found = &it->second;
     }
-    BatchGraph &bg = it->second;
+    lastBatchGraph = found;
+    BatchGraph &bg = *found;
     GraphEntry &g = bg.g;
     if (g.exec) {
         HIP_CHECK(hipGraphLaunch(g.exec, gpu.stream));
@@ -391,6 +418,7 @@ void graphReleaseAll() {
     graphs.clear();
     for (auto &kv : batchGraphs) destroyEntry(kv.second.g);
     batchGraphs.clear();
+    lastBatchGraph = nullptr;
 }
 
 }  // namespace dvmh
