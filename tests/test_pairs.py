@@ -288,3 +288,23 @@ def test_pairs_declined_run_loop_by_loop():
         assert pair_stats() == before
     finally:
         r.dvmh_amd_set_index64_C(0)
+
+
+@pytest.mark.parametrize('body,written', [
+    ('B[i][j] = c * A[i][j];', False),
+    ('B[i][j] = (c == 2.0) ? A[i][j] : 0.0;', False),
+    ('cc = 1.0; B[i][j] = cc * c;', False),
+    ('p.c = 1.0; B[i][j] = c;', False),
+    ('c = c + 1.0;', True),
+    ('c += 2.0;', True),
+    ('++c;', True),
+    ('c--;', True),
+    ('f(&c);', True),
+])
+def test_kernel_scalar_binding(body, written):
+    """A scalar parameter the body never assigns (or takes the address of) is
+    bound in kernels as a const copy, loop-invariant to the compiler; one it
+    writes stays a reference to its device copy."""
+    hipgen, _ = _loops()
+    got = hipgen.scalar_bind(body, ('scalar', 'c', 'double'))
+    assert got == ('double &c = *c_ptr;' if written else 'const double c = *c_ptr;'), got
