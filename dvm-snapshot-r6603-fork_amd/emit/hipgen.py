@@ -2832,6 +2832,46 @@ def line_vec_kernel(L):
                  IND + sync,
                  '}'])
 
+    def head_step(full):
+        # chunk 0 through LDS like a whole chunk, its walk guarded to the
+        # loop's steps; elements that are no step go back unchanged
+        walk_lines = walk('(IndexT)0', True)
+        if full:
+            walk_lines = [IND + ln for ln in walk_lines]
+        else:
+            walk_lines = [IND + 'if (lv_mine) {'] + [IND * 2 + ln for ln in walk_lines] + [IND + '}']
+        store = '*(lv_vec *)(lv_pb[m] + lv_pc * LV_W) = u;'
+        if not full:
+            store = 'if (lv_l0 + (IndexT)(m * 8 + lv_sub) < nlines) ' + store
+        return (['{',
+                 IND + 'lv_vec lv_f[8];',
+                 IND + 'lv_gload(0, lv_f);',
+                 IND + 'if constexpr (LV_AHEAD) lv_ah = lv_ld(lv_x0(0) + (IndexT)CK * (IndexT)DIR);',
+                 IND + '#pragma unroll',
+                 IND + 'for (int m = 0; m < 8; m++)',
+                 IND * 2 + '#pragma unroll',
+                 IND * 2 + 'for (int e = 0; e < LV_W; e++) lv_buf[(m * 8 + lv_sub) * LV_RS + lv_pc * LV_W + e] = lv_f[m][e];',
+                 IND + sync,
+                 IND + f'{xct} lv_w[CK + 2];',
+                 IND + 'lv_w[0] = lv_carry;',
+                 IND + '#pragma unroll',
+                 IND + 'for (int q = 0; q < CK; q++) lv_w[1 + q] = lv_buf[lv_lane * LV_RS + (DIR > 0 ? q : CK - 1 - q)];',
+                 IND + 'lv_w[CK + 1] = lv_ah;'] +
+                walk_lines + [
+                 IND + 'lv_carry = lv_w[CK];',
+                 IND + '#pragma unroll',
+                 IND + 'for (int q = 0; q < CK; q++) lv_buf[lv_lane * LV_RS + (DIR > 0 ? q : CK - 1 - q)] = lv_w[1 + q];',
+                 IND + sync,
+                 IND + '#pragma unroll',
+                 IND + 'for (int m = 0; m < 8; m++) {',
+                 IND * 2 + 'lv_vec u;',
+                 IND * 2 + '#pragma unroll',
+                 IND * 2 + 'for (int e = 0; e < LV_W; e++) u[e] = lv_buf[(m * 8 + lv_sub) * LV_RS + lv_pc * LV_W + e];',
+                 IND * 2 + store,
+                 IND + '}',
+                 IND + sync,
+                 '}'])
+
     def pipeline(full):
         p = ['lv_vec lv_g[8], lv_h[8];',
              'lv_gload(1, lv_h);']
@@ -2857,8 +2897,17 @@ def line_vec_kernel(L):
             IND + '#pragma unroll',
             IND + 'for (int m = 0; m < 8; m++) g[m] = *(const lv_vec *)(lv_pb[m] + (IndexT)DIR * c * (IndexT)CK + lv_pc * LV_W);',
             '};',
-            '// the first chunk is partial somewhere (or wholly before the loop\'s first step): lane by lane',
-            'if (lv_mine)']
+            '// the first chunk is partial somewhere (or wholly before the loop\'s first step).  When it',
+            '// lies inside the indices the body reads on every line of the wave, it goes through LDS',
+            '// too (a walk guarded to the loop\'s steps); otherwise lane by lane',
+            'const IndexT lv_lo0 = DIR > 0 ? lv_x0(0) : lv_x0(0) - (IndexT)(CK - 1);',
+            'if (__all(!lv_mine || (lv_lo0 >= lv_rlo && lv_lo0 + (IndexT)(CK - 1) <= lv_rhi))) {',
+            IND + 'if (lv_l0 + 64 <= nlines) {']
+    pipe += [IND * 2 + ln for ln in head_step(True)]
+    pipe += [IND + '} else {']
+    pipe += [IND * 2 + ln for ln in head_step(False)]
+    pipe += [IND + '}',
+             '} else if (lv_mine)']
     pipe += [IND + ln for ln in partial('(IndexT)0')]
     pipe.append('if (lv_l0 + 64 <= nlines) {   // every line of the wave exists')
     pipe += [IND + ln for ln in pipeline(True)]
