@@ -2716,7 +2716,8 @@ def line_vec_kernel(L):
           f'{xct} *const lv_buf = lv_lds + (threadIdx.x >> 6) * 64 * LV_RS;   // this wave\'s rows',
           'const IndexT lv_l0 = (IndexT)blockIdx.x * (IndexT)blockDim.x + (IndexT)(threadIdx.x & ~63u);   // the wave\'s first line',
           f'if (lv_l0 < nlines && cnt{kx} > 0) {{   // uniform over the wave']
-    inner = ['const bool lv_mine = lv_l0 + (IndexT)lv_lane < nlines;',
+    inner = ['const IndexT lv_left = nlines - lv_l0;   // lines from the wave\'s first to the last',
+             'const bool lv_mine = (IndexT)lv_lane < lv_left;',
              '// row of a line (lines past the end read the last one; nothing of theirs is stored)']
     if r == 3:
         rowof = (f'auto lv_rowOf = [&](IndexT ln) -> {xct} * {{ ln = ln < nlines ? ln : nlines - 1; '
@@ -2801,7 +2802,7 @@ def line_vec_kernel(L):
             walk_lines = [IND + 'if (lv_mine) {'] + [IND * 2 + ln for ln in walk_lines] + [IND + '}']
         store = '*(lv_vec *)(lv_pb[m] + (IndexT)DIR * lv_c * (IndexT)CK + lv_pc * LV_W) = u;'
         if not full:
-            store = 'if (lv_l0 + (IndexT)(m * 8 + lv_sub) < nlines) ' + store
+            store = 'if ((IndexT)(m * 8 + lv_sub) < lv_left) ' + store
         return (['{',
                  IND + f'const IndexT lv_c = {c};',
                  IND + f'lv_gload(lv_c + 1 <= lv_w1 ? lv_c + 1 : lv_w1, {nxt});   // the last chunk reloads itself: no branch',
@@ -2842,7 +2843,7 @@ def line_vec_kernel(L):
             walk_lines = [IND + 'if (lv_mine) {'] + [IND * 2 + ln for ln in walk_lines] + [IND + '}']
         store = '*(lv_vec *)(lv_pb[m] + lv_pc * LV_W) = u;'
         if not full:
-            store = 'if (lv_l0 + (IndexT)(m * 8 + lv_sub) < nlines) ' + store
+            store = 'if ((IndexT)(m * 8 + lv_sub) < lv_left) ' + store
         return (['{',
                  IND + 'lv_vec lv_f[8];',
                  IND + 'lv_gload(0, lv_f);',
@@ -2890,7 +2891,7 @@ def line_vec_kernel(L):
             f'{xct} *lv_pb[8];',
             '#pragma unroll',
             'for (int m = 0; m < 8; m++) {',
-            IND + f'{xct} *const row = lv_rowOf(lv_l0 + (IndexT)(m * 8 + lv_sub));',
+            IND + f'{xct} *const row = lv_rowOf((IndexT)(m * 8 + lv_sub) < lv_left ? lv_l0 + (IndexT)(m * 8 + lv_sub) : nlines - 1);',
             IND + 'lv_pb[m] = row + lv_kA - lv_pos(row);',
             '}',
             'auto lv_gload = [&](IndexT c, lv_vec (&g)[8]) {',
@@ -2902,14 +2903,14 @@ def line_vec_kernel(L):
             '// too (a walk guarded to the loop\'s steps); otherwise lane by lane',
             'const IndexT lv_lo0 = DIR > 0 ? lv_x0(0) : lv_x0(0) - (IndexT)(CK - 1);',
             'if (__all(!lv_mine || (lv_lo0 >= lv_rlo && lv_lo0 + (IndexT)(CK - 1) <= lv_rhi))) {',
-            IND + 'if (lv_l0 + 64 <= nlines) {']
+            IND + 'if (lv_left >= 64) {']
     pipe += [IND * 2 + ln for ln in head_step(True)]
     pipe += [IND + '} else {']
     pipe += [IND * 2 + ln for ln in head_step(False)]
     pipe += [IND + '}',
              '} else if (lv_mine)']
     pipe += [IND + ln for ln in partial('(IndexT)0')]
-    pipe.append('if (lv_l0 + 64 <= nlines) {   // every line of the wave exists')
+    pipe.append('if (lv_left >= 64) {   // every line of the wave exists')
     pipe += [IND + ln for ln in pipeline(True)]
     pipe.append('} else {')
     pipe += [IND + ln for ln in pipeline(False)]
