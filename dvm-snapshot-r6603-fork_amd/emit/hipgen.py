@@ -2394,9 +2394,16 @@ def tile_kernel(L):
           IND + 'dt_myp[u] = dt_idx / dt_nb;',
           IND + 'dt_myq[u] = dt_idx - dt_myp[u] * dt_nb;',
           '}',
-          'const bool dt_few = dt_nab <= 4 * dt_nthr;',
+          'const bool dt_few = dt_nab <= 4 * dt_nthr, dt_one = dt_nab <= dt_nthr;',
           'for (IndexT dt_s = 0; dt_s < dt_na + dt_nb + dt_nc - 2; dt_s++) {',
-          IND + 'if (dt_few) {',
+          IND + 'if (dt_one) {   // at most one column per thread',
+          IND * 2 + 'const IndexT dt_p = dt_myp[0], dt_q = dt_myq[0];',
+          IND * 2 + 'const IndexT dt_r = dt_s - dt_p - dt_q;',
+          IND * 2 + 'if (dt_has[0] && dt_r >= 0 && dt_r < dt_nc) {',
+          IND * 3 + 'const IndexT t0 = dt_a0 + dt_p, t1 = dt_b0 + dt_q, t2 = dt_c0 + dt_r;',
+          ] + [IND * 3 + var_assign(L, i, f't{i}', f'lo{i}', '(IndexT)1') for i in range(3)] + [indent('{\n' + indent(L.body + '\n', 1) + '}', 3)] + [
+          IND * 2 + '}',
+          IND + '} else if (dt_few) {',
           IND * 2 + '#pragma unroll',
           IND * 2 + 'for (int u = 0; u < 4; u++) {',
           IND * 3 + 'const IndexT dt_p = dt_myp[u], dt_q = dt_myq[u];',
