@@ -14,17 +14,21 @@ import dvmh_amd as D
 import helpers as H
 
 KIND_WAVEFRONT = 2
+KIND_CHAIN = 5
 
 
 def _force(block, tune):
+    """The same configuration for the loop's own sweeps and for its chains
+    (the chain handler asks its own tuner kind)."""
     r = D.runtime()
     r.dvmh_amd_tuner_force_C.argtypes = [ctypes.c_long, ctypes.c_void_p, ctypes.c_void_p]
-    if block is None:
-        r.dvmh_amd_tuner_force_C(KIND_WAVEFRONT, None, None)
-        return
-    b = np.array(block, dtype=np.int64)
-    t = np.array(tune, dtype=np.int64)
-    r.dvmh_amd_tuner_force_C(KIND_WAVEFRONT, H.vp(b), H.vp(t))
+    for kind in (KIND_WAVEFRONT, KIND_CHAIN):
+        if block is None:
+            r.dvmh_amd_tuner_force_C(kind, None, None)
+            continue
+        b = np.array(block, dtype=np.int64)
+        t = np.array(tune, dtype=np.int64)
+        r.dvmh_amd_tuner_force_C(kind, H.vp(b), H.vp(t))
 
 
 def _tile(nt, ta, tc):
@@ -47,11 +51,12 @@ def test_chain_host_path():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize('cfg', [(64, 4, 8), (256, 8, 8), (256, 8, 16), (512, 16, 16)])
+@pytest.mark.parametrize('cfg', [(64, 4, 8), (256, 8, 8), (256, 8, 16), (512, 16, 16), (256, 12, 16), (128, 16, 16), (32, 16, 8)])
 @pytest.mark.parametrize('n,iters,group', [(5, 3, 3), (23, 7, 3), (23, 6, 6), (41, 9, 4), (70, 5, 5)])
 def test_chain_bit_exact(cfg, n, iters, group):
     """Tiles overhanging the loop, tiles larger than it (n = 5), groups that
-    divide the sweeps and groups that leave a shorter last group."""
+    divide the sweeps and groups that leave a shorter last group; tiles with at
+    most one column per thread, up to four (128 x 16 x 16) and more (32 x 16 x 8)."""
     block, tune = _tile(*cfg)
     _force(block, tune)
     before = D.stats()
@@ -80,9 +85,12 @@ def test_chain_falls_back_without_tiles():
 
 
 @pytest.mark.gpu
-def test_chain_full_size():
-    """320^3 (the SOR workload's size), 6 sweeps in one chain, 8^3 tiles."""
-    block, tune = _tile(256, 8, 8)
+@pytest.mark.parametrize('cfg', [(256, 8, 8), (256, 12, 16)])
+def test_chain_full_size(cfg):
+    """320^3 (the SOR workload's size), 6 sweeps in one chain: 8^3 tiles, and
+    12 x 12 x 16 tiles on 256 threads (one column per thread, the shape the
+    tuner keeps)."""
+    block, tune = _tile(*cfg)
     _force(block, tune)
     try:
         res = H.run_sor3d_chain(320, 6, 6)
