@@ -2481,12 +2481,16 @@ def tile_kernel(L):
               'const IndexT dt_aLo0 = dt_tplane - (dt_ntB - 1) - (dt_ntC - 1);',
               'const IndexT dt_aLo = dt_aLo0 > 0 ? dt_aLo0 : 0;',
               'const IndexT dt_aHi = dt_tplane < dt_ntA - 1 ? dt_tplane : dt_ntA - 1;',
-              'const IndexT dt_ta = dt_aLo + (IndexT)blockIdx.x / dt_ntB;',
-              'const IndexT dt_tb = (IndexT)blockIdx.x - ((IndexT)blockIdx.x / dt_ntB) * dt_ntB;',
-              'const IndexT dt_tc = dt_tplane - dt_ta - dt_tb;',
+              '// workgroups enumerate (a, j) with j over the shorter of the b and c tile',
+              '// axes; the other follows from a + b + c = tile-plane',
+              'const IndexT dt_nq = dt_ntB < dt_ntC ? dt_ntB : dt_ntC;',
+              'const IndexT dt_ta = dt_aLo + (IndexT)blockIdx.x / dt_nq;',
+              'const IndexT dt_j = (IndexT)blockIdx.x - ((IndexT)blockIdx.x / dt_nq) * dt_nq;',
+              'const IndexT dt_tb = dt_ntB <= dt_ntC ? dt_j : dt_tplane - dt_ta - dt_j;',
+              'const IndexT dt_tc = dt_ntB <= dt_ntC ? dt_tplane - dt_ta - dt_j : dt_j;',
               '// this sweep has no tile here (before its first or after its last',
               '// tile-plane, or off the plane): uniform per workgroup',
-              'if (dt_tplane < 0 || dt_ta > dt_aHi || dt_tc < 0 || dt_tc >= dt_ntC) return;']
+              'if (dt_tplane < 0 || dt_ta > dt_aHi || dt_tb < 0 || dt_tb >= dt_ntB || dt_tc < 0 || dt_tc >= dt_ntC) return;']
         for red in L.reds:
             cb.append(f'{red[2]} {red[1]} = {red[1]}_inits[dt_sw];')
             if len(red) > 3:
@@ -3089,7 +3093,7 @@ def chain_handler(L):
           IND * 2 + 'if (aLo < 0) aLo = 0;',
           IND * 2 + 'if (aHi - aLo + 1 > rows) rows = aHi - aLo + 1;',
           IND + '}',
-          IND + 'dim3 grid((unsigned)(rows * ntB), (unsigned)(sHi - sLo + 1), 1);']
+          IND + 'dim3 grid((unsigned)(rows * (ntB < ntC ? ntB : ntC)), (unsigned)(sHi - sLo + 1), 1);']
     for it in ['int', 'long long']:
         args = array_fact_args(L, it)
         for i in range(3):
